@@ -1,0 +1,204 @@
+/*
+ * fcs.h -- C ABI of the MI355X (gfx950) per-cell stiff chemistry integrator.
+ *
+ * This header is the drop-in boundary for the chemistry step of the
+ * OpenFOAM-10 plug-in `FastChemistryModel`.  Every entry point takes plain
+ * pointers and sizes; no C++ or framework types cross the boundary.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   - solver selection `chemistryType { solver OptRodas34|OptRosenbrock34|
+ *     OptSeulex; method FastChemistryModel; }`
+ *       src/odeChemistrySolvers.C:66-72, src/ChemistryModel/FastChemistryModel.H:230
+ *   - Coeffs keys absTol/relTol/maxSteps        src/ODE/OptRodas34/OptRodas34.C:40-43,
+ *                                                src/ODE/OptSeulex/OptSeulex.C:40-46
+ *   - keys jacobian exact|fast, Treact          src/ChemistryModel/FastChemistryModel.C:54-59,66
+ *   - scalar solve(const scalar deltaT)          src/ChemistryModel/FastChemistryModel.H:292,
+ *                                                FastChemistryModel_transientSolve.H:3-104
+ *   - mechanism tables built by readInfo        src/Reaction/OptReaction.C:348-1339
+ *
+ * Threading: a solver handle is used by one host thread at a time.  All
+ * entry points return 0 on success and a negative FCS_E* code on failure;
+ * the library never aborts the process (the reference's FatalError on
+ * maxSteps becomes a per-cell status bit, see FCS_CELL_*).
+ */
+#ifndef FCS_H
+#define FCS_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define FCS_OK              0
+#define FCS_E_ARG          -1   /* invalid argument / size                     */
+#define FCS_E_HIP          -2   /* HIP runtime error (no device, launch fail)   */
+#define FCS_E_UNSUPPORTED  -3   /* mechanism outside the compiled size range    */
+#define FCS_E_NOMEM        -4   /* device or host allocation failed             */
+#define FCS_E_CELLS        -5   /* one or more cells failed (see cellStatus)    */
+
+/* ---- per-cell status bits (cellStatus output) ------------------------- */
+#define FCS_CELL_OK            0
+#define FCS_CELL_SKIPPED       1   /* T0 <= Treact: RR = 0, not integrated      */
+#define FCS_CELL_MAXSTEPS      2   /* maxSteps exceeded (reference: FatalError) */
+#define FCS_CELL_NONFINITE     4   /* NaN/Inf in the state or error norm        */
+
+/* ---- reaction rate forms (fcs_mech_desc.rxn_type) --------------------- */
+#define FCS_RXN_ARRHENIUS   0   /* (ir)reversibleArrhenius                      */
+#define FCS_RXN_THIRD_BODY  1   /* (ir)reversibleThirdBodyArrhenius             */
+#define FCS_RXN_LINDEMANN   2   /* (ir)reversibleArrheniusLindemannFallOff      */
+#define FCS_RXN_TROE        3   /* (ir)reversibleArrheniusTroeFallOff           */
+#define FCS_RXN_PLOG        4   /* (ir)reversiblePlog (log-p interpolation)     */
+
+/* ---- ODE solvers (chemistryType.solver) -------------------------------- */
+#define FCS_SOLVER_RODAS34        0   /* "OptRodas34"       */
+#define FCS_SOLVER_ROSENBROCK34   1   /* "OptRosenbrock34"  */
+#define FCS_SOLVER_SEULEX         2   /* "OptSeulex"        */
+
+#define FCS_JACOBIAN_EXACT  0   /* jacobian exact; (default)                     */
+#define FCS_JACOBIAN_FAST   1   /* jacobian fast;  diagonal dC/dY only           */
+
+#define FCS_MAX_STOICH      3   /* reactant / product slots per reaction         */
+
+/*
+ * Flat mechanism description (species thermo + reactions, file order).
+ * Units follow the reference: W in kg/kmol, concentrations kmol/m^3,
+ * A in (m^3/kmol)^(order-1)/s, Ta in K, pressures in Pa.
+ * Stoichiometry is integer and expanded: "2OH" is two slots holding OH
+ * (src/Reaction/OptReaction.C:99-126).
+ * Falloff reactions carry k_inf in (A,beta,Ta) and k_0 in (A0,beta0,Ta0).
+ * efficiencies is a dense [n_reactions x n_species] row-major table read for
+ * third-body and falloff reactions only (may be NULL when there are none).
+ * troe is [n_reactions x 4] = (alpha, Tsss, Ts, Tss) read for TROE rows.
+ * PLOG rows use plog_n[r] points stored at [r*plog_max_points + k].
+ */
+typedef struct fcs_mech_desc {
+    int32_t n_species;
+    int32_t n_reactions;
+    const double* W;            /* [ns]                                   */
+    const double* T_low;        /* [ns]                                   */
+    const double* T_high;       /* [ns]                                   */
+    const double* T_common;     /* [ns]                                   */
+    const double* coeffs_high;  /* [ns*7] NASA-7, T >= T_common           */
+    const double* coeffs_low;   /* [ns*7] NASA-7, T <  T_common           */
+    const int32_t* rxn_type;    /* [nr] FCS_RXN_*                         */
+    const int32_t* rxn_reversible; /* [nr] 1: reverse rate from Kc         */
+    const int32_t* rxn_n_lhs;   /* [nr] 1..3                              */
+    const int32_t* rxn_n_rhs;   /* [nr] 1..3                              */
+    const int32_t* rxn_lhs;     /* [nr*3] species index per slot          */
+    const int32_t* rxn_rhs;     /* [nr*3]                                 */
+    const double* A;            /* [nr] forward Arrhenius / k_inf         */
+    const double* beta;
+    const double* Ta;
+    const double* A0;           /* [nr] low-pressure k_0 (falloff rows)   */
+    const double* beta0;
+    const double* Ta0;
+    const double* efficiencies; /* [nr*ns]                                */
+    const double* troe;         /* [nr*4]                                 */
+    int32_t plog_max_points;
+    const int32_t* plog_n;      /* [nr]                                   */
+    const double* plog_p;       /* [nr*plog_max_points]                   */
+    const double* plog_A;
+    const double* plog_beta;
+    const double* plog_Ta;
+} fcs_mech_desc;
+
+/* Solver options: the chemistryProperties keys of the reference. */
+typedef struct fcs_solver_opts {
+    int32_t solver;          /* FCS_SOLVER_*                                 */
+    int32_t jacobian;        /* FCS_JACOBIAN_*                               */
+    double  absTol;          /* <Solver>Coeffs.absTol                        */
+    double  relTol;          /* <Solver>Coeffs.relTol                        */
+    int32_t maxSteps;        /* <Solver>Coeffs.maxSteps (reference default 10000) */
+    double  Treact;          /* cells with T0 <= Treact are skipped (default 0)   */
+    double  deltaTChemMax;   /* maxChemicalTimeStep (OpenFOAM default: great)     */
+} fcs_solver_opts;
+
+typedef struct fcs_mech   fcs_mech;     /* device-resident mechanism tables   */
+typedef struct fcs_solver fcs_solver;   /* mechanism + options + device state */
+
+/* Library / device information. */
+int fcs_device_count(int* count);
+const char* fcs_version(void);
+
+/* Mechanism: replaces OptReaction::readInfo (src/Reaction/OptReaction.C:348). */
+int fcs_mech_create(const fcs_mech_desc* desc, fcs_mech** out);
+int fcs_mech_destroy(fcs_mech* mech);
+
+/* Solver: replaces the chemistrySolver construction selected by
+ * makeChemistrySolver (src/odeChemistrySolvers.C:66-72). */
+int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts,
+                      fcs_solver** out);
+int fcs_solver_destroy(fcs_solver* solver);
+
+/*
+ * fcs_solve: FastChemistryModel::solve(const scalar deltaT), serial branch
+ * (src/ChemistryModel/FastChemistryModel_transientSolve.H:3-104) over host
+ * arrays.  Fields are per species, one pointer each (OpenFOAM stores one
+ * volScalarField per species):
+ *   T0, p0, rho0  [nCells]        old-time temperature, pressure, density
+ *   Y0[i]         [nCells]        old-time mass fractions, i < n_species
+ *   deltaTChem    [nCells] in/out per-cell chemical sub-step
+ *   RR[i]         [nCells] out    (Y - Y0) * rho0 / deltaT, 0 for skipped cells
+ *   deltaTMin     out             min(min over integrated cells of deltaTChem,
+ *                                     2*deltaT)
+ *   cellStatus    [nCells] out    FCS_CELL_* bits (may be NULL)
+ * Returns FCS_E_CELLS when any cell reports MAXSTEPS or NONFINITE.
+ */
+int fcs_solve(fcs_solver* solver, int64_t nCells, double deltaT,
+              const double* T0, const double* p0, const double* rho0,
+              const double* const* Y0, double* deltaTChem,
+              double* const* RR, double* deltaTMin, int32_t* cellStatus);
+
+/* Same contract, also returning the integrated end state T (the reference
+ * keeps it local) and the number of Rodas/Rosenbrock attempts or Seulex
+ * columns per cell (diagnostics; any output may be NULL). */
+int fcs_solve_ex(fcs_solver* solver, int64_t nCells, double deltaT,
+                 const double* T0, const double* p0, const double* rho0,
+                 const double* const* Y0, double* deltaTChem,
+                 double* const* RR, double* deltaTMin, int32_t* cellStatus,
+                 double* Tend, int32_t* nAttempts);
+
+/*
+ * Device-resident batch: inputs uploaded once, integrated in place in HBM.
+ * Layout in HBM is structure-of-arrays, species-major: Y[i*nCells + c].
+ */
+typedef struct fcs_batch fcs_batch;
+int fcs_batch_create(fcs_solver* solver, int64_t nCells, fcs_batch** out);
+int fcs_batch_destroy(fcs_batch* batch);
+int fcs_batch_upload(fcs_batch* batch, const double* T0, const double* p0,
+                     const double* rho0, const double* Y0 /* [ns*nCells] */,
+                     const double* deltaTChem);
+/* Runs the integration of every cell of the batch over deltaT on the
+ * batch's stream, restarting from the uploaded state each call.  When
+ * elapsed_ms is non-NULL the call waits and returns the time between HIP
+ * events recorded on that stream around the kernel launches. */
+int fcs_batch_run(fcs_batch* batch, double deltaT, float* elapsed_ms);
+int fcs_batch_download(fcs_batch* batch, double* RR /* [ns*nCells] */,
+                       double* deltaTChem, double* deltaTMin,
+                       int32_t* cellStatus, double* Tend, int32_t* nAttempts);
+
+/*
+ * Kernel-level entry points over host arrays (batch of independent states,
+ * species-major SoA, n = n_species + 1).  They run the same device code the
+ * integrator runs and exist for parity checks and profiling.
+ */
+/* FastChemistryModel::derivatives (src/ChemistryModel/FastChemistryModel.C:194-273):
+ * YT [n*nStates] (Y clipped >= 0 on output) -> dYTdt [n*nStates] */
+int fcs_eval_derivatives(fcs_solver* solver, int64_t nStates, const double* p,
+                         double* YT, double* dYTdt);
+/* FastChemistryModel::jacobian (src/ChemistryModel/FastChemistryModel.C:489-572):
+ * YT [n*nStates] -> f(y) [n*nStates] and Jy row-major [nStates][n][n] */
+int fcs_eval_jacobian(fcs_solver* solver, int64_t nStates, const double* p,
+                      double* YT, double* dYTdt, double* Jy);
+/* LUsolver::Block4LUDecompose + xSolve (src/LUsolver/LUsolver.C:3058-3171):
+ * A [nSys][n][n] row-major, b [nSys][n] -> x in b.  n = n_species + 1. */
+int fcs_lu_solve(fcs_solver* solver, int64_t nSys, const double* A, double* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FCS_H */

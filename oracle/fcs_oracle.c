@@ -1,0 +1,1119 @@
+/*
+ * fcs_oracle.c -- scalar FP64 restatement of the reference's per-cell stiff
+ * chemistry integration (FastChemistryModel + OptRodas34 / OptRosenbrock34 /
+ * OptSeulex + LUsolver + OptReaction kernels).
+ *
+ * TEST INFRASTRUCTURE: the checker for the gfx950 kernels, never the product.
+ * Reference citations are paths relative to the reference tree.
+ *
+ * Pinning status of each function (what checks it besides the GPU):
+ *   thermo ............ closed-form NASA-7 identities (dH/dT = cp, d(-G/RT)/dT
+ *                       = H/(RT^2)) in tests/test_oracle_thermo_rates.py
+ *   rates ............. Arrhenius closed form, Troe/Lindemann limits, PLOG at
+ *                       tabulated pressures, detailed balance (q = 0 at an
+ *                       equilibrium built from Kc), element conservation
+ *   derivatives ....... sum dY/dt = 0, enthalpy conservation
+ *   jacobian .......... central finite differences of the restated RHS;
+ *                       exact dense conversion == rank-1 identity
+ *   LU ................ golden vectors from the reference's own LUsolver.C
+ *                       compiled as-is (oracle/_ref), numpy residuals
+ *   Rodas34/Ros34 ..... tableau order conditions on y' = lambda y (stability
+ *                       function vs exp to O(z^5)/O(z^4)), accuracy vs an
+ *                       independent scipy Radau integration; controller step
+ *                       ratios checked against the documented bounds.  The
+ *                       exact accept/reject sequence is parity unpinned (the
+ *                       steppers need OpenFOAM to build).
+ *   Seulex ............ accuracy vs scipy Radau; order/step sequence parity
+ *                       unpinned for the same reason.
+ */
+#include "fcs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* src/Reaction/OptReaction.H:404,409 */
+#define RU   8314.470066505449722171761095523834228515625
+#define PSTD 100000.0
+/* Kc floor: src/Reaction/ReactionFunctionI.H:17, JacobianFunctionI.H:1153 */
+#define KC_FLOOR 1.4901171103413047e-8
+/* Troe clamp of the 4-wide rate/Jacobian paths: src/Reaction/ReactionRate.C:475 */
+#define TROE_SMALL 2.2e-16
+/* zero-pivot replacement: src/LUsolver/LUsolver.C:153-156 */
+#define PIVOT_EPS 2.2204460492503131e-16
+/* OpenFOAM-10 scalar constants used by the steppers (not in the reference
+ * tree; values of OpenFOAM's doubleScalar.H) */
+#define OF_SMALL  1.0e-15
+#define OF_GREAT  1.0e15
+#define OF_VGREAT 1.0e300
+
+struct orc_mech {
+    int ns, nr, n;
+    double *W, *invW, *Tc, *clo, *chi;
+    double TlowMin, ThighMax, TcMin, TcMax;
+    int32_t *type, *rev, *nl, *nrh, *lhs, *rhs;
+    double *A, *beta, *Ta, *A0, *beta0, *Ta0, *eff, *troe;
+    int pmax;
+    int32_t* plog_n;
+    double *plog_p, *plog_lnA, *plog_beta, *plog_Ta;
+};
+
+static double* dup_d(const double* s, size_t n)
+{
+    double* d = (double*)malloc(n * sizeof(double) + 8);
+    if (s) memcpy(d, s, n * sizeof(double)); else memset(d, 0, n * sizeof(double));
+    return d;
+}
+static int32_t* dup_i(const int32_t* s, size_t n)
+{
+    int32_t* d = (int32_t*)malloc(n * sizeof(int32_t) + 8);
+    if (s) memcpy(d, s, n * sizeof(int32_t)); else memset(d, 0, n * sizeof(int32_t));
+    return d;
+}
+
+orc_mech* orc_mech_create(const fcs_mech_desc* d)
+{
+    orc_mech* m = (orc_mech*)calloc(1, sizeof(orc_mech));
+    int ns = d->n_species, nr = d->n_reactions;
+    m->ns = ns; m->nr = nr; m->n = ns + 1;
+    m->W = dup_d(d->W, ns);
+    m->invW = dup_d(NULL, ns);
+    for (int i = 0; i < ns; ++i) m->invW[i] = 1.0 / m->W[i];
+    m->Tc = dup_d(d->T_common, ns);
+    m->clo = dup_d(d->coeffs_low, 7 * (size_t)ns);
+    m->chi = dup_d(d->coeffs_high, 7 * (size_t)ns);
+    /* T clamp range and Tcommon range: src/Reaction/OptReaction.C:1261-1269 */
+    m->TlowMin = d->T_low[0]; m->ThighMax = d->T_high[0];
+    m->TcMin = d->T_common[0]; m->TcMax = d->T_common[0];
+    for (int i = 0; i < ns; ++i) {
+        if (d->T_low[i] < m->TlowMin) m->TlowMin = d->T_low[i];
+        if (d->T_high[i] > m->ThighMax) m->ThighMax = d->T_high[i];
+        if (d->T_common[i] < m->TcMin) m->TcMin = d->T_common[i];
+        if (d->T_common[i] > m->TcMax) m->TcMax = d->T_common[i];
+    }
+    m->type = dup_i(d->rxn_type, nr);
+    m->rev = dup_i(d->rxn_reversible, nr);
+    m->nl = dup_i(d->rxn_n_lhs, nr);
+    m->nrh = dup_i(d->rxn_n_rhs, nr);
+    m->lhs = dup_i(d->rxn_lhs, 3 * (size_t)nr);
+    m->rhs = dup_i(d->rxn_rhs, 3 * (size_t)nr);
+    m->A = dup_d(d->A, nr); m->beta = dup_d(d->beta, nr); m->Ta = dup_d(d->Ta, nr);
+    m->A0 = dup_d(d->A0, nr); m->beta0 = dup_d(d->beta0, nr); m->Ta0 = dup_d(d->Ta0, nr);
+    m->eff = dup_d(d->efficiencies, (size_t)nr * ns);
+    m->troe = dup_d(d->troe, 4 * (size_t)nr);
+    m->pmax = d->plog_max_points > 0 ? d->plog_max_points : 1;
+    m->plog_n = dup_i(d->plog_n, nr);
+    size_t np = (size_t)nr * m->pmax;
+    m->plog_p = dup_d(d->plog_p, np);
+    m->plog_lnA = dup_d(d->plog_A, np);
+    m->plog_beta = dup_d(d->plog_beta, np);
+    m->plog_Ta = dup_d(d->plog_Ta, np);
+    for (size_t k = 0; k < np; ++k)
+        m->plog_lnA[k] = m->plog_lnA[k] > 0 ? log(m->plog_lnA[k]) : 0.0;
+    return m;
+}
+
+void orc_mech_destroy(orc_mech* m)
+{
+    if (!m) return;
+    free(m->W); free(m->invW); free(m->Tc); free(m->clo); free(m->chi);
+    free(m->type); free(m->rev); free(m->nl); free(m->nrh); free(m->lhs); free(m->rhs);
+    free(m->A); free(m->beta); free(m->Ta); free(m->A0); free(m->beta0); free(m->Ta0);
+    free(m->eff); free(m->troe); free(m->plog_n); free(m->plog_p); free(m->plog_lnA);
+    free(m->plog_beta); free(m->plog_Ta);
+    free(m);
+}
+
+int orc_n_species(const orc_mech* m) { return m->ns; }
+
+static double clampT(const orc_mech* m, double T)
+{
+    /* src/Reaction/ReactionRate.C:14-15 */
+    T = T < m->TlowMin ? m->TlowMin : T;
+    T = T > m->ThighMax ? m->ThighMax : T;
+    return T;
+}
+
+/* JANAF coefficient choice: src/Reaction/ThermoFunctionI.H:1-31 */
+static const double* coeffs_at(const orc_mech* m, int i, double T)
+{
+    if (T < m->TcMin) return m->clo + 7 * i;
+    if (T > m->TcMax) return m->chi + 7 * i;
+    return (T < m->Tc[i]) ? m->clo + 7 * i : m->chi + 7 * i;
+}
+
+/* Mass-based NASA-7 thermo: src/Reaction/ThermoFunctionI.H:33-123 */
+void orc_thermo(const orc_mech* m, double T, double* cp, double* ha,
+                double* negG, double* dBdT, double* dcpdT)
+{
+    const double lnT = log(T), iT = 1.0 / T;
+    for (int i = 0; i < m->ns; ++i) {
+        const double* a = coeffs_at(m, i, T);
+        const double r = RU * m->invW[i];
+        if (cp) cp[i] = ((((a[4] * T + a[3]) * T + a[2]) * T + a[1]) * T + a[0]) * r;
+        if (ha) ha[i] = (((((a[4] * 0.2 * T + a[3] * 0.25) * T + a[2] * (1.0 / 3.0)) * T
+                          + a[1] * 0.5) * T + a[0]) * T + a[5]) * r;
+        if (negG) negG[i] = (((a[4] * 0.05 * T + a[3] * (1.0 / 12.0)) * T + a[2] * (1.0 / 6.0)) * T
+                             + a[1] * 0.5) * T - a[5] * iT + a[6] + (lnT - 1.0) * a[0];
+        if (dBdT) dBdT[i] = ((a[4] * 0.2 * T + a[3] * 0.25) * T + a[2] * (1.0 / 3.0)) * T
+                            + a[1] * 0.5 + (a[5] * iT + a[0]) * iT;
+        if (dcpdT) dcpdT[i] = (((4.0 * a[4] * T + 3.0 * a[3]) * T + 2.0 * a[2]) * T + a[1]) * r;
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * Rate constants of one reaction.  Returns kf (including M for third-body
+ * and falloff forms).  With jac != 0 it also returns dkf/dT and, for
+ * third-body / falloff forms, dkf/dM (the reference's dKfdC_).
+ *   Arrhenius / third body:  src/Reaction/ReactionRate.C:84-355, Jacobian.C:108-422
+ *   Lindemann:               ReactionRate.C:356-453, Jacobian.C:425-573
+ *   Troe (4-wide path):      ReactionRate.C:459-504, Jacobian.C:575-677
+ *   PLOG:                    ReactionRate.C:577-672, Jacobian.C:798-910
+ * ---------------------------------------------------------------------- */
+static double rate_k(const orc_mech* m, int r, double T, double lnT, double p,
+                     const double* C, int jac, double* dkfdT, double* dkfdM)
+{
+    const double iT = 1.0 / T;
+    const int t = m->type[r];
+    *dkfdT = 0.0; *dkfdM = 0.0;
+    if (t == FCS_RXN_PLOG) {
+        const int np = m->plog_n[r];
+        const double* P = m->plog_p + (size_t)r * m->pmax;
+        const double* la = m->plog_lnA + (size_t)r * m->pmax;
+        const double* b = m->plog_beta + (size_t)r * m->pmax;
+        const double* ta = m->plog_Ta + (size_t)r * m->pmax;
+        double lk, dlk;
+        if (p <= P[0]) {
+            lk = la[0] + b[0] * lnT - ta[0] * iT;
+            dlk = (b[0] + ta[0] * iT) * iT;
+        } else if (p >= P[np - 1]) {
+            lk = la[np - 1] + b[np - 1] * lnT - ta[np - 1] * iT;
+            dlk = (b[np - 1] + ta[np - 1] * iT) * iT;
+        } else {
+            int j = 0;
+            for (int q = 0; q < np - 1; ++q)
+                if (P[q] <= p && p <= P[q + 1]) { j = q; break; }
+            const double f = (log(p) - log(P[j])) / (log(P[j + 1]) - log(P[j]));
+            lk = la[j] + b[j] * lnT - ta[j] * iT
+               + (la[j + 1] - la[j] + (b[j + 1] - b[j]) * lnT - (ta[j + 1] - ta[j]) * iT) * f;
+            dlk = iT * (b[j] + ta[j] * iT + (b[j + 1] - b[j] + (ta[j + 1] - ta[j]) * iT) * f);
+        }
+        const double k = exp(lk);
+        *dkfdT = k * dlk;
+        return k;
+    }
+    const double k = m->A[r] * exp(m->beta[r] * lnT - m->Ta[r] * iT);
+    const double dk = k * (m->beta[r] + m->Ta[r] * iT) * iT;
+    if (t == FCS_RXN_ARRHENIUS) { *dkfdT = dk; return k; }
+    double M = 0.0;
+    const double* e = m->eff + (size_t)r * m->ns;
+    for (int j = 0; j < m->ns; ++j) M += e[j] * C[j];
+    if (t == FCS_RXN_THIRD_BODY) {
+        *dkfdM = k;
+        *dkfdT = dk * M;
+        return k * M;
+    }
+    /* falloff: (A,beta,Ta) = k_inf, (A0,beta0,Ta0) = k_0 */
+    const double kinf = k, dkinf = dk;
+    const double k0 = m->A0[r] * exp(m->beta0[r] * lnT - m->Ta0[r] * iT);
+    const double dk0 = k0 * (m->beta0[r] + m->Ta0[r] * iT) * iT;
+    if (t == FCS_RXN_LINDEMANN) {
+        const double Pr = k0 * M / kinf;
+        const double tmp = 1.0 / (1.0 + Pr);
+        if (jac) {
+            const double dPrdT = (M * dk0 - Pr * dkinf) / kinf;
+            *dkfdT = tmp * Pr * dkinf + tmp * tmp * dPrdT * kinf;
+            *dkfdM = k0 * tmp * tmp;
+        }
+        return M * tmp * k0;
+    }
+    /* Troe */
+    const double* tr = m->troe + 4 * (size_t)r;
+    const double alpha = tr[0], Tsss = tr[1], Ts = tr[2], Tss = tr[3];
+    const double e1 = exp(-T / Tsss), e2 = exp(-Tss * iT), e3 = exp(-T / Ts);
+    const double Fcent = (1.0 - alpha) * e1 + alpha * e3 + e2;
+    const double logFc = log10(Fcent > TROE_SMALL ? Fcent : TROE_SMALL);
+    const double c = -0.4 - 0.67 * logFc;
+    const double nn = 0.75 - 1.27 * logFc;
+    if (!jac) {
+        double Pr = k0 * M / kinf;
+        Pr = Pr > TROE_SMALL ? Pr : TROE_SMALL;
+        const double logPr = log10(Pr);
+        const double x1 = nn - 0.14 * (logPr + c);
+        const double x2 = (logPr + c) / x1;
+        const double F = pow(10.0, logFc / (1.0 + x2 * x2));
+        return M * (k0 * F / (1.0 + Pr));
+    }
+    /* Jacobian path: Pr += 1e-100, log uses max(Pr, small) (Jacobian.C:591-596) */
+    const double Pr0 = M * k0 / kinf;
+    const int prOk = Pr0 >= TROE_SMALL;
+    const double Pr = Pr0 + 1e-100;
+    const double logPr = log10(Pr > TROE_SMALL ? Pr : TROE_SMALL);
+    const double x1 = nn - 0.14 * (logPr + c);
+    const double x2 = (logPr + c) / x1;
+    const double x3 = 1.0 + x2 * x2;
+    const double x4 = logFc / x3;
+    const double F = pow(10.0, x4);
+    const double ln10 = log(10.0);
+    const double dFcdT = (alpha - 1.0) / Tsss * e1 - alpha / Ts * e3 + e2 * Tss * iT * iT;
+    const double dlogFcdT = (Fcent >= TROE_SMALL) ? dFcdT / Fcent / ln10 : 0.0;
+    const double dcdT = -0.67 * dlogFcdT, dndT = -1.27 * dlogFcdT;
+    const double dx1dT = dndT - 0.14 * dcdT;
+    const double dx2dT = (dcdT - x2 * dx1dT) / x1;
+    const double dx3dT = 2.0 * x2 * dx2dT;
+    const double dx4dT = (dlogFcdT - x4 * dx3dT) / x3;
+    double dFdT = ln10 * F * dx4dT;
+    const double dlogPrdPr = prOk ? 1.0 / (Pr * ln10) : 0.0;
+    const double dx1dPr = -0.14 * dlogPrdPr;
+    const double dx2dPr = (dlogPrdPr - x2 * dx1dPr) / x1;
+    const double dx3dPr = 2.0 * x2 * dx2dPr;
+    const double dx4dPr = -x4 * dx3dPr / x3;
+    const double dFdPr = ln10 * F * dx4dPr;
+    const double dPrdT = (M * dk0 - Pr * dkinf) / kinf;
+    dFdT += dFdPr * dPrdT;
+    const double i1 = 1.0 / (1.0 + Pr);
+    *dkfdT = F * i1 * (Pr * dkinf) + kinf * F * i1 * i1 * dPrdT + k0 * i1 * dFdT * M;
+    *dkfdM = k0 * i1 * (F * i1 + Pr * dFdPr);
+    return (F * k0 * i1) * M;
+}
+
+/* Equilibrium constant in concentration units and d ln Kc / dT:
+ * src/Reaction/ReactionFunctionI.H:1-52, ChemicalEquilibriumFunctionI.H:14-25,
+ * JacobianFunctionI.H:1150-1158 */
+static double kc_of(const orc_mech* m, int r, const double* expG, const double* dBdT,
+                    double T, double* dlnKc)
+{
+    double num = 1.0, den = 1.0, sdb = 0.0;
+    for (int s = 0; s < m->nrh[r]; ++s) { int k = m->rhs[3 * r + s]; num *= expG[k]; if (dBdT) sdb += dBdT[k]; }
+    for (int s = 0; s < m->nl[r]; ++s) { int k = m->lhs[3 * r + s]; den *= expG[k]; if (dBdT) sdb -= dBdT[k]; }
+    const int dn = m->nrh[r] - m->nl[r];
+    const double PbyRT = PSTD / (RU * T);
+    double pw = 1.0;
+    if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
+    if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
+    if (dlnKc) *dlnKc = sdb - dn / T;
+    return (num / den) * pw;
+}
+
+void orc_rates(const orc_mech* m, double p, double Tin, const double* C, double* omega)
+{
+    const int ns = m->ns;
+    const double T = clampT(m, Tin), lnT = log(T);
+    double* negG = (double*)alloca(ns * sizeof(double));
+    double* expG = (double*)alloca(ns * sizeof(double));
+    orc_thermo(m, T, NULL, NULL, negG, NULL, NULL);
+    for (int i = 0; i < ns; ++i) expG[i] = exp(negG[i]);
+    for (int i = 0; i < ns; ++i) omega[i] = 0.0;
+    for (int r = 0; r < m->nr; ++r) {
+        double dk, dm;
+        const double kf = rate_k(m, r, T, lnT, p, C, 0, &dk, &dm);
+        double kr = 0.0;
+        if (m->rev[r]) {
+            double Kc = kc_of(m, r, expG, NULL, T, NULL);
+            Kc = Kc > KC_FLOOR ? Kc : KC_FLOOR;
+            kr = kf / Kc;
+        }
+        double CF = 1.0, CR = 1.0;
+        for (int s = 0; s < m->nl[r]; ++s) CF *= C[m->lhs[3 * r + s]];
+        for (int s = 0; s < m->nrh[r]; ++s) CR *= C[m->rhs[3 * r + s]];
+        const double q = kf * CF - kr * CR;
+        for (int s = 0; s < m->nl[r]; ++s) omega[m->lhs[3 * r + s]] -= q;
+        for (int s = 0; s < m->nrh[r]; ++s) omega[m->rhs[3 * r + s]] += q;
+    }
+}
+
+/* src/ChemistryModel/FastChemistryModel.C:340-417 (6-argument derivatives) */
+void orc_derivatives(const orc_mech* m, double p, double* YT, double* dYTdt)
+{
+    const int ns = m->ns;
+    const double T = YT[ns];
+    for (int i = 0; i < ns; ++i) YT[i] = YT[i] > 0.0 ? YT[i] : 0.0;
+    /* density with the unclamped T (quirk Q3) */
+    const double RuTByP = RU * T / p;
+    double s = 0.0;
+    for (int i = 0; i < ns; ++i) s += YT[i] * m->invW[i] * RuTByP;
+    const double invrho = s, rho = 1.0 / s;
+    double* C = (double*)alloca(ns * sizeof(double));
+    double* cp = (double*)alloca(ns * sizeof(double));
+    double* ha = (double*)alloca(ns * sizeof(double));
+    for (int i = 0; i < ns; ++i) C[i] = rho * m->invW[i] * YT[i];
+    orc_rates(m, p, T, C, dYTdt);
+    orc_thermo(m, clampT(m, T), cp, ha, NULL, NULL, NULL);
+    double cpM = 0.0, dTdt = 0.0;
+    for (int i = 0; i < ns; ++i) {
+        dYTdt[i] = dYTdt[i] * (m->W[i] * invrho);
+        cpM += YT[i] * cp[i];
+        dTdt -= ha[i] * dYTdt[i];
+    }
+    dYTdt[ns] = dTdt / cpM;
+}
+
+/* src/ChemistryModel/FastChemistryModel.C:489-572 -> OptReaction::ddNdtByVdcTp
+ * (src/Reaction/Jacobian.C:6-1001) and the dC/dY conversions
+ * (exact: Jacobian.C:2814-2857 ddYdtdY, :3709-3740 ddYdtdTP, :3743-3791
+ *  ddTdtdYT; fast: FastddYdtdY_Vec :4570-4841). */
+void orc_jacobian(const orc_mech* m, double p, double* YT, double* dYTdt,
+                  double* Jy, int jac, double* JcOut)
+{
+    const int ns = m->ns, n = m->n;
+    for (int i = 0; i < ns; ++i) YT[i] = YT[i] > 0.0 ? YT[i] : 0.0;
+    const double T = clampT(m, YT[ns]);
+    const double lnT = log(T), iT = 1.0 / T;
+    double* cp = (double*)alloca(ns * sizeof(double));
+    double* ha = (double*)alloca(ns * sizeof(double));
+    double* negG = (double*)alloca(ns * sizeof(double));
+    double* expG = (double*)alloca(ns * sizeof(double));
+    double* dBdT = (double*)alloca(ns * sizeof(double));
+    double* dcpdT = (double*)alloca(ns * sizeof(double));
+    double* C = (double*)alloca(ns * sizeof(double));
+    double* om = (double*)alloca(ns * sizeof(double));
+    double* Jc = JcOut ? JcOut : (double*)malloc((size_t)n * n * sizeof(double));
+    orc_thermo(m, T, cp, ha, negG, dBdT, dcpdT);
+    /* JacobianThermo: density with the clamped T (ThermoFunctionI.H:57-289) */
+    double MWinv = 0.0, s = 0.0;
+    const double RuTByP = RU * T / p;
+    for (int i = 0; i < ns; ++i) {
+        s += RuTByP * (YT[i] * m->invW[i]);
+        MWinv += YT[i] * m->invW[i];
+    }
+    const double MW = 1.0 / MWinv, invrho = s, rho = 1.0 / s;
+    double cpM = 0.0, dcpMdT = 0.0;
+    for (int i = 0; i < ns; ++i) {
+        C[i] = rho * YT[i] * m->invW[i];
+        cpM += YT[i] * cp[i];
+        dcpMdT += YT[i] * dcpdT[i];
+        expG[i] = exp(negG[i]);
+        om[i] = 0.0;
+    }
+    memset(Jc, 0, (size_t)n * n * sizeof(double));
+    for (int r = 0; r < m->nr; ++r) {
+        double dkfdT, dkfdM;
+        const double kf = rate_k(m, r, T, lnT, p, C, 1, &dkfdT, &dkfdM);
+        double kr = 0.0, dkrdT = 0.0, invKc = 0.0;
+        if (m->rev[r]) {
+            double dlnKc;
+            double Kc = kc_of(m, r, expG, dBdT, T, &dlnKc);
+            const int floored = !(Kc > KC_FLOOR);
+            Kc = floored ? KC_FLOOR : Kc;
+            invKc = 1.0 / Kc;
+            kr = kf * invKc;
+            dkrdT = dkfdT * invKc - (floored ? 0.0 : kr * dlnKc);
+        }
+        const int nl = m->nl[r], nrr = m->nrh[r];
+        const int32_t* L = m->lhs + 3 * r;
+        const int32_t* R = m->rhs + 3 * r;
+        double CF = 1.0, CR = 1.0;
+        for (int a = 0; a < nl; ++a) CF *= C[L[a]];
+        for (int a = 0; a < nrr; ++a) CR *= C[R[a]];
+        const double q = kf * CF - kr * CR;
+        const double dqdT = dkfdT * CF - dkrdT * CR;
+        for (int a = 0; a < nl; ++a) { om[L[a]] -= q; Jc[L[a] * n + ns] -= dqdT; }
+        for (int a = 0; a < nrr; ++a) { om[R[a]] += q; Jc[R[a] * n + ns] += dqdT; }
+        /* d(CF)/dC: one term per reactant slot (repeated species add up) */
+        for (int a = 0; a < nl; ++a) {
+            double d = kf;
+            for (int b = 0; b < nl; ++b) if (b != a) d *= C[L[b]];
+            const int col = L[a];
+            for (int b = 0; b < nl; ++b) Jc[L[b] * n + col] -= d;
+            for (int b = 0; b < nrr; ++b) Jc[R[b] * n + col] += d;
+        }
+        if (m->rev[r]) {
+            for (int a = 0; a < nrr; ++a) {
+                double d = kr;
+                for (int b = 0; b < nrr; ++b) if (b != a) d *= C[R[b]];
+                const int col = R[a];
+                for (int b = 0; b < nl; ++b) Jc[L[b] * n + col] += d;
+                for (int b = 0; b < nrr; ++b) Jc[R[b] * n + col] -= d;
+            }
+        }
+        const int t = m->type[r];
+        if (t == FCS_RXN_THIRD_BODY || t == FCS_RXN_LINDEMANN || t == FCS_RXN_TROE) {
+            const double dkrdM = dkfdM * invKc;
+            const double w = dkfdM * CF - dkrdM * CR;
+            const double* e = m->eff + (size_t)r * ns;
+            for (int j = 0; j < ns; ++j) {
+                const double v = w * e[j];
+                for (int b = 0; b < nl; ++b) Jc[L[b] * n + j] -= v;
+                for (int b = 0; b < nrr; ++b) Jc[R[b] * n + j] += v;
+            }
+        }
+    }
+    /* conversions */
+    double* Yd = (double*)alloca(ns * sizeof(double));
+    for (int i = 0; i < ns; ++i) Yd[i] = om[i] * (m->W[i] * invrho);
+    if (jac == FCS_JACOBIAN_EXACT) {
+        double* dCdY = (double*)alloca(ns * sizeof(double));
+        for (int j = 0; j < ns; ++j) {
+            const double rmv = MW * m->invW[j];
+            for (int k = 0; k < ns; ++k)
+                dCdY[k] = rho * m->invW[k] * ((k == j ? 1.0 : 0.0) - rmv * YT[k]);
+            for (int i = 0; i < ns; ++i) {
+                double acc = 0.0;
+                for (int k = 0; k < ns; ++k) acc += Jc[i * n + k] * dCdY[k];
+                Jy[i * n + j] = (m->W[i] * invrho) * acc + rmv * Yd[i];
+            }
+        }
+    } else {
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j < ns; ++j)
+                Jy[i * n + j] = (m->W[i] * invrho) * (Jc[i * n + j] * (rho * m->invW[j]))
+                              + (MW * m->invW[j]) * Yd[i];
+    }
+    for (int i = 0; i < ns; ++i) {
+        double acc = Jc[i * n + ns];
+        for (int j = 0; j < ns; ++j) acc -= Jc[i * n + j] * C[j] * iT;
+        Jy[i * n + ns] = (m->W[i] * invrho) * acc + iT * Yd[i];
+        dYTdt[i] = Yd[i];
+    }
+    double dTdt = 0.0;
+    for (int i = 0; i < ns; ++i) dTdt -= Yd[i] * ha[i];
+    dTdt /= cpM;
+    double JTT = 0.0;
+    for (int i = 0; i < ns; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < ns; ++j) acc -= Jy[j * n + i] * ha[j];
+        acc -= cp[i] * dTdt;
+        Jy[ns * n + i] = acc / cpM;
+        JTT -= Yd[i] * cp[i] + Jy[i * n + ns] * ha[i];
+    }
+    JTT -= dTdt * dcpMdT;
+    Jy[ns * n + ns] = JTT / cpM;
+    dYTdt[ns] = dTdt;
+    if (!JcOut) free(Jc);
+}
+
+/* ------------------------------------------------------------------------
+ * LU with pivot search restricted to each 4-row diagonal block
+ * (src/LUsolver/LUsolver.C:106-246 full blocks, :503-551 2-row tail,
+ *  :553-650 3-row tail, :3058-3141 driver).  Right-looking elimination; the
+ * row interchanges and pivot comparisons follow the reference exactly,
+ * including the signed running maximum in the second column of a block
+ * (LUsolver.C:177,183) and the unguarded last pivot of a 1-row tail.
+ * ---------------------------------------------------------------------- */
+static void swap_rows(int n, double* a, int r1, int r2)
+{
+    if (r1 == r2) return;
+    for (int j = 0; j < n; ++j) {
+        double t = a[r1 * n + j]; a[r1 * n + j] = a[r2 * n + j]; a[r2 * n + j] = t;
+    }
+}
+
+static void eliminate(int n, double* a, int k, double* invD, int guard)
+{
+    if (guard && a[k * n + k] == 0.0) a[k * n + k] = PIVOT_EPS;
+    const double inv = 1.0 / a[k * n + k];
+    invD[k] = inv;
+    for (int i = k + 1; i < n; ++i) {
+        const double l = a[i * n + k] * inv;
+        a[i * n + k] = l;
+        for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
+    }
+}
+
+void orc_lu_factor(int n, double* a, int32_t* piv, double* invD)
+{
+    for (int i = 0; i < n; ++i) piv[i] = i;
+    const int rem = n % 4, full = n - rem;
+    for (int k0 = 0; k0 < full; k0 += 4) {
+        /* column k0: first strict maximum of |a| over the 4 block rows */
+        int iMax = 0; double temp = 0.0;
+        for (int d = 0; d < 4; ++d) {
+            const double v = fabs(a[(k0 + d) * n + k0]);
+            if (temp < v) { iMax = d; temp = v; }
+        }
+        if (iMax) { swap_rows(n, a, k0, k0 + iMax); piv[k0] = k0 + iMax; }
+        eliminate(n, a, k0, invD, 1);
+        /* column k0+1: running maximum whose update stores a signed value */
+        {
+            const int k = k0 + 1;
+            double t = fabs(a[(k0 + 1) * n + k]);
+            int im = 1;
+            if (t < fabs(a[(k0 + 2) * n + k])) { im = 2; t = a[(k0 + 2) * n + k]; }
+            if (t < fabs(a[(k0 + 3) * n + k])) { im = 3; t = a[(k0 + 3) * n + k]; }
+            if (im != 1) { swap_rows(n, a, k, k0 + im); piv[k] = k0 + im; }
+            eliminate(n, a, k, invD, 1);
+        }
+        /* column k0+2 */
+        {
+            const int k = k0 + 2;
+            if (fabs(a[k * n + k]) < fabs(a[(k + 1) * n + k])) { swap_rows(n, a, k, k + 1); piv[k] = k + 1; }
+            eliminate(n, a, k, invD, 1);
+        }
+        eliminate(n, a, k0 + 3, invD, 1);
+    }
+    if (rem == 1) {
+        invD[n - 1] = 1.0 / a[n * n - 1];
+    } else if (rem == 2) {
+        const int k = n - 2;
+        if (!(fabs(a[k * n + k]) >= fabs(a[(k + 1) * n + k]))) { swap_rows(n, a, k, k + 1); piv[k] = k + 1; }
+        eliminate(n, a, k, invD, 1);
+        eliminate(n, a, k + 1, invD, 1);
+    } else if (rem == 3) {
+        const int k0 = n - 3;
+        int iMax = 0; double temp = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double v = fabs(a[(k0 + d) * n + k0]);
+            if (temp < v) { iMax = d; temp = v; }
+        }
+        if (iMax) { swap_rows(n, a, k0, k0 + iMax); piv[k0] = k0 + iMax; }
+        eliminate(n, a, k0, invD, 1);
+        if (fabs(a[(k0 + 1) * n + k0 + 1]) < fabs(a[(k0 + 2) * n + k0 + 1])) {
+            swap_rows(n, a, k0 + 1, k0 + 2); piv[k0 + 1] = k0 + 2;
+        }
+        eliminate(n, a, k0 + 1, invD, 1);
+        eliminate(n, a, k0 + 2, invD, 1);
+    }
+}
+
+/* src/LUsolver/LUsolver.C:3922-4111 (xSolve_Vec_*): swaps, unit-L forward,
+ * U backward scaled by invD */
+void orc_lu_solve(int n, const double* a, const int32_t* piv, const double* invD, double* b)
+{
+    for (int j = 0; j < n; ++j) {
+        const int t = piv[j];
+        if (t != j) { double x = b[t]; b[t] = b[j]; b[j] = x; }
+    }
+    for (int i = 1; i < n; ++i) {
+        double s = b[i];
+        for (int j = 0; j < i; ++j) s -= a[i * n + j] * b[j];
+        b[i] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int j = i + 1; j < n; ++j) s -= a[i * n + j] * b[j];
+        b[i] = s * invD[i];
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * Steppers
+ * ---------------------------------------------------------------------- */
+typedef struct work {
+    int n;
+    double *y0, *yT, *k1, *k2, *k3, *k4, *k5, *dy, *err, *dydx, *dfdx, *a, *J;
+    int32_t* piv;
+    double* invD;
+    /* Seulex */
+    double *yS0, *ySeq, *scale, *table, *dyS;
+} work;
+
+static void work_init(work* w, int n)
+{
+    w->n = n;
+    double* b = (double*)calloc((size_t)(16 + 13) * n + 2 * (size_t)n * n, sizeof(double));
+    double** v[] = {&w->y0, &w->yT, &w->k1, &w->k2, &w->k3, &w->k4, &w->k5, &w->dy, &w->err,
+                    &w->dydx, &w->dfdx, &w->invD, &w->yS0, &w->ySeq, &w->scale, &w->dyS};
+    for (int i = 0; i < 16; ++i) *v[i] = b + (size_t)i * n;
+    w->table = b + 16 * (size_t)n;
+    w->a = b + 29 * (size_t)n;
+    w->J = w->a + (size_t)n * n;
+    w->piv = (int32_t*)calloc(n, sizeof(int32_t));
+}
+static void work_free(work* w) { free(w->y0); free(w->piv); }
+
+/* Controller constants: src/ODE/OptRodas34/OptRodas34.H:105-112 */
+static const double SAFE = 0.9, ALPHA_INC = 0.2, ALPHA_DEC = 0.25, MIN_SCALE = 0.2, MAX_SCALE = 10.0;
+
+static double err_norm(int n, const double* y0, const double* y, const double* e,
+                       double absTol, double relTol)
+{
+    /* src/ODE/OptRodas34/OptRodas34.C:822-877 */
+    double mx = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double a0 = fabs(y0[i]), a1 = fabs(y[i]);
+        const double tol = absTol + relTol * (a1 > a0 ? a1 : a0);
+        const double r = fabs(e[i]) / tol;
+        mx = r > mx ? r : mx;
+    }
+    return mx;
+}
+
+/* Rodas3(4) coefficients: src/ODE/OptRodas34/OptRodas34.H:116-149 */
+/* stage times c2 = 0.386, c3 = 0.21, c4 = 0.63 do not enter: the system is autonomous */
+static const double R_d1 = 0.25, R_d2 = -0.1043, R_d3 = 0.1035, R_d4 = -0.3620000000000023e-01;
+static const double R_a21 = 0.1544e1, R_a31 = 0.9466785280815826, R_a32 = 0.2557011698983284;
+static const double R_a41 = 0.3314825187068521e1, R_a42 = 0.2896124015972201e1, R_a43 = 0.9986419139977817;
+static const double R_a51 = 0.1221224509226641e1, R_a52 = 0.6019134481288629e1, R_a53 = 0.1253708332932087e2,
+                    R_a54 = -0.6878860361058950;
+static const double R_c21 = -0.56688e1, R_c31 = -0.2430093356833875e1, R_c32 = -0.2063599157091915;
+static const double R_c41 = -0.1073529058151375, R_c42 = -0.9594562251023355e1, R_c43 = -0.2047028614809616e2;
+static const double R_c51 = 0.7496443313967647e1, R_c52 = -0.1024680431464352e2, R_c53 = -0.3399990352819905e2,
+                    R_c54 = 0.1170890893206160e2;
+static const double R_c61 = 0.8083246795921522e1, R_c62 = -0.7981132988064893e1, R_c63 = -0.3152159432874371e2,
+                    R_c64 = 0.1631930543123136e2, R_c65 = -0.6058818238834054e1;
+static const double R_gamma = 0.25;
+
+static void form_matrix(int n, double* a, const double* J, double diag)
+{
+    for (int i = 0; i < n * n; ++i) a[i] = -J[i];
+    for (int i = 0; i < n; ++i) a[i * n + i] += diag;
+}
+
+/* src/ODE/OptRodas34/OptRodas34.C:291-879 */
+static double rodas34_attempt(const orc_mech* m, const fcs_solver_opts* o, double p,
+                              double dx, work* w)
+{
+    const int n = w->n;
+    const double invdx = 1.0 / dx;
+    double *y0 = w->y0, *yT = w->yT, *k1 = w->k1, *k2 = w->k2, *k3 = w->k3, *k4 = w->k4,
+           *k5 = w->k5, *dy = w->dy, *err = w->err, *dydx = w->dydx, *dfdx = w->dfdx;
+    orc_jacobian(m, p, y0, dfdx, w->J, o->jacobian, NULL);
+    form_matrix(n, w->a, w->J, 1.0 * invdx * (1.0 / R_gamma));
+    orc_lu_factor(n, w->a, w->piv, w->invD);
+    const double dxd1 = dx * R_d1 + 1;
+    for (int i = 0; i < n; ++i) k1[i] = dxd1 * dfdx[i];
+    orc_lu_solve(n, w->a, w->piv, w->invD, k1);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i] + R_a21 * k1[i];
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double e = dx * R_d2, c1 = R_c21 * invdx;
+        for (int i = 0; i < n; ++i) k2[i] = dydx[i] + e * dfdx[i] + c1 * k1[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k2);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i] + R_a31 * k1[i] + R_a32 * k2[i];
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double e = dx * R_d3, c1 = R_c31 * invdx, c2 = R_c32 * invdx;
+        for (int i = 0; i < n; ++i) k3[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k3);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i] + R_a41 * k1[i] + R_a42 * k2[i] + R_a43 * k3[i];
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double e = dx * R_d4, c1 = R_c41 * invdx, c2 = R_c42 * invdx, c3 = R_c43 * invdx;
+        for (int i = 0; i < n; ++i) k4[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k4);
+    for (int i = 0; i < n; ++i) {
+        dy[i] = R_a51 * k1[i] + R_a52 * k2[i] + R_a53 * k3[i] + R_a54 * k4[i];
+        yT[i] = y0[i] + dy[i];
+    }
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double c1 = R_c51 * invdx, c2 = R_c52 * invdx, c3 = R_c53 * invdx, c4 = R_c54 * invdx;
+        for (int i = 0; i < n; ++i) k5[i] = dydx[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i] + c4 * k4[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k5);
+    for (int i = 0; i < n; ++i) { dy[i] = k5[i] + dy[i]; yT[i] = y0[i] + dy[i]; }
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double c1 = R_c61 * invdx, c2 = R_c62 * invdx, c3 = R_c63 * invdx, c4 = R_c64 * invdx,
+                     c5 = R_c65 * invdx;
+        for (int i = 0; i < n; ++i)
+            err[i] = dydx[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i] + c4 * k4[i] + c5 * k5[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, err);
+    for (int i = 0; i < n; ++i) yT[i] = (y0[i] + dy[i]) + err[i];
+    return err_norm(n, y0, yT, err, o->absTol, o->relTol);
+}
+
+/* ROS3(4) coefficients: src/ODE/OptRosenbrock34/OptRosenbrock34.H:127-158 */
+static const double S_a21 = 2.0, S_a31 = 48.0 / 25.0, S_a32 = 6.0 / 25.0;
+static const double S_c21 = -8.0, S_c31 = 372.0 / 25.0, S_c32 = 12.0 / 5.0;
+static const double S_c41 = -112.0 / 125.0, S_c42 = -54.0 / 125.0, S_c43 = -2.0 / 5.0;
+static const double S_b1 = 19.0 / 9.0, S_b2 = 1.0 / 2.0, S_b3 = 25.0 / 108.0, S_b4 = 125.0 / 108.0;
+static const double S_e1 = 34.0 / 108.0, S_e2 = 7.0 / 36.0, S_e4 = 125.0 / 108.0;
+static const double S_gamma = 0.5;
+static const double S_d1 = 1.0 / 2.0, S_d2 = -3.0 / 2.0, S_d3 = 605.0 / 250.0, S_d4 = 29.0 / 250.0;
+
+/* src/ODE/OptRosenbrock34/OptRosenbrock34.C:281-677 */
+static double ros34_attempt(const orc_mech* m, const fcs_solver_opts* o, double p,
+                            double dx, work* w)
+{
+    const int n = w->n;
+    const double invdx = 1.0 / dx;
+    double *y0 = w->y0, *yT = w->yT, *k1 = w->k1, *k2 = w->k2, *k3 = w->k3, *k4 = w->k4,
+           *err = w->err, *dydx = w->dydx, *dfdx = w->dfdx;
+    orc_jacobian(m, p, y0, dfdx, w->J, o->jacobian, NULL);
+    form_matrix(n, w->a, w->J, invdx * (1.0 / S_gamma));
+    orc_lu_factor(n, w->a, w->piv, w->invD);
+    const double dxd1 = dx * S_d1 + 1;
+    for (int i = 0; i < n; ++i) k1[i] = dxd1 * dfdx[i];
+    orc_lu_solve(n, w->a, w->piv, w->invD, k1);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i] + S_a21 * k1[i];
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double e = dx * S_d2, c1 = S_c21 * invdx;
+        for (int i = 0; i < n; ++i) k2[i] = dydx[i] + e * dfdx[i] + c1 * k1[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k2);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i] + S_a31 * k1[i] + S_a32 * k2[i];
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double e = dx * S_d3, c1 = S_c31 * invdx, c2 = S_c32 * invdx;
+        for (int i = 0; i < n; ++i) k3[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k3);
+    {
+        const double e = dx * S_d4, c1 = S_c41 * invdx, c2 = S_c42 * invdx, c3 = S_c43 * invdx;
+        for (int i = 0; i < n; ++i) k4[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k4);
+    for (int i = 0; i < n; ++i) {
+        yT[i] = y0[i] + S_b1 * k1[i] + S_b2 * k2[i] + S_b3 * k3[i] + S_b4 * k4[i];
+        err[i] = S_e1 * k1[i] + S_e2 * k2[i] + S_e4 * k4[i];
+    }
+    return err_norm(n, y0, yT, err, o->absTol, o->relTol);
+}
+
+static void logrec(orc_step_rec* log, int32_t cap, int32_t* len, double x, double dx,
+                   double err, int acc, int order)
+{
+    if (!log || !len || *len >= cap) return;
+    orc_step_rec* r = log + (*len)++;
+    r->x = x; r->dx = dx; r->err = err; r->accepted = acc; r->order = order;
+}
+
+/* ODESolve + adaptiveSolve: src/ODE/OptRodas34/OptRodas34.C:123-286 */
+static int rosenbrock_integrate(const orc_mech* m, const fcs_solver_opts* o, double p,
+                                double xEnd, double* dxTryIO, work* w, int32_t* nAtt,
+                                orc_step_rec* log, int32_t cap, int32_t* len)
+{
+    const int n = w->n;
+    const double xStart = 0.0;
+    const double ratio = pow(MAX_SCALE / SAFE, -1.0 / ALPHA_INC);
+    const double safeMax = SAFE * MAX_SCALE;
+    double x = xStart, dxTry = *dxTryIO;
+    int last = 0;
+    for (int nStep = 0; nStep < o->maxSteps; ++nStep) {
+        const double dxTry0 = dxTry;
+        if ((x + dxTry - xEnd) * (x + dxTry - xStart) > 0) { last = 1; dxTry = xEnd - x; }
+        double dx = dxTry;
+        double Err = (o->solver == FCS_SOLVER_RODAS34) ? rodas34_attempt(m, o, p, dx, w)
+                                                       : ros34_attempt(m, o, p, dx, w);
+        ++*nAtt;
+        if (!(Err == Err)) return FCS_CELL_NONFINITE;
+        while (Err > 1) {
+            logrec(log, cap, len, x, dx, Err, 0, 0);
+            const double sc = SAFE * pow(Err, -ALPHA_DEC);
+            dx *= sc > MIN_SCALE ? sc : MIN_SCALE;
+            Err = (o->solver == FCS_SOLVER_RODAS34) ? rodas34_attempt(m, o, p, dx, w)
+                                                    : ros34_attempt(m, o, p, dx, w);
+            ++*nAtt;
+            if (!(Err == Err)) return FCS_CELL_NONFINITE;
+            if (*nAtt > 100 * o->maxSteps) return FCS_CELL_MAXSTEPS;
+        }
+        logrec(log, cap, len, x, dx, Err, 1, 0);
+        x += dx;
+        for (int i = 0; i < n; ++i) w->y0[i] = w->yT[i];
+        if (Err > ratio) {
+            double f = SAFE * pow(Err, -ALPHA_INC);
+            f = f > MIN_SCALE ? f : MIN_SCALE;
+            f = f < MAX_SCALE ? f : MAX_SCALE;
+            dxTry = f * dx;
+        } else {
+            dxTry = safeMax * dx;
+        }
+        if ((x - xEnd) * (xEnd - xStart) >= 0) {
+            if (nStep > 0 && last) dxTry = dxTry0;
+            *dxTryIO = dxTry;
+            return 0;
+        }
+    }
+    return FCS_CELL_MAXSTEPS;
+}
+
+/* ---- Seulex: src/ODE/OptSeulex/OptSeulex.C:33-731 ---------------------- */
+#define SX_KMAX 12
+#define SX_IMAX 13
+typedef struct seulex_state {
+    int nSeq[SX_IMAX];
+    double cpu[SX_IMAX], coeff[SX_IMAX][SX_IMAX];
+    double jacRedo, theta, logTol;
+    int kTarg;
+    double dxOpt[SX_IMAX], temp[SX_IMAX];
+} seulex_state;
+
+static void seulex_init(seulex_state* s, const fcs_solver_opts* o)
+{
+    const double cpuFunc = 1, cpuJac = 5, cpuLU = 1, cpuSolve = 1;
+    s->nSeq[0] = 2; s->nSeq[1] = 3;
+    for (int i = 2; i < SX_IMAX; ++i) s->nSeq[i] = 2 * s->nSeq[i - 2];
+    s->cpu[0] = cpuJac + cpuLU + s->nSeq[0] * (cpuFunc + cpuSolve);
+    for (int k = 0; k < SX_KMAX; ++k)
+        s->cpu[k + 1] = s->cpu[k] + (s->nSeq[k + 1] - 1) * (cpuFunc + cpuSolve) + cpuLU;
+    for (int k = 0; k < SX_IMAX; ++k)
+        for (int l = 0; l < k; ++l) s->coeff[k][l] = 1.0 / ((double)s->nSeq[k] / s->nSeq[l] - 1.0);
+    s->jacRedo = o->relTol < 1e-4 ? o->relTol : 1e-4;
+    s->theta = 2 * s->jacRedo;
+    s->logTol = -log10(o->relTol + o->absTol) * 0.6 + 0.5;
+    s->kTarg = 0;
+}
+
+static int seul(const orc_mech* m, double p, seulex_state* s,
+                double dxTot, int k, work* w)
+{
+    const int n = w->n;
+    const int nSteps = s->nSeq[k];
+    const double invdx = nSteps / dxTot;
+    double *y0 = w->yS0, *yT = w->yT, *dy = w->dyS, *dydx = w->dydx, *scale = w->scale;
+    form_matrix(n, w->a, w->J, invdx);
+    orc_lu_factor(n, w->a, w->piv, w->invD);
+    orc_derivatives(m, p, y0, dy);
+    orc_lu_solve(n, w->a, w->piv, w->invD, dy);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i];
+    for (int nn = 1; nn < nSteps; ++nn) {
+        for (int i = 0; i < n; ++i) yT[i] += dy[i];
+        if (nn == 1 && k <= 1) {
+            double dy1 = 0.0;
+            for (int i = 0; i < n; ++i) dy1 += (dy[i] / scale[i]) * (dy[i] / scale[i]);
+            dy1 = sqrt(dy1);
+            orc_derivatives(m, p, yT, dydx);
+            for (int i = 0; i < n; ++i) dy[i] = dydx[i] - dy[i] * invdx;
+            orc_lu_solve(n, w->a, w->piv, w->invD, dy);
+            double dy2 = 0.0;
+            for (int i = 0; i < n; ++i) {
+                if (fabs(dy[i]) > sqrt(OF_VGREAT)) { dy2 = OF_VGREAT; break; }
+                dy2 += (dy[i] / scale[i]) * (dy[i] / scale[i]);
+            }
+            dy2 = sqrt(dy2);
+            const double den = 1.0 > dy1 + OF_SMALL ? 1.0 : dy1 + OF_SMALL;
+            s->theta = dy2 / den;
+            if (s->theta > 1.0) return 0;
+        }
+        orc_derivatives(m, p, yT, dy);
+        orc_lu_solve(n, w->a, w->piv, w->invD, dy);
+    }
+    for (int i = 0; i < n; ++i) w->ySeq[i] = yT[i] + dy[i];
+    return 1;
+}
+
+static void extrapolate(const seulex_state* s, int n, int k, double* table, double* y)
+{
+    for (int j = k - 1; j > 0; --j)
+        for (int i = 0; i < n; ++i)
+            table[(j - 1) * n + i] = table[j * n + i] + s->coeff[k][j] * (table[j * n + i] - table[(j - 1) * n + i]);
+    for (int i = 0; i < n; ++i) y[i] = table[i] + s->coeff[k][0] * (table[i] - y[i]);
+}
+
+typedef struct sx_step { double dxTry, dxDid; int first, last, reject, prevReject; } sx_step;
+
+static int seulex_step(const orc_mech* m, const fcs_solver_opts* o, double p, seulex_state* s,
+                       double* x, sx_step* st, work* w, int32_t* nAtt, int* kOut)
+{
+    const int n = w->n;
+    const double kF1 = 0.7, kF2 = 0.9, sF1 = 0.8, sF2 = 0.93, sF3 = 0.1, sF4 = 4, sF5 = 0.5;
+    double* y = w->y0;
+    s->temp[0] = OF_GREAT;
+    double dx = st->dxTry;
+    for (int i = 0; i < n; ++i) w->yS0[i] = y[i];
+    s->dxOpt[0] = fabs(0.1 * dx);
+    if (st->first || st->prevReject) s->theta = 2 * s->jacRedo;
+    if (st->first) {
+        int kt = (int)s->logTol;
+        kt = kt < SX_KMAX - 1 ? kt : SX_KMAX - 1;
+        s->kTarg = kt > 1 ? kt : 1;
+    }
+    for (int i = 0; i < n; ++i) w->scale[i] = o->absTol + o->relTol * fabs(y[i]);
+    int jacUpdated = 0;
+    if (s->theta > s->jacRedo) { orc_jacobian(m, p, y, w->dfdx, w->J, o->jacobian, NULL); jacUpdated = 1; }
+    int k = 0;
+    double dxNew = fabs(dx);
+    int firstk = 1;
+    while (firstk || st->reject) {
+        dx = dxNew;
+        firstk = 0;
+        st->reject = 0;
+        double errOld = 0;
+        for (k = 0; k <= s->kTarg + 1; ++k) {
+            const int ok = seul(m, p, s, dx, k, w);
+            ++*nAtt;
+            if (!ok) { st->reject = 1; dxNew = fabs(dx) * sF5; break; }
+            if (k == 0) {
+                for (int i = 0; i < n; ++i) y[i] = w->ySeq[i];
+            } else {
+                for (int i = 0; i < n; ++i) w->table[(k - 1) * n + i] = w->ySeq[i];
+                extrapolate(s, n, k, w->table, y);
+                double err = 0;
+                for (int i = 0; i < n; ++i) {
+                    w->scale[i] = o->absTol + o->relTol * fabs(w->yS0[i]);
+                    const double d = (y[i] - w->table[i]) / w->scale[i];
+                    err += d * d;
+                }
+                err = sqrt(err / n);
+                if (!(err == err)) return FCS_CELL_NONFINITE;
+                if (err > 1 / OF_SMALL || (k > 1 && err >= errOld)) {
+                    st->reject = 1; dxNew = fabs(dx) * sF5; break;
+                }
+                errOld = 4 * err < 1 ? 4 * err : 1;
+                const double expo = 1.0 / (k + 1);
+                const double facmin = pow(sF3, expo);
+                double fac;
+                if (err == 0) fac = 1.0 / facmin;
+                else {
+                    fac = sF2 / pow(err / sF1, expo);
+                    const double hi = 1 / facmin < fac ? 1 / facmin : fac;
+                    fac = facmin / sF4 > hi ? facmin / sF4 : hi;
+                }
+                s->dxOpt[k] = fabs(dx * fac);
+                s->temp[k] = s->cpu[k] / s->dxOpt[k];
+                if ((st->first || st->last) && err <= 1.0) break;
+                if (k == s->kTarg - 1 && !st->prevReject && !st->first && !st->last) {
+                    if (err <= 1.0) break;
+                    else if (err > s->nSeq[s->kTarg] * s->nSeq[s->kTarg + 1] * 4.0) {
+                        st->reject = 1;
+                        s->kTarg = k;
+                        if (s->kTarg > 1 && s->temp[k - 1] < kF1 * s->temp[k]) s->kTarg--;
+                        dxNew = s->dxOpt[s->kTarg];
+                        break;
+                    }
+                }
+                if (k == s->kTarg) {
+                    if (err <= 1.0) break;
+                    else if (err > s->nSeq[k + 1] * 2.0) {
+                        st->reject = 1;
+                        if (s->kTarg > 1 && s->temp[k - 1] < kF1 * s->temp[k]) s->kTarg--;
+                        dxNew = s->dxOpt[s->kTarg];
+                        break;
+                    }
+                }
+                if (k == s->kTarg + 1) {
+                    if (err > 1.0) {
+                        st->reject = 1;
+                        if (s->kTarg > 1 && s->temp[s->kTarg - 1] < kF1 * s->temp[s->kTarg]) s->kTarg--;
+                        dxNew = s->dxOpt[s->kTarg];
+                    }
+                    break;
+                }
+            }
+        }
+        if (st->reject) {
+            st->prevReject = 1;
+            if (!jacUpdated) {
+                s->theta = 2.0 * s->jacRedo;
+                if (s->theta > s->jacRedo && !jacUpdated) {
+                    orc_jacobian(m, p, y, w->dfdx, w->J, o->jacobian, NULL);
+                    jacUpdated = 1;
+                }
+            }
+        }
+        if (*nAtt > 100 * o->maxSteps) return FCS_CELL_MAXSTEPS;
+    }
+    st->dxDid = dx;
+    *x += dx;
+    int kopt;
+    if (k == 1) kopt = 2;
+    else if (k <= s->kTarg) {
+        kopt = k;
+        if (s->temp[k - 1] < kF1 * s->temp[k]) kopt = k - 1;
+        else if (s->temp[k] < kF2 * s->temp[k - 1]) kopt = (k + 1 < SX_KMAX - 1) ? k + 1 : SX_KMAX - 1;
+    } else {
+        kopt = k - 1;
+        if (k > 2 && s->temp[k - 2] < kF1 * s->temp[k - 1]) kopt = k - 2;
+        if (s->temp[k] < kF2 * s->temp[kopt]) kopt = (k < SX_KMAX - 1) ? k : SX_KMAX - 1;
+    }
+    if (st->prevReject) {
+        s->kTarg = kopt < k ? kopt : k;
+        dxNew = fabs(dx) < s->dxOpt[s->kTarg] ? fabs(dx) : s->dxOpt[s->kTarg];
+        st->prevReject = 0;
+    } else {
+        if (kopt <= k) dxNew = s->dxOpt[kopt];
+        else {
+            if (k < s->kTarg && s->temp[k] < kF2 * s->temp[k - 1])
+                dxNew = s->dxOpt[k] * s->cpu[kopt + 1] / s->cpu[k];
+            else
+                dxNew = s->dxOpt[k] * s->cpu[kopt] / s->cpu[k];
+        }
+        s->kTarg = kopt;
+    }
+    st->dxTry = dxNew;
+    *kOut = k;
+    return 0;
+}
+
+/* ODESolve: src/ODE/OptSeulex/OptSeulex.C:171-259 */
+static int seulex_integrate(const orc_mech* m, const fcs_solver_opts* o, double p, double xEnd,
+                            double* dxTryIO, work* w, int32_t* nAtt, orc_step_rec* log,
+                            int32_t cap, int32_t* len)
+{
+    seulex_state s;
+    seulex_init(&s, o);
+    sx_step st = {*dxTryIO, 0.0, 1, 0, 0, 0};
+    const double xStart = 0.0;
+    double x = xStart;
+    for (int nStep = 0; nStep < o->maxSteps; ++nStep) {
+        const double dxTry0 = st.dxTry;
+        st.reject = 0;
+        if ((x + st.dxTry - xEnd) * (x + st.dxTry - xStart) > 0) { st.last = 1; st.dxTry = xEnd - x; }
+        int kk = 0;
+        const int rc = seulex_step(m, o, p, &s, &x, &st, w, nAtt, &kk);
+        if (rc) return rc;
+        logrec(log, cap, len, x, st.dxDid, 0.0, 1, kk);
+        if ((x - xEnd) * (xEnd - xStart) >= 0) {
+            if (nStep > 0 && st.last) st.dxTry = dxTry0;
+            *dxTryIO = st.dxTry;
+            return 0;
+        }
+        st.first = 0;
+        if (st.reject) st.prevReject = 1;
+    }
+    return FCS_CELL_MAXSTEPS;
+}
+
+int orc_integrate_cell(const orc_mech* m, const fcs_solver_opts* o, double p, double* y,
+                       double deltaT, double* dxTry, int32_t* nAttempts, orc_step_rec* log,
+                       int32_t cap, int32_t* len)
+{
+    work w;
+    work_init(&w, m->n);
+    for (int i = 0; i < m->ns; ++i) w.y0[i] = y[i] > 0 ? y[i] : 0.0;  /* OptRodas34.C:72-81 */
+    w.y0[m->ns] = y[m->ns];
+    int32_t na = 0;
+    if (len) *len = 0;
+    int rc = (o->solver == FCS_SOLVER_SEULEX)
+                 ? seulex_integrate(m, o, p, deltaT, dxTry, &w, &na, log, cap, len)
+                 : rosenbrock_integrate(m, o, p, deltaT, dxTry, &w, &na, log, cap, len);
+    for (int i = 0; i < m->ns; ++i) y[i] = w.y0[i] > 0 ? w.y0[i] : 0.0;  /* OptRodas34.C:114-117 */
+    y[m->ns] = w.y0[m->ns];
+    for (int i = 0; i <= m->ns; ++i) if (!(y[i] == y[i]) || isinf(y[i])) rc |= FCS_CELL_NONFINITE;
+    if (nAttempts) *nAttempts = na;
+    work_free(&w);
+    return rc;
+}
+
+/* src/ChemistryModel/FastChemistryModel_transientSolve.H:3-104 */
+int orc_solve(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells, double deltaT,
+              const double* T0, const double* p0, const double* rho0, const double* Y0,
+              double* deltaTChem, double* RR, double* deltaTMin, int32_t* cellStatus,
+              double* Tend, int32_t* nAttempts, int nThreads)
+{
+    const int ns = m->ns;
+    double dmin = OF_GREAT;
+    int anyFail = 0;
+#ifdef _OPENMP
+    if (nThreads < 1) nThreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nThreads) reduction(min:dmin) reduction(|:anyFail)
+#endif
+    for (int64_t c = 0; c < nCells; ++c) {
+        double y[512];
+        int32_t st = 0, na = 0;
+        const double T = T0[c];
+        if (T > o->Treact) {
+            for (int i = 0; i < ns; ++i) y[i] = Y0[(size_t)i * nCells + c];
+            y[ns] = T;
+            double dt = deltaTChem[c];
+            st = orc_integrate_cell(m, o, p0[c], y, deltaT, &dt, &na, NULL, 0, NULL);
+            deltaTChem[c] = dt;
+            dmin = dt < dmin ? dt : dmin;
+            if (deltaTChem[c] > o->deltaTChemMax) deltaTChem[c] = o->deltaTChemMax;
+            for (int i = 0; i < ns; ++i)
+                RR[(size_t)i * nCells + c] = (y[i] * rho0[c] - Y0[(size_t)i * nCells + c] * rho0[c]) / deltaT;
+            if (Tend) Tend[c] = y[ns];
+        } else {
+            st = FCS_CELL_SKIPPED;
+            for (int i = 0; i < ns; ++i) RR[(size_t)i * nCells + c] = 0.0;
+            if (Tend) Tend[c] = T;
+        }
+        if (cellStatus) cellStatus[c] = st;
+        if (nAttempts) nAttempts[c] = na;
+        if (st & (FCS_CELL_MAXSTEPS | FCS_CELL_NONFINITE)) anyFail |= 1;
+    }
+    const double twodt = 2 * deltaT;
+    if (deltaTMin) *deltaTMin = dmin < twodt ? dmin : twodt;
+    return anyFail ? FCS_E_CELLS : FCS_OK;
+}

@@ -1,0 +1,151 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+PKG = os.path.join(ROOT, "fastchemistrysolver-openfoam-10_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from fcs.abi import MechDesc, SolverOpts, dptr, iptr, make_desc, make_opts  # noqa: E402
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class StepRec(C.Structure):
+    _fields_ = [("x", C.c_double), ("dx", C.c_double), ("err", C.c_double),
+                ("accepted", C.c_int32), ("order", C.c_int32)]
+
+
+def build(force=False):
+    lib = os.path.join(HERE, "liboracle.so")
+    if force or not os.path.exists(lib):
+        subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(build())
+        L = _lib
+        L.orc_mech_create.restype = C.c_void_p
+        L.orc_mech_create.argtypes = [C.POINTER(MechDesc)]
+        L.orc_mech_destroy.argtypes = [C.c_void_p]
+        L.orc_thermo.argtypes = [C.c_void_p, C.c_double] + [_dp] * 5
+        L.orc_rates.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, _dp]
+        L.orc_derivatives.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.orc_jacobian.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, C.c_int, _dp]
+        L.orc_lu_factor.argtypes = [C.c_int, _dp, _ip, _dp]
+        L.orc_lu_solve.argtypes = [C.c_int, _dp, _ip, _dp, _dp]
+        L.orc_integrate_cell.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_double, _dp,
+                                         C.c_double, _dp, _ip, C.POINTER(StepRec), C.c_int32, _ip]
+        L.orc_solve.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, C.c_double,
+                                _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
+    return _lib
+
+
+class Oracle:
+    """One mechanism loaded into the CPU oracle."""
+
+    def __init__(self, mech):
+        self.mech = mech
+        self.ns = mech.n_species
+        self.n = self.ns + 1
+        self._desc, self._keep = make_desc(mech.arrays())
+        self._h = lib().orc_mech_create(C.byref(self._desc))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_mech_destroy(self._h)
+            self._h = None
+
+    def thermo(self, T):
+        out = [np.zeros(self.ns) for _ in range(5)]
+        lib().orc_thermo(self._h, float(T), *[dptr(o) for o in out])
+        return dict(zip(["cp", "ha", "negG", "dBdT", "dcpdT"], out))
+
+    def rates(self, p, T, Cc):
+        Cc = np.ascontiguousarray(Cc, np.float64)
+        om = np.zeros(self.ns)
+        lib().orc_rates(self._h, float(p), float(T), dptr(Cc), dptr(om))
+        return om
+
+    def derivatives(self, p, YT):
+        y = np.array(YT, np.float64)
+        d = np.zeros(self.n)
+        lib().orc_derivatives(self._h, float(p), dptr(y), dptr(d))
+        return d, y
+
+    def jacobian(self, p, YT, jac="exact", want_jc=False):
+        y = np.array(YT, np.float64)
+        f = np.zeros(self.n)
+        J = np.zeros((self.n, self.n))
+        Jc = np.zeros((self.n, self.n)) if want_jc else None
+        lib().orc_jacobian(self._h, float(p), dptr(y), dptr(f), dptr(J),
+                           0 if jac == "exact" else 1, dptr(Jc) if want_jc else None)
+        return (f, J, y, Jc) if want_jc else (f, J, y)
+
+    def integrate_cell(self, p, YT, deltaT, dxTry, log_cap=0, **opts):
+        o = make_opts(**opts)
+        y = np.array(YT, np.float64)
+        dx = np.array([dxTry], np.float64)
+        na = np.zeros(1, np.int32)
+        ln = np.zeros(1, np.int32)
+        recs = (StepRec * max(log_cap, 1))()
+        rc = lib().orc_integrate_cell(self._h, C.byref(o), float(p), dptr(y), float(deltaT),
+                                      dptr(dx), iptr(na), recs, int(log_cap), iptr(ln))
+        log = [(recs[i].x, recs[i].dx, recs[i].err, recs[i].accepted, recs[i].order)
+               for i in range(int(ln[0]))]
+        return dict(rc=rc, y=y, dxTry=float(dx[0]), nAttempts=int(na[0]), log=log)
+
+    def solve(self, field, deltaT, threads=1, rho=None, **opts):
+        """FastChemistryModel::solve(deltaT) over a Field (species-major Y)."""
+        o = make_opts(**opts)
+        nc = field.n_cells
+        T0 = np.ascontiguousarray(field.T, np.float64)
+        p0 = np.ascontiguousarray(field.p, np.float64)
+        rho0 = np.ascontiguousarray(field.rho(self.mech) if rho is None else rho, np.float64)
+        Y0 = np.ascontiguousarray(field.Y, np.float64)
+        dtc = np.array(field.deltaTChem, np.float64)
+        RR = np.zeros((self.ns, nc))
+        dmin = np.zeros(1)
+        st = np.zeros(nc, np.int32)
+        Tend = np.zeros(nc)
+        na = np.zeros(nc, np.int32)
+        rc = lib().orc_solve(self._h, C.byref(o), nc, float(deltaT), dptr(T0), dptr(p0),
+                             dptr(rho0), dptr(Y0), dptr(dtc), dptr(RR), dptr(dmin),
+                             iptr(st), dptr(Tend), iptr(na), int(threads))
+        return dict(rc=rc, RR=RR, deltaTChem=dtc, deltaTMin=float(dmin[0]), status=st,
+                    Tend=Tend, nAttempts=na, rho0=rho0)
+
+
+def lu_factor(a):
+    a = np.array(a, np.float64, order="C")
+    n = a.shape[0]
+    piv = np.zeros(n, np.int32)
+    invD = np.zeros(n)
+    lib().orc_lu_factor(n, dptr(a), iptr(piv), dptr(invD))
+    return a, piv, invD
+
+
+def lu_solve(a, b):
+    f, piv, invD = lu_factor(a)
+    x = np.array(b, np.float64)
+    lib().orc_lu_solve(f.shape[0], dptr(f), iptr(piv), dptr(invD), dptr(x))
+    return x, f, piv
