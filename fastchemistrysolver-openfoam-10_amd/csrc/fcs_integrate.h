@@ -1,0 +1,479 @@
+// fcs_integrate.h -- gfx950 device code of the ODE steppers: Rodas3(4),
+// ROS3(4) and linearly-implicit Euler extrapolation (Seulex), each with the
+// reference's step-size controller, run by one wavefront per cell.
+#pragma once
+#include "fcs_device.h"
+
+namespace fcs {
+
+// Controller constants: src/ODE/OptRodas34/OptRodas34.H:105-112
+constexpr double SAFE = 0.9, ALPHA_INC = 0.2, ALPHA_DEC = 0.25, MIN_SCALE = 0.2, MAX_SCALE = 10.0;
+
+// Rodas3(4) coefficients (Sandu et al. 1997): src/ODE/OptRodas34/OptRodas34.H:116-149
+namespace rodas {
+constexpr double d1 = 0.25, d2 = -0.1043, d3 = 0.1035, d4 = -0.3620000000000023e-01;
+constexpr double a21 = 0.1544e1, a31 = 0.9466785280815826, a32 = 0.2557011698983284;
+constexpr double a41 = 0.3314825187068521e1, a42 = 0.2896124015972201e1, a43 = 0.9986419139977817;
+constexpr double a51 = 0.1221224509226641e1, a52 = 0.6019134481288629e1, a53 = 0.1253708332932087e2,
+                 a54 = -0.6878860361058950;
+constexpr double c21 = -0.56688e1, c31 = -0.2430093356833875e1, c32 = -0.2063599157091915;
+constexpr double c41 = -0.1073529058151375, c42 = -0.9594562251023355e1, c43 = -0.2047028614809616e2;
+constexpr double c51 = 0.7496443313967647e1, c52 = -0.1024680431464352e2, c53 = -0.3399990352819905e2,
+                 c54 = 0.1170890893206160e2;
+constexpr double c61 = 0.8083246795921522e1, c62 = -0.7981132988064893e1, c63 = -0.3152159432874371e2,
+                 c64 = 0.1631930543123136e2, c65 = -0.6058818238834054e1;
+constexpr double invGamma = 4.0;
+}  // namespace rodas
+
+// ROS3(4) (Shampine 1982): src/ODE/OptRosenbrock34/OptRosenbrock34.H:127-158
+namespace ros {
+constexpr double a21 = 2.0, a31 = 48.0 / 25.0, a32 = 6.0 / 25.0;
+constexpr double c21 = -8.0, c31 = 372.0 / 25.0, c32 = 12.0 / 5.0;
+constexpr double c41 = -112.0 / 125.0, c42 = -54.0 / 125.0, c43 = -2.0 / 5.0;
+constexpr double b1 = 19.0 / 9.0, b2 = 1.0 / 2.0, b3 = 25.0 / 108.0, b4 = 125.0 / 108.0;
+constexpr double e1 = 34.0 / 108.0, e2 = 7.0 / 36.0, e4 = 125.0 / 108.0;
+constexpr double d1 = 1.0 / 2.0, d2 = -3.0 / 2.0, d3 = 605.0 / 250.0, d4 = 29.0 / 250.0;
+constexpr double invGamma = 2.0;
+}  // namespace ros
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// Max-norm error (src/ODE/OptRodas34/OptRodas34.C:822-877); NaN in any
+// lane is returned as NaN.
+template <int NP>
+__device__ inline double error_norm(const DevOpts& o, double y0, double yT, double err)
+{
+    const int lane = lane_id();
+    double r = 0.0;
+    if (lane < NP) {
+        const double a0 = fabs(y0), a1 = fabs(yT);
+        r = fabs(err) / (o.absTol + o.relTol * (a1 > a0 ? a1 : a0));
+    }
+    const bool bad = wave_any(!(r == r));
+    const double mx = wave_max(r);
+    return bad ? __builtin_nan("") : mx;
+}
+
+template <int NP>
+__device__ inline void form_matrix(double (&a)[NP], double diag)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        a[j] = -a[j];
+        if (lane == j) a[j] += diag;
+    }
+}
+
+// Rosenbrock tableau in the transformed (Hairer-Wanner) form the reference
+// codes stage by stage.  Stage s (0-based) solves
+//   (I/(gamma dx) - J) k_s = f(y0 + sum_j A[s][j] k_j) + D[s] dx f(y0)
+//                            + sum_j C[s][j]/dx k_j           (s > 0)
+//   (I/(gamma dx) - J) k_0 = (1 + D[0] dx) f(y0)
+// with f(y0) returned by the Jacobian evaluation (quirk of the reference).
+// A stage with newF = 0 reuses the previous stage's f.
+struct Tableau {
+    int stages;
+    double invGamma;
+    int newF[6];
+    double A[6][6];
+    double C[6][6];
+    double D[6];
+    double B[6];    // solution weights (final y = y0 + sum B k)
+    double E[6];    // error weights   (err = sum E k)
+};
+
+// Rodas3(4): src/ODE/OptRodas34/OptRodas34.H:116-149 and the stage sums of
+// OptRodas34.C:345-820.  Stage 5 evaluates f at y0 + dy with
+// dy = sum a5j k_j, stage 6 at y0 + dy + k5; the new state is
+// y0 + dy + k5 + k6 and the error estimate is k6.
+__constant__ const Tableau kRodas34 = {
+    6, rodas::invGamma, {0, 1, 1, 1, 1, 1},
+    {{0, 0, 0, 0, 0, 0},
+     {rodas::a21, 0, 0, 0, 0, 0},
+     {rodas::a31, rodas::a32, 0, 0, 0, 0},
+     {rodas::a41, rodas::a42, rodas::a43, 0, 0, 0},
+     {rodas::a51, rodas::a52, rodas::a53, rodas::a54, 0, 0},
+     {rodas::a51, rodas::a52, rodas::a53, rodas::a54, 1.0, 0}},
+    {{0, 0, 0, 0, 0, 0},
+     {rodas::c21, 0, 0, 0, 0, 0},
+     {rodas::c31, rodas::c32, 0, 0, 0, 0},
+     {rodas::c41, rodas::c42, rodas::c43, 0, 0, 0},
+     {rodas::c51, rodas::c52, rodas::c53, rodas::c54, 0, 0},
+     {rodas::c61, rodas::c62, rodas::c63, rodas::c64, rodas::c65, 0}},
+    {rodas::d1, rodas::d2, rodas::d3, rodas::d4, 0, 0},
+    {rodas::a51, rodas::a52, rodas::a53, rodas::a54, 1.0, 1.0},
+    {0, 0, 0, 0, 0, 1.0}};
+
+// ROS3(4): src/ODE/OptRosenbrock34/OptRosenbrock34.H:127-158 and
+// OptRosenbrock34.C:340-640 (stage 4 reuses the stage-3 derivative).
+__constant__ const Tableau kRos34 = {
+    4, ros::invGamma, {0, 1, 1, 0, 0, 0},
+    {{0, 0, 0, 0, 0, 0},
+     {ros::a21, 0, 0, 0, 0, 0},
+     {ros::a31, ros::a32, 0, 0, 0, 0},
+     {ros::a31, ros::a32, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0}},
+    {{0, 0, 0, 0, 0, 0},
+     {ros::c21, 0, 0, 0, 0, 0},
+     {ros::c31, ros::c32, 0, 0, 0, 0},
+     {ros::c41, ros::c42, ros::c43, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0}},
+    {ros::d1, ros::d2, ros::d3, ros::d4, 0, 0},
+    {ros::b1, ros::b2, ros::b3, ros::b4, 0, 0},
+    {ros::e1, ros::e2, 0.0, ros::e4, 0, 0}};
+
+// One Rosenbrock attempt from y0 with step dx.  y0 is clipped in place by
+// the Jacobian (src/ChemistryModel/FastChemistryModel.C:498-501); yT
+// receives the candidate state.  Returns the error norm.
+template <int NP>
+__device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOpts& o, const Tableau& tb,
+                                     double p, double lnp, double dx, double& y0, double& yT)
+{
+    double a[NP];
+    const double invdx = 1.0 / dx;
+    const double dfdx = jacobian<NP>(M, L, p, lnp, o.jac, y0, a);
+    form_matrix<NP>(a, 1.0 * invdx * tb.invGamma);
+    const LuState st = lu_factor<NP>(L, a);
+    double k[6] = {0, 0, 0, 0, 0, 0};
+    double dydx = 0.0;
+    for (int s = 0; s < tb.stages; ++s) {
+        double rhs;
+        if (s == 0) {
+            rhs = (dx * tb.D[0] + 1) * dfdx;
+        } else {
+            if (tb.newF[s]) {
+                double yS = y0;
+#pragma unroll
+                for (int j = 0; j < 5; ++j) yS += tb.A[s][j] * k[j];
+                dydx = derivatives<NP>(M, L, p, lnp, yS);
+            }
+            rhs = dydx + (dx * tb.D[s]) * dfdx;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) rhs += (tb.C[s][j] * invdx) * k[j];
+        }
+        const double ks = lu_solve<NP>(a, st, rhs);
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            if (j == s) k[j] = ks;
+    }
+    double err = 0.0;
+    yT = y0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        yT += tb.B[j] * k[j];
+        err += tb.E[j] * k[j];
+    }
+    return error_norm<NP>(o, y0, yT, err);
+}
+
+// ODESolve + adaptiveSolve of the Rosenbrock solvers
+// (src/ODE/OptRodas34/OptRodas34.C:123-286).  Returns status bits.
+template <int NP>
+__device__ int rosenbrock_integrate(const DevMech& M, const Lds& L, const DevOpts& o, const Tableau& tb,
+                                    double p, double xEnd, double& y0, double& dxTryIO, int& nAtt)
+{
+    const double lnp = log(p);
+    const double ratio = pow(MAX_SCALE / SAFE, -1.0 / ALPHA_INC);
+    const double safeMax = SAFE * MAX_SCALE;
+    const double xStart = 0.0;
+    double x = xStart, dxTry = dxTryIO;
+    bool last = false;
+    double yT = 0.0;
+    const int attemptCap = 100 * o.maxSteps;
+    for (int nStep = 0; nStep < o.maxSteps; ++nStep) {
+        const double dxTry0 = dxTry;
+        if ((x + dxTry - xEnd) * (x + dxTry - xStart) > 0) { last = true; dxTry = xEnd - x; }
+        double dx = dxTry;
+        double Err;
+        for (;;) {
+            Err = rosenbrock_attempt<NP>(M, L, o, tb, p, lnp, dx, y0, yT);
+            ++nAtt;
+            if (!(Err == Err)) return CELL_NONFINITE;
+            if (!(Err > 1)) break;
+            if (nAtt > attemptCap) return CELL_MAXSTEPS;
+            const double sc = SAFE * pow(Err, -ALPHA_DEC);
+            dx *= sc > MIN_SCALE ? sc : MIN_SCALE;
+        }
+        x += dx;
+        y0 = yT;
+        if (Err > ratio) {
+            double f = SAFE * pow(Err, -ALPHA_INC);
+            f = f > MIN_SCALE ? f : MIN_SCALE;
+            f = f < MAX_SCALE ? f : MAX_SCALE;
+            dxTry = f * dx;
+        } else {
+            dxTry = safeMax * dx;
+        }
+        if ((x - xEnd) * (xEnd - xStart) >= 0) {
+            if (nStep > 0 && last) dxTry = dxTry0;
+            dxTryIO = dxTry;
+            return 0;
+        }
+    }
+    return CELL_MAXSTEPS;
+}
+
+// ---------------------------------------------------------------- Seulex
+// src/ODE/OptSeulex/OptSeulex.C:33-731.  Per-lane state: the extrapolation
+// table (kMax rows), the step-start state y0_ and the scale vector; the
+// Jacobian Jy stays resident in the LDS buffer between Jacobian updates.
+constexpr int SX_KMAX = 12, SX_IMAX = 13;
+
+struct SeulexConst {
+    int nSeq[SX_IMAX];
+    double cpu[SX_IMAX];
+    double coeff[SX_IMAX][SX_IMAX];
+};
+
+// Step-number sequence, work model and extrapolation coefficients
+// (src/ODE/OptSeulex/OptSeulex.C:63-89).
+constexpr SeulexConst make_seulex_const()
+{
+    SeulexConst s{};
+    const double cpuFunc = 1, cpuJac = 5, cpuLU = 1, cpuSolve = 1;
+    s.nSeq[0] = 2;
+    s.nSeq[1] = 3;
+    for (int i = 2; i < SX_IMAX; ++i) s.nSeq[i] = 2 * s.nSeq[i - 2];
+    s.cpu[0] = cpuJac + cpuLU + s.nSeq[0] * (cpuFunc + cpuSolve);
+    for (int k = 0; k < SX_KMAX; ++k) s.cpu[k + 1] = s.cpu[k] + (s.nSeq[k + 1] - 1) * (cpuFunc + cpuSolve) + cpuLU;
+    for (int k = 0; k < SX_IMAX; ++k)
+        for (int l = 0; l < SX_IMAX; ++l)
+            s.coeff[k][l] = (l < k) ? 1.0 / ((double)s.nSeq[k] / s.nSeq[l] - 1.0) : 0.0;
+    return s;
+}
+__constant__ const SeulexConst kSeulex = make_seulex_const();
+
+// Jy row of this lane <-> LDS (kept across seul() calls of one step).
+template <int NP>
+__device__ __forceinline__ void jy_store(const Lds& L, const double (&a)[NP])
+{
+#pragma unroll
+    for (int j = 0; j < NP; ++j) L.J[j * WAVE + lane_id()] = a[j];
+}
+template <int NP>
+__device__ __forceinline__ void jy_load(const Lds& L, double (&a)[NP])
+{
+#pragma unroll
+    for (int j = 0; j < NP; ++j) a[j] = L.J[j * WAVE + lane_id()];
+}
+
+// seul(): src/ODE/OptSeulex/OptSeulex.C:562-706, written with a single RHS
+// evaluation site.  y0s (this lane's y0_) is clipped in place by the first
+// RHS evaluation; returns false when the theta divergence test fails.
+template <int NP>
+__device__ bool seul(const DevMech& M, const Lds& L, double p, double lnp, double& y0s, double dxTot, int k,
+                     int nSteps, double scale, double& theta, double& ySeq)
+{
+    const int lane = lane_id();
+    double a[NP];
+    jy_load<NP>(L, a);
+    const double invdx = nSteps / dxTot;
+    form_matrix<NP>(a, invdx);
+    const LuState st = lu_factor<NP>(L, a);
+    double yT = 0.0, dy = 0.0;
+    int nn = 0;
+    bool thetaPhase = false;
+    double dy1 = 0.0;
+    for (;;) {
+        double ev = (nn == 0) ? y0s : yT;
+        const double f = derivatives<NP>(M, L, p, lnp, ev);
+        if (nn == 0) y0s = ev; else yT = ev;
+        dy = lu_solve<NP>(a, st, thetaPhase ? f - dy * invdx : f);
+        if (nn == 0) yT = y0s;
+        if (thetaPhase) {
+            const bool huge = wave_any(lane < NP && fabs(dy) > sqrt(OF_VGREAT));
+            double dy2 = huge ? OF_VGREAT : wave_sum(lane < NP ? (dy / scale) * (dy / scale) : 0.0);
+            dy2 = sqrt(dy2);
+            const double den = 1.0 > dy1 + OF_SMALL ? 1.0 : dy1 + OF_SMALL;
+            theta = dy2 / den;
+            if (theta > 1.0) return false;
+            thetaPhase = false;
+            continue;
+        }
+        ++nn;
+        if (nn >= nSteps) break;
+        yT += dy;
+        if (nn == 1 && k <= 1) {
+            dy1 = sqrt(wave_sum(lane < NP ? (dy / scale) * (dy / scale) : 0.0));
+            thetaPhase = true;
+        }
+    }
+    ySeq = yT + dy;
+    return true;
+}
+
+// ODESolve + SeulexSolve (src/ODE/OptSeulex/OptSeulex.C:171-559).
+template <int NP>
+__device__ int seulex_integrate(const DevMech& M, const Lds& L, const DevOpts& o, double p, double xEnd,
+                                double& y, double& dxTryIO, int& nAtt)
+{
+    const SeulexConst& S = kSeulex;
+    const int lane = lane_id();
+    const double lnp = log(p);
+    const double kF1 = 0.7, kF2 = 0.9, sF1 = 0.8, sF2 = 0.93, sF3 = 0.1, sF4 = 4, sF5 = 0.5;
+    const double jacRedo = o.relTol < 1e-4 ? o.relTol : 1e-4;
+    const double logTol = -log10(o.relTol + o.absTol) * 0.6 + 0.5;
+    double theta = 2 * jacRedo;
+    int kTarg = 0;
+    double dxOpt[SX_IMAX], temp[SX_IMAX];
+    double table[SX_KMAX];
+#pragma unroll
+    for (int q = 0; q < SX_KMAX; ++q) table[q] = 0.0;
+    // stepState (src/ODE/OptSeulex/OptSeulex.H:75-95)
+    double dxTry = dxTryIO;
+    bool first = true, last = false, reject = false, prevReject = false;
+    const double xStart = 0.0;
+    double x = xStart;
+    const int attemptCap = 100 * o.maxSteps;
+    for (int nStep = 0; nStep < o.maxSteps; ++nStep) {
+        const double dxTry0 = dxTry;
+        reject = false;
+        if ((x + dxTry - xEnd) * (x + dxTry - xStart) > 0) { last = true; dxTry = xEnd - x; }
+        // ---- SeulexSolve ----
+        temp[0] = OF_GREAT;
+        double dx = dxTry;
+        double y0s = y;
+        dxOpt[0] = fabs(0.1 * dx);
+        if (first || prevReject) theta = 2 * jacRedo;
+        if (first) {
+            int kt = (int)logTol;
+            kt = kt < SX_KMAX - 1 ? kt : SX_KMAX - 1;
+            kTarg = kt > 1 ? kt : 1;
+        }
+        double scale = o.absTol + o.relTol * fabs(y);
+        bool jacUpdated = false;
+        bool needJac = theta > jacRedo;
+        int k = 0;
+        double dxNew = fabs(dx);
+        bool firstk = true;
+        while (firstk || reject) {
+            if (needJac) {
+                double jy[NP];
+                (void)jacobian<NP>(M, L, p, lnp, o.jac, y, jy);
+                wave_sync();
+                jy_store<NP>(L, jy);
+                wave_sync();
+                jacUpdated = true;
+                needJac = false;
+            }
+            dx = dxNew;
+            firstk = false;
+            reject = false;
+            double errOld = 0;
+            for (k = 0; k <= kTarg + 1; ++k) {
+                double ySeq = 0.0;
+                const bool ok = seul<NP>(M, L, p, lnp, y0s, dx, k, S.nSeq[k], scale, theta, ySeq);
+                ++nAtt;
+                if (!ok) { reject = true; dxNew = fabs(dx) * sF5; break; }
+                if (k == 0) {
+                    y = ySeq;
+                    continue;
+                }
+                // table[k-1] = ySeq, then extrapolate (OptSeulex.C:711-731)
+#pragma unroll
+                for (int q = 0; q < SX_KMAX; ++q)
+                    if (q == k - 1) table[q] = ySeq;
+#pragma unroll
+                for (int j = SX_KMAX - 1; j > 0; --j)
+                    if (j <= k - 1) table[j - 1] = table[j] + S.coeff[k][j] * (table[j] - table[j - 1]);
+                y = table[0] + S.coeff[k][0] * (table[0] - y);
+                scale = o.absTol + o.relTol * fabs(y0s);
+                const double dd = (y - table[0]) / scale;
+                const double err = sqrt(wave_sum(lane < NP ? dd * dd : 0.0) / (M.ns + 1));
+                if (!(err == err)) return CELL_NONFINITE;
+                if (err > 1 / OF_SMALL || (k > 1 && err >= errOld)) {
+                    reject = true;
+                    dxNew = fabs(dx) * sF5;
+                    break;
+                }
+                errOld = 4 * err < 1 ? 4 * err : 1;
+                const double expo = 1.0 / (k + 1);
+                const double facmin = pow(sF3, expo);
+                double fac;
+                if (err == 0) fac = 1.0 / facmin;
+                else {
+                    fac = sF2 / pow(err / sF1, expo);
+                    const double hi = 1 / facmin < fac ? 1 / facmin : fac;
+                    fac = facmin / sF4 > hi ? facmin / sF4 : hi;
+                }
+                dxOpt[k] = fabs(dx * fac);
+                temp[k] = S.cpu[k] / dxOpt[k];
+                if ((first || last) && err <= 1.0) break;
+                if (k == kTarg - 1 && !prevReject && !first && !last) {
+                    if (err <= 1.0) break;
+                    else if (err > S.nSeq[kTarg] * S.nSeq[kTarg + 1] * 4.0) {
+                        reject = true;
+                        kTarg = k;
+                        if (kTarg > 1 && temp[k - 1] < kF1 * temp[k]) kTarg--;
+                        dxNew = dxOpt[kTarg];
+                        break;
+                    }
+                }
+                if (k == kTarg) {
+                    if (err <= 1.0) break;
+                    else if (err > S.nSeq[k + 1] * 2.0) {
+                        reject = true;
+                        if (kTarg > 1 && temp[k - 1] < kF1 * temp[k]) kTarg--;
+                        dxNew = dxOpt[kTarg];
+                        break;
+                    }
+                }
+                if (k == kTarg + 1) {
+                    if (err > 1.0) {
+                        reject = true;
+                        if (kTarg > 1 && temp[kTarg - 1] < kF1 * temp[kTarg]) kTarg--;
+                        dxNew = dxOpt[kTarg];
+                    }
+                    break;
+                }
+            }
+            if (reject) {
+                prevReject = true;
+                if (!jacUpdated) {
+                    theta = 2.0 * jacRedo;
+                    if (theta > jacRedo && !jacUpdated) needJac = true;
+                }
+            }
+            if (nAtt > attemptCap) return CELL_MAXSTEPS;
+        }
+        x += dx;
+        int kopt;
+        if (k == 1) kopt = 2;
+        else if (k <= kTarg) {
+            kopt = k;
+            if (temp[k - 1] < kF1 * temp[k]) kopt = k - 1;
+            else if (temp[k] < kF2 * temp[k - 1]) kopt = (k + 1 < SX_KMAX - 1) ? k + 1 : SX_KMAX - 1;
+        } else {
+            kopt = k - 1;
+            if (k > 2 && temp[k - 2] < kF1 * temp[k - 1]) kopt = k - 2;
+            if (temp[k] < kF2 * temp[kopt]) kopt = (k < SX_KMAX - 1) ? k : SX_KMAX - 1;
+        }
+        if (prevReject) {
+            kTarg = kopt < k ? kopt : k;
+            dxNew = fabs(dx) < dxOpt[kTarg] ? fabs(dx) : dxOpt[kTarg];
+            prevReject = false;
+        } else {
+            if (kopt <= k) dxNew = dxOpt[kopt];
+            else {
+                if (k < kTarg && temp[k] < kF2 * temp[k - 1]) dxNew = dxOpt[k] * S.cpu[kopt + 1] / S.cpu[k];
+                else dxNew = dxOpt[k] * S.cpu[kopt] / S.cpu[k];
+            }
+            kTarg = kopt;
+        }
+        dxTry = dxNew;
+        // ---- end SeulexSolve ----
+        if ((x - xEnd) * (xEnd - xStart) >= 0) {
+            if (nStep > 0 && last) dxTry = dxTry0;
+            dxTryIO = dxTry;
+            return 0;
+        }
+        first = false;
+        if (reject) prevReject = true;
+    }
+    return CELL_MAXSTEPS;
+}
+
+}  // namespace fcs

@@ -1,0 +1,198 @@
+"""ctypes binding of libfcs.so (include/fcs.h).
+
+The library is the only compute path: if it is missing or no GPU is
+visible, the calls raise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (FCS_E_CELLS, FCS_OK, MechDesc, SolverOpts, dptr, iptr, make_desc, make_opts,
+                  row_pointers)
+from .build import LIB
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+EXPORTS = [
+    "fcs_device_count", "fcs_version", "fcs_mech_create", "fcs_mech_destroy", "fcs_solver_create",
+    "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_batch_create", "fcs_batch_destroy",
+    "fcs_batch_upload", "fcs_batch_run", "fcs_batch_download", "fcs_eval_derivatives",
+    "fcs_eval_jacobian", "fcs_lu_solve",
+]
+
+
+class FcsError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libfcs.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB
+    if not os.path.exists(path):
+        raise FcsError(f"{path} not built: run __graft_entry__.build() or python -m fcs.build")
+    L = C.CDLL(path)
+    L.fcs_version.restype = C.c_char_p
+    L.fcs_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.fcs_mech_create.argtypes = [C.POINTER(MechDesc), C.POINTER(_vp)]
+    L.fcs_mech_destroy.argtypes = [_vp]
+    L.fcs_solver_create.argtypes = [_vp, C.POINTER(SolverOpts), C.POINTER(_vp)]
+    L.fcs_solver_destroy.argtypes = [_vp]
+    L.fcs_solve.argtypes = [_vp, C.c_int64, C.c_double, _dp, _dp, _dp, C.POINTER(_dp), _dp,
+                            C.POINTER(_dp), _dp, _ip]
+    L.fcs_solve_ex.argtypes = L.fcs_solve.argtypes + [_dp, _ip]
+    L.fcs_batch_create.argtypes = [_vp, C.c_int64, C.POINTER(_vp)]
+    L.fcs_batch_destroy.argtypes = [_vp]
+    L.fcs_batch_upload.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+    L.fcs_batch_run.argtypes = [_vp, C.c_double, C.POINTER(C.c_float)]
+    L.fcs_batch_download.argtypes = [_vp, _dp, _dp, _dp, _ip, _dp, _ip]
+    L.fcs_eval_derivatives.argtypes = [_vp, C.c_int64, _dp, _dp, _dp]
+    L.fcs_eval_jacobian.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp]
+    L.fcs_lu_solve.argtypes = [_vp, C.c_int64, _dp, _dp]
+    for name in EXPORTS:
+        getattr(L, name)
+        if name != "fcs_version":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc, what, allow=()):
+    if rc != FCS_OK and rc not in allow:
+        raise FcsError(f"{what} failed with code {rc}")
+    return rc
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(load().fcs_device_count(C.byref(n)), "fcs_device_count")
+    return n.value
+
+
+class Mech:
+    """Device-resident mechanism tables (fcs_mech_create)."""
+
+    def __init__(self, mech):
+        self.mech = mech
+        self.ns = mech.n_species
+        self._desc, self._keep = make_desc(mech.arrays())
+        h = _vp()
+        check(load().fcs_mech_create(C.byref(self._desc), C.byref(h)), "fcs_mech_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fcs_mech_destroy(self._h)
+            self._h = None
+
+
+class Solver:
+    """fcs_solver: a mechanism plus the chemistryProperties options."""
+
+    def __init__(self, mech: Mech, **opts):
+        self.mechh = mech
+        self.ns = mech.ns
+        self.n = mech.ns + 1
+        self.opts = make_opts(**opts)
+        h = _vp()
+        check(load().fcs_solver_create(mech._h, C.byref(self.opts), C.byref(h)), "fcs_solver_create")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fcs_solver_destroy(self._h)
+            self._h = None
+
+    # ---- solve(deltaT) over host fields
+    def solve(self, field, deltaT, rho=None):
+        nc = field.n_cells
+        T0 = np.ascontiguousarray(field.T, np.float64)
+        p0 = np.ascontiguousarray(field.p, np.float64)
+        rho0 = np.ascontiguousarray(field.rho(self.mechh.mech) if rho is None else rho, np.float64)
+        Y0 = np.ascontiguousarray(field.Y, np.float64)
+        dtc = np.array(field.deltaTChem, np.float64)
+        RR = np.zeros((self.ns, nc))
+        dmin = np.zeros(1)
+        st = np.zeros(nc, np.int32)
+        Tend = np.zeros(nc)
+        na = np.zeros(nc, np.int32)
+        rc = load().fcs_solve_ex(self._h, nc, float(deltaT), dptr(T0), dptr(p0), dptr(rho0),
+                                 row_pointers(Y0), dptr(dtc), row_pointers(RR), dptr(dmin), iptr(st),
+                                 dptr(Tend), iptr(na))
+        check(rc, "fcs_solve_ex", allow=(FCS_E_CELLS,))
+        return dict(rc=rc, RR=RR, deltaTChem=dtc, deltaTMin=float(dmin[0]), status=st, Tend=Tend,
+                    nAttempts=na, rho0=rho0)
+
+    # ---- kernel-level entries
+    def derivatives(self, p, YT):
+        YT = np.array(YT, np.float64, order="C").reshape(self.n, -1)
+        nS = YT.shape[1]
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(p, np.float64), (nS,)))
+        out = np.zeros_like(YT)
+        check(load().fcs_eval_derivatives(self._h, nS, dptr(p), dptr(YT), dptr(out)), "fcs_eval_derivatives")
+        return out, YT
+
+    def jacobian(self, p, YT):
+        YT = np.array(YT, np.float64, order="C").reshape(self.n, -1)
+        nS = YT.shape[1]
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(p, np.float64), (nS,)))
+        f = np.zeros_like(YT)
+        J = np.zeros((nS, self.n, self.n))
+        check(load().fcs_eval_jacobian(self._h, nS, dptr(p), dptr(YT), dptr(f), dptr(J)), "fcs_eval_jacobian")
+        return f, J, YT
+
+    def lu_solve(self, A, b):
+        A = np.ascontiguousarray(A, np.float64).reshape(-1, self.n, self.n)
+        x = np.array(b, np.float64, order="C").reshape(-1, self.n)
+        check(load().fcs_lu_solve(self._h, A.shape[0], dptr(A), dptr(x)), "fcs_lu_solve")
+        return x
+
+
+class Batch:
+    """Device-resident batch of cells (inputs stay in HBM between runs)."""
+
+    def __init__(self, solver: Solver, field, rho=None):
+        self.s = solver
+        self.nc = field.n_cells
+        h = _vp()
+        check(load().fcs_batch_create(solver._h, self.nc, C.byref(h)), "fcs_batch_create")
+        self._h = h
+        self._in = dict(
+            T0=np.ascontiguousarray(field.T, np.float64), p0=np.ascontiguousarray(field.p, np.float64),
+            rho0=np.ascontiguousarray(field.rho(solver.mechh.mech) if rho is None else rho, np.float64),
+            Y0=np.ascontiguousarray(field.Y, np.float64), dtc=np.ascontiguousarray(field.deltaTChem, np.float64))
+        i = self._in
+        check(load().fcs_batch_upload(h, dptr(i["T0"]), dptr(i["p0"]), dptr(i["rho0"]), dptr(i["Y0"]),
+                                      dptr(i["dtc"])), "fcs_batch_upload")
+
+    def run(self, deltaT, timed=True) -> float:
+        ms = C.c_float(0.0)
+        check(load().fcs_batch_run(self._h, float(deltaT), C.byref(ms) if timed else None), "fcs_batch_run")
+        return ms.value
+
+    def download(self):
+        ns, nc = self.s.ns, self.nc
+        RR = np.zeros((ns, nc))
+        dtc = np.zeros(nc)
+        dmin = np.zeros(1)
+        st = np.zeros(nc, np.int32)
+        Tend = np.zeros(nc)
+        na = np.zeros(nc, np.int32)
+        check(load().fcs_batch_download(self._h, dptr(RR), dptr(dtc), dptr(dmin), iptr(st), dptr(Tend),
+                                        iptr(na)), "fcs_batch_download")
+        return dict(RR=RR, deltaTChem=dtc, deltaTMin=float(dmin[0]), status=st, Tend=Tend, nAttempts=na)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fcs_batch_destroy(self._h)
+            self._h = None

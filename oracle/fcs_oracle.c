@@ -648,6 +648,13 @@ static const double R_c61 = 0.8083246795921522e1, R_c62 = -0.7981132988064893e1,
                     R_c64 = 0.1631930543123136e2, R_c65 = -0.6058818238834054e1;
 static const double R_gamma = 0.25;
 
+/* Test hook: with 1 the Rosenbrock stages take df/dt = 0 (the textbook
+ * autonomous form) instead of the reference's f(y0) in the dfdx slot
+ * (quirk of src/ChemistryModel/FastChemistryModel.C:489-572), which lets the
+ * order-of-accuracy tests see the tableau's own order. */
+static int g_dfdt_zero = 0;
+void orc_set_textbook_rosenbrock(int on) { g_dfdt_zero = on; }
+
 static void form_matrix(int n, double* a, const double* J, double diag)
 {
     for (int i = 0; i < n * n; ++i) a[i] = -J[i];
@@ -665,27 +672,28 @@ static double rodas34_attempt(const orc_mech* m, const fcs_solver_opts* o, doubl
     orc_jacobian(m, p, y0, dfdx, w->J, o->jacobian, NULL);
     form_matrix(n, w->a, w->J, 1.0 * invdx * (1.0 / R_gamma));
     orc_lu_factor(n, w->a, w->piv, w->invD);
-    const double dxd1 = dx * R_d1 + 1;
+    const double tq = g_dfdt_zero ? 0.0 : 1.0;
+    const double dxd1 = tq * dx * R_d1 + 1;
     for (int i = 0; i < n; ++i) k1[i] = dxd1 * dfdx[i];
     orc_lu_solve(n, w->a, w->piv, w->invD, k1);
     for (int i = 0; i < n; ++i) yT[i] = y0[i] + R_a21 * k1[i];
     orc_derivatives(m, p, yT, dydx);
     {
-        const double e = dx * R_d2, c1 = R_c21 * invdx;
+        const double e = tq * dx * R_d2, c1 = R_c21 * invdx;
         for (int i = 0; i < n; ++i) k2[i] = dydx[i] + e * dfdx[i] + c1 * k1[i];
     }
     orc_lu_solve(n, w->a, w->piv, w->invD, k2);
     for (int i = 0; i < n; ++i) yT[i] = y0[i] + R_a31 * k1[i] + R_a32 * k2[i];
     orc_derivatives(m, p, yT, dydx);
     {
-        const double e = dx * R_d3, c1 = R_c31 * invdx, c2 = R_c32 * invdx;
+        const double e = tq * dx * R_d3, c1 = R_c31 * invdx, c2 = R_c32 * invdx;
         for (int i = 0; i < n; ++i) k3[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i];
     }
     orc_lu_solve(n, w->a, w->piv, w->invD, k3);
     for (int i = 0; i < n; ++i) yT[i] = y0[i] + R_a41 * k1[i] + R_a42 * k2[i] + R_a43 * k3[i];
     orc_derivatives(m, p, yT, dydx);
     {
-        const double e = dx * R_d4, c1 = R_c41 * invdx, c2 = R_c42 * invdx, c3 = R_c43 * invdx;
+        const double e = tq * dx * R_d4, c1 = R_c41 * invdx, c2 = R_c42 * invdx, c3 = R_c43 * invdx;
         for (int i = 0; i < n; ++i) k4[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i];
     }
     orc_lu_solve(n, w->a, w->piv, w->invD, k4);
@@ -732,25 +740,26 @@ static double ros34_attempt(const orc_mech* m, const fcs_solver_opts* o, double 
     orc_jacobian(m, p, y0, dfdx, w->J, o->jacobian, NULL);
     form_matrix(n, w->a, w->J, invdx * (1.0 / S_gamma));
     orc_lu_factor(n, w->a, w->piv, w->invD);
-    const double dxd1 = dx * S_d1 + 1;
+    const double tq = g_dfdt_zero ? 0.0 : 1.0;
+    const double dxd1 = tq * dx * S_d1 + 1;
     for (int i = 0; i < n; ++i) k1[i] = dxd1 * dfdx[i];
     orc_lu_solve(n, w->a, w->piv, w->invD, k1);
     for (int i = 0; i < n; ++i) yT[i] = y0[i] + S_a21 * k1[i];
     orc_derivatives(m, p, yT, dydx);
     {
-        const double e = dx * S_d2, c1 = S_c21 * invdx;
+        const double e = tq * dx * S_d2, c1 = S_c21 * invdx;
         for (int i = 0; i < n; ++i) k2[i] = dydx[i] + e * dfdx[i] + c1 * k1[i];
     }
     orc_lu_solve(n, w->a, w->piv, w->invD, k2);
     for (int i = 0; i < n; ++i) yT[i] = y0[i] + S_a31 * k1[i] + S_a32 * k2[i];
     orc_derivatives(m, p, yT, dydx);
     {
-        const double e = dx * S_d3, c1 = S_c31 * invdx, c2 = S_c32 * invdx;
+        const double e = tq * dx * S_d3, c1 = S_c31 * invdx, c2 = S_c32 * invdx;
         for (int i = 0; i < n; ++i) k3[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i];
     }
     orc_lu_solve(n, w->a, w->piv, w->invD, k3);
     {
-        const double e = dx * S_d4, c1 = S_c41 * invdx, c2 = S_c42 * invdx, c3 = S_c43 * invdx;
+        const double e = tq * dx * S_d4, c1 = S_c41 * invdx, c2 = S_c42 * invdx, c3 = S_c43 * invdx;
         for (int i = 0; i < n; ++i) k4[i] = dydx[i] + e * dfdx[i] + c1 * k1[i] + c2 * k2[i] + c3 * k3[i];
     }
     orc_lu_solve(n, w->a, w->piv, w->invD, k4);

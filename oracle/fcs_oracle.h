@@ -47,6 +47,9 @@ void orc_lu_factor(int n, double* a, int32_t* piv, double* invD);
 void orc_lu_solve(int n, const double* a, const int32_t* piv,
                   const double* invD, double* b);
 
+/* Test hook: 1 = Rosenbrock stages with df/dt = 0 (textbook form). */
+void orc_set_textbook_rosenbrock(int on);
+
 /* Step log of the last single-cell integration (for controller checks). */
 typedef struct orc_step_rec {
     double x, dx, err;

@@ -55,6 +55,7 @@ def lib():
         L.orc_lu_solve.argtypes = [C.c_int, _dp, _ip, _dp, _dp]
         L.orc_integrate_cell.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_double, _dp,
                                          C.c_double, _dp, _ip, C.POINTER(StepRec), C.c_int32, _ip]
+        L.orc_set_textbook_rosenbrock.argtypes = [C.c_int]
         L.orc_solve.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, C.c_double,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
     return _lib
