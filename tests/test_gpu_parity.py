@@ -1,0 +1,201 @@
+"""Parity of the gfx950 library (through the C ABI) with the CPU oracle.
+
+Bar: per-species |dY| <= 1e-6 absolute or 1e-4 relative and |dT| <= 1e-3 K
+after solve(deltaT); kernel-level outputs at FP64 round-off scaled by the
+row magnitude.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fcs.mechanism as M
+from oracle import Oracle, lu_solve as orc_lu_solve
+
+pytestmark = pytest.mark.gpu
+
+DY_ABS, DY_REL, DT_ABS = 1e-6, 1e-4, 1e-3
+
+
+@pytest.fixture(scope="module")
+def fcslib():
+    from fcs import lib
+    lib.load()
+    assert lib.device_count() >= 1, "no GPU visible to the HIP runtime"
+    return lib
+
+
+def states(mech, cfg, n):
+    fld = M.config_field(cfg, mech, n_cells=max(n, 8))
+    YT = np.vstack([fld.Y[:, :n], fld.T[None, :n]])
+    return fld.p[:n], YT
+
+
+def end_state(res, field, dt):
+    return field.Y + res["RR"] * dt / res["rho0"][None, :]
+
+
+def assert_solve_parity(g, o, field, dt):
+    assert (g["status"] == o["status"]).all()
+    Yg, Yo = end_state(g, field, dt), end_state(o, field, dt)
+    dY = np.abs(Yg - Yo)
+    ok = (dY <= DY_ABS) | (dY <= DY_REL * np.abs(Yo))
+    assert ok.all(), f"max |dY| {dY.max():.3e} at {np.unravel_index(dY.argmax(), dY.shape)}"
+    assert np.abs(g["Tend"] - o["Tend"]).max() <= DT_ABS
+    np.testing.assert_allclose(g["deltaTChem"], o["deltaTChem"], rtol=0.2)
+    assert g["deltaTMin"] == pytest.approx(o["deltaTMin"], rel=0.2)
+
+
+# ------------------------------------------------------------ kernel level
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_derivatives(fcslib, cfg):
+    mech = M.config_mechanism(cfg)
+    o = Oracle(mech)
+    s = fcslib.Solver(fcslib.Mech(mech))
+    p, YT = states(mech, cfg, 64)
+    YT[3, :8] = -1e-3      # exercises the in-place clip
+    d, yc = s.derivatives(p, YT)
+    for c in range(64):
+        dref, yref = o.derivatives(p[c], YT[:, c])
+        np.testing.assert_array_equal(yc[:, c], yref)
+        scale = np.abs(dref[:-1]).max()
+        assert np.abs(d[:-1, c] - dref[:-1]).max() <= 1e-11 * scale
+        assert d[-1, c] == pytest.approx(dref[-1], rel=1e-10, abs=1e-11 * scale)
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+@pytest.mark.parametrize("jac", ["exact", "fast"])
+def test_jacobian(fcslib, cfg, jac):
+    mech = M.config_mechanism(cfg)
+    o = Oracle(mech)
+    s = fcslib.Solver(fcslib.Mech(mech), jacobian=jac)
+    p, YT = states(mech, cfg, 32)
+    f, J, _ = s.jacobian(p, YT)
+    for c in range(32):
+        fr, Jr, _ = o.jacobian(p[c], YT[:, c], jac)
+        np.testing.assert_allclose(f[:, c], fr, rtol=1e-10, atol=1e-11 * np.abs(fr[:-1]).max())
+        # the GPU uses the rank-1 form of the exact conversion: compare per row
+        scale = np.abs(Jr).max(axis=1, keepdims=True) + 1e-300
+        assert (np.abs(J[c] - Jr) / scale).max() <= 1e-9
+
+
+@pytest.mark.parametrize("n_species", [10, 53])
+def test_lu_against_reference_golden(fcslib, n_species):
+    """The device block-4 LU reproduces the reference LUsolver's solutions
+    (tests/golden/lu_golden.npz, reference code compiled as-is)."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "lu_golden.npz"))
+    mech = M.config_mechanism(0 if n_species == 10 else 1)
+    n = mech.n_species + 1
+    s = fcslib.Solver(fcslib.Mech(mech))
+    A, b, x = [], [], []
+    for i, sz in enumerate(g["sizes"]):
+        if sz == n and i % 4 != 3:
+            A.append(g["A"][g["moff"][i]:g["moff"][i + 1]].reshape(n, n))
+            b.append(g["b"][g["voff"][i]:g["voff"][i + 1]])
+            x.append(g["x"][g["voff"][i]:g["voff"][i + 1]])
+    assert A
+    xg = s.lu_solve(np.array(A), np.array(b))
+    for k in range(len(A)):
+        np.testing.assert_allclose(xg[k], x[k], rtol=0, atol=1e-10 * np.abs(x[k]).max())
+
+
+def test_lu_pivot_quirk_on_device(fcslib, mech0):
+    """Signed running maximum in the second column of a block: the device
+    takes the same row as the reference (solution equals the oracle's)."""
+    n = mech0.n_species + 1
+    s = fcslib.Solver(fcslib.Mech(mech0))
+    rng = np.random.default_rng(3)
+    A = np.eye(n) + 0.01 * rng.standard_normal((n, n))
+    A[1:4, 0] = 0.0
+    A[1, 1], A[2, 1], A[3, 1] = 1.0, -5.0, 2.0
+    b = rng.standard_normal(n)
+    x = s.lu_solve(A[None], b[None])[0]
+    xo, _, piv = orc_lu_solve(A, b)
+    assert piv[1] == 3
+    np.testing.assert_allclose(x, xo, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------ solve(deltaT)
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34", "OptSeulex"])
+def test_solve_config0_all_cells(fcslib, mech0, solver):
+    field = M.config_field(0, mech0)
+    dt = M.CONFIG_DT[0]
+    g = fcslib.Solver(fcslib.Mech(mech0), solver=solver).solve(field, dt)
+    o = Oracle(mech0).solve(field, dt, threads=1, solver=solver)
+    assert g["rc"] == 0 and o["rc"] == 0
+    assert_solve_parity(g, o, field, dt)
+
+
+def test_solve_config0_fast_jacobian(fcslib, mech0):
+    field = M.config_field(0, mech0, n_cells=256)
+    dt = M.CONFIG_DT[0]
+    g = fcslib.Solver(fcslib.Mech(mech0), jacobian="fast").solve(field, dt)
+    o = Oracle(mech0).solve(field, dt, threads=1, jacobian="fast")
+    assert_solve_parity(g, o, field, dt)
+
+
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptSeulex"])
+def test_solve_config1_sample(fcslib, mech1, solver):
+    field = M.config_field(1, mech1).subset(np.random.default_rng(1).choice(65536, 192, replace=False))
+    dt = M.CONFIG_DT[1]
+    g = fcslib.Solver(fcslib.Mech(mech1), solver=solver).solve(field, dt)
+    o = Oracle(mech1).solve(field, dt, threads=1, solver=solver)
+    assert g["rc"] == 0 and o["rc"] == 0
+    assert_solve_parity(g, o, field, dt)
+
+
+def test_full_size_invariants_and_determinism(fcslib, mech1):
+    """65,536 cells: every cell integrates, mass is conserved cell by cell,
+    and two runs give bit-identical results."""
+    field = M.config_field(1, mech1)
+    s = fcslib.Solver(fcslib.Mech(mech1))
+    b = fcslib.Batch(s, field)
+    b.run(1e-6)
+    r1 = b.download()
+    b.run(1e-6)
+    r2 = b.download()
+    assert (r1["status"] == 0).all()
+    for k in ("RR", "deltaTChem", "Tend", "nAttempts"):
+        np.testing.assert_array_equal(r1[k], r2[k])
+    assert np.abs(r1["RR"].sum(axis=0)).max() <= 1e-9 * np.abs(r1["RR"]).max()
+
+
+# ------------------------------------------------------------ edge cases
+def test_cold_cells_cap_and_batch_api(fcslib, mech0):
+    field = M.config_field(0, mech0, n_cells=128)
+    field.T[::3] = 450.0
+    dt = 1e-5
+    s = fcslib.Solver(fcslib.Mech(mech0), Treact=500.0, deltaTChemMax=2e-6)
+    g = s.solve(field, dt)
+    o = Oracle(mech0).solve(field, dt, threads=1, Treact=500.0, deltaTChemMax=2e-6)
+    assert_solve_parity(g, o, field, dt)
+    cold = field.T <= 500.0
+    assert (g["status"][cold] == 1).all() and (g["RR"][:, cold] == 0).all()
+    np.testing.assert_array_equal(g["deltaTChem"][cold], field.deltaTChem[cold])
+    assert (g["deltaTChem"][~cold] <= 2e-6).all()
+    b = fcslib.Batch(s, field)
+    b.run(dt)
+    r = b.download()
+    for k in ("RR", "deltaTChem", "Tend", "status"):
+        np.testing.assert_array_equal(r[k], g[k])
+    assert r["deltaTMin"] == g["deltaTMin"]
+
+
+def test_maxsteps_and_nonfinite_are_reported(fcslib, mech0):
+    from fcs.abi import FCS_E_CELLS
+    field = M.config_field(0, mech0, n_cells=8)
+    s = fcslib.Solver(fcslib.Mech(mech0), maxSteps=3)
+    g = s.solve(field, 1e-5)
+    assert g["rc"] == FCS_E_CELLS and (g["status"] & 2).all()
+    field2 = M.config_field(0, mech0, n_cells=8)
+    field2.Y[0, 2] = np.nan
+    g2 = fcslib.Solver(fcslib.Mech(mech0)).solve(field2, 1e-5)
+    assert g2["rc"] == FCS_E_CELLS and g2["status"][2] & 4
+    assert (g2["status"][[0, 1, 3, 4, 5, 6, 7]] == 0).all()
+
+
+def test_empty_batch(fcslib, mech0):
+    s = fcslib.Solver(fcslib.Mech(mech0))
+    field = M.config_field(0, mech0, n_cells=8).subset(np.arange(0))
+    g = s.solve(field, 1e-5)
+    assert g["rc"] == 0 and g["deltaTMin"] == 2e-5
