@@ -169,51 +169,63 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
     return error_norm<NP>(o, y0, yT, err);
 }
 
-// ODESolve + adaptiveSolve of the Rosenbrock solvers
-// (src/ODE/OptRodas34/OptRodas34.C:123-286).  Returns status bits.
+// Step-controller state of one cell between launches (flags bits).
+enum { STEP_LAST = 1, STEP_RETRY = 2 };
+// Return value of a step that leaves the cell in flight.
+constexpr int STEP_CONTINUE = -1;
+
+// One attempt of ODESolve + adaptiveSolve of the Rosenbrock solvers
+// (src/ODE/OptRodas34/OptRodas34.C:123-286), written as a resumable state
+// machine: the controller variables of the reference's loops (x, dxTry, the
+// trial step dx of a step being retried, dxTry0, last, the step and attempt
+// counters) are carried by the caller between calls.  Each call makes
+// exactly one attempt; it returns STEP_CONTINUE while the cell is in
+// flight, else its final status (0 when xEnd is reached).  Rejections are
+// capped at 100 * maxSteps attempts in total, a guard the reference does not
+// have (its rejection loop is unbounded).
 template <int NP>
-__device__ int rosenbrock_integrate(const DevMech& M, const Lds& L, const DevOpts& o, const Tableau& tb,
-                                    double p, double xEnd, double& y0, double& dxTryIO, int& nAtt)
+__device__ int rosenbrock_step(const DevMech& M, const Lds& L, const DevOpts& o, const Tableau& tb, double p,
+                               double xEnd, double& y0, double& x, double& dxTry, double& dx, double& dxTry0,
+                               int& flags, int& nStep, int& nAtt)
 {
-    const double lnp = log(p);
-    const double ratio = pow(MAX_SCALE / SAFE, -1.0 / ALPHA_INC);
-    const double safeMax = SAFE * MAX_SCALE;
     const double xStart = 0.0;
-    double x = xStart, dxTry = dxTryIO;
-    bool last = false;
-    double yT = 0.0;
-    const int attemptCap = 100 * o.maxSteps;
-    for (int nStep = 0; nStep < o.maxSteps; ++nStep) {
-        const double dxTry0 = dxTry;
-        if ((x + dxTry - xEnd) * (x + dxTry - xStart) > 0) { last = true; dxTry = xEnd - x; }
-        double dx = dxTry;
-        double Err;
-        for (;;) {
-            Err = rosenbrock_attempt<NP>(M, L, o, tb, p, lnp, dx, y0, yT);
-            ++nAtt;
-            if (!(Err == Err)) return CELL_NONFINITE;
-            if (!(Err > 1)) break;
-            if (nAtt > attemptCap) return CELL_MAXSTEPS;
-            const double sc = SAFE * pow(Err, -ALPHA_DEC);
-            dx *= sc > MIN_SCALE ? sc : MIN_SCALE;
+    if (!(flags & STEP_RETRY)) {
+        dxTry0 = dxTry;
+        if ((x + dxTry - xEnd) * (x + dxTry - xStart) > 0) {
+            flags |= STEP_LAST;
+            dxTry = xEnd - x;
         }
-        x += dx;
-        y0 = yT;
-        if (Err > ratio) {
-            double f = SAFE * pow(Err, -ALPHA_INC);
-            f = f > MIN_SCALE ? f : MIN_SCALE;
-            f = f < MAX_SCALE ? f : MAX_SCALE;
-            dxTry = f * dx;
-        } else {
-            dxTry = safeMax * dx;
-        }
-        if ((x - xEnd) * (xEnd - xStart) >= 0) {
-            if (nStep > 0 && last) dxTry = dxTry0;
-            dxTryIO = dxTry;
-            return 0;
-        }
+        dx = dxTry;
     }
-    return CELL_MAXSTEPS;
+    double yT = 0.0;
+    const double Err = rosenbrock_attempt<NP>(M, L, o, tb, p, log(p), dx, y0, yT);
+    ++nAtt;
+    if (!(Err == Err)) return CELL_NONFINITE;
+    if (Err > 1) {
+        if (nAtt > 100 * o.maxSteps) return CELL_MAXSTEPS;
+        const double sc = SAFE * pow(Err, -ALPHA_DEC);
+        dx *= sc > MIN_SCALE ? sc : MIN_SCALE;
+        flags |= STEP_RETRY;
+        return STEP_CONTINUE;
+    }
+    flags &= ~STEP_RETRY;
+    x += dx;
+    y0 = yT;
+    const double ratio = pow(MAX_SCALE / SAFE, -1.0 / ALPHA_INC);
+    if (Err > ratio) {
+        double f = SAFE * pow(Err, -ALPHA_INC);
+        f = f > MIN_SCALE ? f : MIN_SCALE;
+        f = f < MAX_SCALE ? f : MAX_SCALE;
+        dxTry = f * dx;
+    } else {
+        dxTry = SAFE * MAX_SCALE * dx;
+    }
+    if ((x - xEnd) * (xEnd - xStart) >= 0) {
+        if (nStep > 0 && (flags & STEP_LAST)) dxTry = dxTry0;
+        return 0;
+    }
+    if (++nStep >= o.maxSteps) return CELL_MAXSTEPS;
+    return STEP_CONTINUE;
 }
 
 // ---------------------------------------------------------------- Seulex

@@ -18,73 +18,131 @@ using namespace fcs;
 #define FCS_FOR_EACH_NP(X) X(12) X(56)
 
 // ============================================================== kernels
-// Persistent cell queue: every wavefront takes the next cell index from an
-// atomic counter until the batch is drained, so cells with very different
-// sub-step counts balance across the chip without host involvement.  This
-// replaces the reference's MPI load balancer
-// (src/ChemistryModel/FastChemistryModel_transientSolve.H:106-686).
+// Device arrays of one batch of cells: the flow-step inputs and outputs of
+// FastChemistryModel::solve (src/ChemistryModel/
+// FastChemistryModel_transientSolve.H:3-104) and the integrator state that
+// each cell carries between step launches.  All per-cell arrays are SoA over
+// cells; Y0/RR/Yw are [rows][nc] (Yw has the temperature as row ns).
+struct CellArrays {
+    int nc;
+    double dt;                       // flow time step deltaT
+    const double *T0, *p0, *rho0, *Y0, *dtcIn;
+    double *dtcOut, *RR, *Tend;
+    int *status, *nAtt;
+    unsigned long long* dtMinBits;   // min over cells of the new deltaTChem (bits)
+    // integrator state
+    double *Yw, *x, *dxTry, *dx, *dxTry0;
+    int *flags, *nStep;
+};
+
+// Clears the active-list counters and the deltaTMin accumulator (a kernel
+// rather than memset nodes, so the sequence can be captured in a graph).
+__global__ void k_reset(unsigned int* counts, unsigned long long* dtMinBits)
+{
+    counts[0] = counts[1] = counts[2] = 0u;
+    *dtMinBits = 0x7ff0000000000000ull;  // +inf
+}
+
+// One thread per cell: cold cells are finished at once
+// (FastChemistryModel_transientSolve.H:70-76); reacting cells get their
+// initial state y = [max(0, Y), T] (src/ODE/OptRodas34/OptRodas34.C:72-81)
+// and join the active list.
+__global__ __launch_bounds__(256) void k_setup(DevOpts o, int ns, CellArrays a, int* list, unsigned int* count)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    const size_t nc = (size_t)a.nc;
+    const double T = a.T0[c];
+    if (!(T > o.Treact)) {
+        for (int i = 0; i < ns; ++i) a.RR[i * nc + c] = 0.0;
+        a.status[c] = CELL_SKIPPED;
+        a.dtcOut[c] = a.dtcIn[c];
+        a.Tend[c] = T;
+        a.nAtt[c] = 0;
+        return;
+    }
+    for (int i = 0; i < ns; ++i) {
+        const double Y = a.Y0[i * nc + c];
+        a.Yw[i * nc + c] = Y > 0.0 ? Y : 0.0;
+    }
+    a.Yw[ns * nc + c] = T;
+    a.x[c] = 0.0;
+    a.dxTry[c] = a.dtcIn[c];
+    a.dx[c] = 0.0;
+    a.dxTry0[c] = 0.0;
+    a.flags[c] = 0;
+    a.nStep[c] = 0;
+    a.nAtt[c] = 0;
+    list[atomicAdd(count, 1u)] = c;
+}
+
+// Writes the flow-step results of a finished cell (OptRodas34.C:114-117,
+// FastChemistryModel_transientSolve.H:59-67): RR = (max(0, y) rho0 - Y0 rho0)
+// / deltaT, the new deltaTChem (the input one when the integration failed)
+// capped by deltaTChemMax, and the deltaTMin reduction.
+template <int NP>
+__device__ void finish_cell(const DevMech& M, const DevOpts& o, const CellArrays& a, int c, int st, double y,
+                            double dxTry, int nAtt)
+{
+    const int lane = lane_id();
+    const size_t nc = (size_t)a.nc;
+    if (wave_any(!(y == y) || isinf(y))) st |= CELL_NONFINITE;
+    if (lane < M.ns) {
+        const double Yn = y > 0.0 ? y : 0.0;
+        const double r0 = a.rho0[c];
+        a.RR[lane * nc + c] = (Yn * r0 - a.Y0[lane * nc + c] * r0) / a.dt;
+    }
+    if (lane == NP - 1) a.Tend[c] = y;
+    if (lane == 0) {
+        const double dtc = (st & (CELL_MAXSTEPS | CELL_NONFINITE)) ? a.dtcIn[c] : dxTry;
+        a.status[c] = st;
+        a.nAtt[c] = nAtt;
+        a.dtcOut[c] = dtc < o.dtChemMax ? dtc : o.dtChemMax;
+        if (dtc > 0.0) atomicMin(a.dtMinBits, (unsigned long long)__double_as_longlong(dtc));
+    }
+}
+
+// One Rosenbrock attempt for every cell of the active list (one wavefront
+// per cell, cells strided over the grid).  Cells still in flight are
+// appended to the next list; finished cells write their results.  The
+// counter *countZero (the list after next) is cleared for the next launch.
+// Every loop here is bounded by the list length read at entry.
 template <int NP, int SOLVER>
-__global__ __launch_bounds__(64) void k_integrate(DevMech M, DevOpts o, int nc, double dt, const double* T0,
-                                                  const double* p0, const double* rho0, const double* Y0,
-                                                  const double* dtcIn, double* dtcOut, double* RR,
-                                                  int* status, double* Tend, int* nAttOut,
-                                                  unsigned long long* dtMinBits, unsigned int* counter)
+__global__ __launch_bounds__(64) void k_step(DevMech M, DevOpts o, CellArrays a, const int* listIn,
+                                             const unsigned int* countIn, int* listOut, unsigned int* countOut,
+                                             unsigned int* countZero)
 {
     extern __shared__ double lds_raw[];
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
     const int lane = lane_id();
-    const bool sp = lane < M.ns;
-    for (;;) {
-        unsigned int c = 0;
-        if (lane == 0) c = atomicAdd(counter, 1u);
-        c = __shfl(c, 0, WAVE);
-        if (c >= (unsigned int)nc) break;
-        const double T = T0[c];
-        double Yin = 0.0;
-        if (sp) Yin = Y0[(size_t)lane * nc + c];
-        if (!(T > o.Treact)) {
-            // src/ChemistryModel/FastChemistryModel_transientSolve.H:70-76
-            if (sp) RR[(size_t)lane * nc + c] = 0.0;
+    const int ns = M.ns;
+    const size_t nc = (size_t)a.nc;
+    const int row = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
+    const unsigned int n = *countIn;
+    if (blockIdx.x == 0 && lane == 0) *countZero = 0u;
+    for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int c = listIn[i];
+        double y = row >= 0 ? a.Yw[row * nc + c] : 0.0;
+        double x = a.x[c], dxTry = a.dxTry[c], dx = a.dx[c], dxTry0 = a.dxTry0[c];
+        int flags = a.flags[c], nStep = a.nStep[c], nAtt = a.nAtt[c];
+        const int st = rosenbrock_step<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, a.p0[c], a.dt, y,
+                                           x, dxTry, dx, dxTry0, flags, nStep, nAtt);
+        if (st == STEP_CONTINUE) {
+            if (row >= 0) a.Yw[row * nc + c] = y;
             if (lane == 0) {
-                status[c] = CELL_SKIPPED;
-                dtcOut[c] = dtcIn[c];
-                if (Tend) Tend[c] = T;
-                if (nAttOut) nAttOut[c] = 0;
+                a.x[c] = x;
+                a.dxTry[c] = dxTry;
+                a.dx[c] = dx;
+                a.dxTry0[c] = dxTry0;
+                a.flags[c] = flags;
+                a.nStep[c] = nStep;
+                a.nAtt[c] = nAtt;
+                listOut[atomicAdd(countOut, 1u)] = c;
             }
-            continue;
-        }
-        const double p = p0[c];
-        // y00 = [max(0, Y), T] (src/ODE/OptRodas34/OptRodas34.C:72-81)
-        double y = sp ? (Yin > 0.0 ? Yin : 0.0) : (lane == NP - 1 ? T : 0.0);
-        double dxTry = dtcIn[c];
-        int nAtt = 0;
-        int st;
-        if (SOLVER == SOLVER_SEULEX) st = seulex_integrate<NP>(M, L, o, p, dt, y, dxTry, nAtt);
-        else st = rosenbrock_integrate<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, p, dt, y, dxTry,
-                                          nAtt);
-        if (wave_any(!(y == y) || isinf(y))) st |= CELL_NONFINITE;
-        // writeback (OptRodas34.C:114-117, transientSolve.H:59-67)
-        if (sp) {
-            const double Yn = y > 0.0 ? y : 0.0;
-            const double r0 = rho0[c];
-            RR[(size_t)lane * nc + c] = (Yn * r0 - Yin * r0) / dt;
-        }
-        if (lane == NP - 1 && Tend) Tend[c] = y;
-        if (lane == 0) {
-            status[c] = st;
-            if (nAttOut) nAttOut[c] = nAtt;
-            dtcOut[c] = dxTry < o.dtChemMax ? dxTry : o.dtChemMax;
-            if (dxTry > 0.0) atomicMin(dtMinBits, (unsigned long long)__double_as_longlong(dxTry));
+        } else {
+            finish_cell<NP>(M, o, a, c, st, y, dxTry, nAtt);
         }
     }
-}
-
-// Resets the queue counter and the deltaTMin accumulator on the stream
-// (a kernel rather than memset/memcpy nodes, so the sequence can be captured).
-__global__ void k_queue_init(unsigned int* counter, unsigned long long* dtMinBits)
-{
-    *counter = 0u;
-    *dtMinBits = 0x7ff0000000000000ull;  // +inf
 }
 
 // One wavefront per state: derivatives.  External layout [n][nS] with T at
@@ -127,6 +185,30 @@ __global__ __launch_bounds__(64) void k_jacobian(DevMech M, int jacType, int nS,
     }
 }
 
+// One Rosenbrock attempt per state (OptRodas34::Rodas34Solve,
+// src/ODE/OptRodas34/OptRodas34.C:291-879, or
+// OptRosenbrock34::Rosenbrock34Solve, src/ODE/OptRosenbrock34/
+// OptRosenbrock34.C:281-677).  External layout [n][nS], T at row ns; the
+// start state is clipped in place as the Jacobian evaluation does.
+template <int NP, int SOLVER>
+__global__ __launch_bounds__(64) void k_attempt(DevMech M, DevOpts o, int nS, const double* p, const double* dx,
+                                                double* YT, double* YTnew, double* err)
+{
+    extern __shared__ double lds_raw[];
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const int s = blockIdx.x, lane = lane_id(), ns = M.ns;
+    const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
+    double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
+    double yT = 0.0;
+    const double e = rosenbrock_attempt<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, p[s], log(p[s]),
+                                            dx[s], y, yT);
+    if (ext >= 0) {
+        YT[(size_t)ext * nS + s] = y;
+        YTnew[(size_t)ext * nS + s] = yT;
+    }
+    if (lane == 0) err[s] = e;
+}
+
 template <int NP>
 __global__ __launch_bounds__(64) void k_lu_solve(DevMech M, int nSys, const double* A, double* b)
 {
@@ -150,17 +232,18 @@ namespace {
 
 struct KernelSet {
     int NP;
-    const void* integrate[3];   // by FCS_SOLVER_*
+    const void* step[2];        // Rodas34, Rosenbrock34
     const void* deriv;
     const void* jac;
     const void* lu;
+    const void* attempt[2];     // Rodas34, Rosenbrock34
 };
 
 #define FCS_KSET(N)                                                                                        \
     {N,                                                                                                    \
-     {(const void*)k_integrate<N, SOLVER_RODAS34>, (const void*)k_integrate<N, SOLVER_ROS34>,              \
-      (const void*)k_integrate<N, SOLVER_SEULEX>},                                                         \
-     (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>},
+     {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>},                       \
+     (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>,                \
+     {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>}},
 const KernelSet kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET)};
 #undef FCS_KSET
 
@@ -205,7 +288,78 @@ struct fcs_mech {
     std::vector<int> perm;   // device order -> file order
 };
 
-struct fcs_batch;
+// Device buffers of one batch of cells (one allocation) plus the pinned
+// host words and events of the termination poll.
+struct DevCells {
+    static constexpr int kPoll = 4;   // step launches in flight ahead of the poll
+    size_t cap = 0;
+    int ns = 0;
+    void* base = nullptr;
+    double *T0 = nullptr, *p0 = nullptr, *rho0 = nullptr, *Y0 = nullptr, *dtcIn = nullptr, *dtcOut = nullptr,
+           *RR = nullptr, *Tend = nullptr, *Yw = nullptr, *x = nullptr, *dxTry = nullptr, *dx = nullptr,
+           *dxTry0 = nullptr;
+    int *status = nullptr, *nAtt = nullptr, *flags = nullptr, *nStep = nullptr, *list0 = nullptr, *list1 = nullptr;
+    unsigned int* counts = nullptr;          // [3] active-list lengths
+    unsigned long long* dtMin = nullptr;
+    unsigned int* hostCount = nullptr;       // pinned [kPoll]
+    hipEvent_t ev[kPoll] = {};
+
+    int alloc(size_t nc, int nsp)
+    {
+        release();
+        const size_t d = align256(nc * 8), i = align256(nc * 4);
+        const size_t nsd = align256(nc * nsp * 8), nwd = align256(nc * (nsp + 1) * 8);
+        const size_t total = 10 * d + 2 * nsd + nwd + 6 * i + 256 + 256;
+        if (hipMalloc(&base, total) != hipSuccess) {
+            base = nullptr;
+            return FCS_E_NOMEM;
+        }
+        char* q = (char*)base;
+        auto take = [&](size_t bytes) { char* r = q; q += bytes; return r; };
+        T0 = (double*)take(d); p0 = (double*)take(d); rho0 = (double*)take(d); dtcIn = (double*)take(d);
+        dtcOut = (double*)take(d); Tend = (double*)take(d); x = (double*)take(d); dxTry = (double*)take(d);
+        dx = (double*)take(d); dxTry0 = (double*)take(d);
+        Y0 = (double*)take(nsd); RR = (double*)take(nsd); Yw = (double*)take(nwd);
+        status = (int*)take(i); nAtt = (int*)take(i); flags = (int*)take(i); nStep = (int*)take(i);
+        list0 = (int*)take(i); list1 = (int*)take(i);
+        counts = (unsigned int*)take(256);
+        dtMin = (unsigned long long*)take(256);
+        if (hipHostMalloc((void**)&hostCount, kPoll * sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
+            hostCount = nullptr;
+            release();
+            return FCS_E_NOMEM;
+        }
+        for (auto& e : ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                e = nullptr;
+                release();
+                return FCS_E_HIP;
+            }
+        cap = nc;
+        ns = nsp;
+        return FCS_OK;
+    }
+    void release()
+    {
+        if (base) (void)hipFree(base);
+        if (hostCount) (void)hipHostFree(hostCount);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+        base = nullptr;
+        hostCount = nullptr;
+        cap = 0;
+    }
+    CellArrays view(int nc, double dt) const
+    {
+        CellArrays a;
+        a.nc = nc;
+        a.dt = dt;
+        a.T0 = T0; a.p0 = p0; a.rho0 = rho0; a.Y0 = Y0; a.dtcIn = dtcIn;
+        a.dtcOut = dtcOut; a.RR = RR; a.Tend = Tend; a.status = status; a.nAtt = nAtt; a.dtMinBits = dtMin;
+        a.Yw = Yw; a.x = x; a.dxTry = dxTry; a.dx = dx; a.dxTry0 = dxTry0; a.flags = flags; a.nStep = nStep;
+        return a;
+    }
+};
 
 struct fcs_solver {
     const fcs_mech* mech = nullptr;
@@ -214,23 +368,13 @@ struct fcs_solver {
     size_t ldsBytes = 0;
     int grid = 0;
     hipStream_t stream = nullptr;
-    // scratch buffers of fcs_solve
-    size_t cap = 0;
-    double *T0 = nullptr, *p0 = nullptr, *rho0 = nullptr, *Y0 = nullptr, *dtcIn = nullptr, *dtcOut = nullptr,
-           *RR = nullptr, *Tend = nullptr;
-    int *status = nullptr, *nAtt = nullptr;
-    unsigned long long* dtMin = nullptr;
-    unsigned int* counter = nullptr;
+    DevCells cells;   // scratch of fcs_solve
 };
 
 struct fcs_batch {
     fcs_solver* s = nullptr;
     int64_t nc = 0;
-    double *T0 = nullptr, *p0 = nullptr, *rho0 = nullptr, *Y0 = nullptr, *dtcIn = nullptr, *dtcOut = nullptr,
-           *RR = nullptr, *Tend = nullptr;
-    int *status = nullptr, *nAtt = nullptr;
-    unsigned long long* dtMin = nullptr;
-    unsigned int* counter = nullptr;
+    DevCells cells;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double lastDt = 0.0;
 };
@@ -457,8 +601,8 @@ static int solver_grid(fcs_solver* s)
     int dev = 0, cus = 0, per = 0;
     HIP_OK(hipGetDevice(&dev));
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, s->ks->integrate[s->opts.solver], 64,
-                                                        s->ldsBytes));
+    const int k = s->opts.solver == FCS_SOLVER_ROSENBROCK34 ? 1 : 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, s->ks->step[k], 64, s->ldsBytes));
     s->grid = std::max(1, per) * std::max(1, cus);
     return FCS_OK;
 }
@@ -467,7 +611,7 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
 {
     if (!mech || !opts || !out) return FCS_E_ARG;
     if (opts->solver < 0 || opts->solver > 2 || opts->jacobian < 0 || opts->jacobian > 1 || opts->maxSteps < 1 ||
-        !(opts->absTol > 0) || !(opts->relTol > 0))
+        opts->maxSteps > 10000000 || !(opts->absTol > 0) || !(opts->relTol > 0))
         return FCS_E_ARG;
     const KernelSet* ks = pick_kernels(mech->ns + 1);
     if (!ks) return FCS_E_UNSUPPORTED;
@@ -482,7 +626,7 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
         return FCS_E_HIP;
     }
     if (s->ldsBytes > 65536)
-        for (const void* f : {ks->integrate[0], ks->integrate[1], ks->integrate[2], ks->deriv, ks->jac, ks->lu})
+        for (const void* f : {ks->step[0], ks->step[1], ks->deriv, ks->jac, ks->lu, ks->attempt[0], ks->attempt[1]})
             (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->ldsBytes);
     if (solver_grid(s) != FCS_OK) {
         (void)hipStreamDestroy(s->stream);
@@ -493,23 +637,10 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
     return FCS_OK;
 }
 
-static void free_scratch(fcs_solver* s)
-{
-    for (void* p : {(void*)s->T0, (void*)s->p0, (void*)s->rho0, (void*)s->Y0, (void*)s->dtcIn, (void*)s->dtcOut,
-                    (void*)s->RR, (void*)s->Tend, (void*)s->status, (void*)s->nAtt, (void*)s->dtMin,
-                    (void*)s->counter})
-        if (p) (void)hipFree(p);
-    s->T0 = s->p0 = s->rho0 = s->Y0 = s->dtcIn = s->dtcOut = s->RR = s->Tend = nullptr;
-    s->status = s->nAtt = nullptr;
-    s->dtMin = nullptr;
-    s->counter = nullptr;
-    s->cap = 0;
-}
-
 int fcs_solver_destroy(fcs_solver* s)
 {
     if (!s) return FCS_E_ARG;
-    free_scratch(s);
+    s->cells.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return FCS_OK;
@@ -528,22 +659,54 @@ static DevOpts dev_opts(const fcs_solver_opts& o)
     return d;
 }
 
-// Launches the queue kernel on stream st over device arrays.
-static int launch_integrate(fcs_solver* s, int nc, double dt, const double* T0, const double* p0,
-                            const double* rho0, const double* Y0, const double* dtcIn, double* dtcOut, double* RR,
-                            int* status, double* Tend, int* nAtt, unsigned long long* dtMin, unsigned int* counter)
+// FastChemistryModel::solve(deltaT) over the nc cells held in d (inputs
+// already on the device): setup, then one attempt per active cell per step
+// launch until the active list is empty.  The list length after each launch
+// is copied to pinned memory behind an event; the host waits only on the
+// launch kPoll-1 places back, so the GPU always has step launches queued.
+// Every cell finishes within 101 * maxSteps attempts (maxSteps accepted
+// steps plus the rejection cap), which bounds the launch count.
+static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt)
 {
-    hipLaunchKernelGGL(k_queue_init, dim3(1), dim3(1), 0, s->stream, counter, dtMin);
-    HIP_OK(hipGetLastError());
+    if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
+    hipStream_t st = s->stream;
+    CellArrays a = d.view(nc, dt);
     DevMech M = s->mech->dm;
     DevOpts o = dev_opts(s->opts);
-    void* args[] = {&M, &o, &nc, &dt, &T0, &p0, &rho0, &Y0, &dtcIn, &dtcOut, &RR, &status, &Tend, &nAtt, &dtMin,
-                    &counter};
-    const int grid = std::min(s->grid, std::max(nc, 1));
-    HIP_OK(hipLaunchKernel(s->ks->integrate[s->opts.solver], dim3(grid), dim3(64), args, s->ldsBytes, s->stream));
+    int ns = s->mech->ns;
+    {
+        void* args[] = {&d.counts, &d.dtMin};
+        HIP_OK(hipLaunchKernel((const void*)k_reset, dim3(1), dim3(1), args, 0, st));
+    }
+    if (nc > 0) {
+        unsigned int* c0 = d.counts;
+        void* args[] = {&o, &ns, &a, &d.list0, &c0};
+        HIP_OK(hipLaunchKernel((const void*)k_setup, dim3((nc + 255) / 256), dim3(256), args, 0, st));
+    }
+    const void* f = s->ks->step[s->opts.solver == FCS_SOLVER_ROSENBROCK34 ? 1 : 0];
+    const int grid = std::max(1, std::min(s->grid, nc));
+    const long long maxLaunches = 101LL * s->opts.maxSteps + 2 * DevCells::kPoll + 2;
+    for (long long it = 0;; ++it) {
+        int* lin = (it & 1) ? d.list1 : d.list0;
+        int* lout = (it & 1) ? d.list0 : d.list1;
+        unsigned int* cin = d.counts + it % 3;
+        unsigned int* cout = d.counts + (it + 1) % 3;
+        unsigned int* czero = d.counts + (it + 2) % 3;
+        void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
+        HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st));
+        const int slot = (int)(it % DevCells::kPoll);
+        HIP_OK(hipMemcpyAsync(d.hostCount + slot, cout, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipEventRecord(d.ev[slot], st));
+        const long long back = it - (DevCells::kPoll - 1);
+        if (back >= 0) {
+            const int bs = (int)(back % DevCells::kPoll);
+            HIP_OK(hipEventSynchronize(d.ev[bs]));
+            if (d.hostCount[bs] == 0u) break;
+        }
+        if (it > maxLaunches) return FCS_E_HIP;
+    }
     return FCS_OK;
 }
-
 
 int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0, const double* p0,
                  const double* rho0, const double* const* Y0, double* deltaTChem, double* const* RR,
@@ -552,51 +715,42 @@ int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0,
     if (!s || nCells < 0 || nCells > 0x7fffffff || !(deltaT > 0) || (nCells > 0 && (!T0 || !p0 || !rho0 || !Y0 ||
                                                                                    !deltaTChem || !RR)))
         return FCS_E_ARG;
+    if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
     const int ns = s->mech->ns;
     const size_t nc = (size_t)nCells;
     if (nc == 0) {
         if (deltaTMin) *deltaTMin = std::min(OF_GREAT, 2 * deltaT);
         return FCS_OK;
     }
-    if (nc > s->cap) {
-        free_scratch(s);
-        int rc = FCS_OK;
-        rc |= grow(s->T0, nc) | grow(s->p0, nc) | grow(s->rho0, nc) | grow(s->Y0, nc * ns) | grow(s->dtcIn, nc) |
-              grow(s->dtcOut, nc) | grow(s->RR, nc * ns) | grow(s->Tend, nc) | grow(s->status, nc) |
-              grow(s->nAtt, nc) | grow(s->dtMin, 1) | grow(s->counter, 1);
-        if (rc != FCS_OK) {
-            free_scratch(s);
-            return FCS_E_NOMEM;
-        }
-        s->cap = nc;
-    }
-    HIP_OK(hipMemcpyAsync(s->T0, T0, nc * 8, hipMemcpyHostToDevice, s->stream));
-    HIP_OK(hipMemcpyAsync(s->p0, p0, nc * 8, hipMemcpyHostToDevice, s->stream));
-    HIP_OK(hipMemcpyAsync(s->rho0, rho0, nc * 8, hipMemcpyHostToDevice, s->stream));
-    HIP_OK(hipMemcpyAsync(s->dtcIn, deltaTChem, nc * 8, hipMemcpyHostToDevice, s->stream));
+    DevCells& d = s->cells;
+    if (nc > d.cap && d.alloc(nc, ns) != FCS_OK) return FCS_E_NOMEM;
+    hipStream_t st = s->stream;
+    HIP_OK(hipMemcpyAsync(d.T0, T0, nc * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.p0, p0, nc * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.rho0, rho0, nc * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.dtcIn, deltaTChem, nc * 8, hipMemcpyHostToDevice, st));
     for (int i = 0; i < ns; ++i)
-        HIP_OK(hipMemcpyAsync(s->Y0 + (size_t)i * nc, Y0[i], nc * 8, hipMemcpyHostToDevice, s->stream));
-    int rc = launch_integrate(s, (int)nc, deltaT, s->T0, s->p0, s->rho0, s->Y0, s->dtcIn, s->dtcOut, s->RR,
-                              s->status, s->Tend, s->nAtt, s->dtMin, s->counter);
+        HIP_OK(hipMemcpyAsync(d.Y0 + (size_t)i * nc, Y0[i], nc * 8, hipMemcpyHostToDevice, st));
+    int rc = run_cells(s, d, (int)nc, deltaT);
     if (rc != FCS_OK) return rc;
     for (int i = 0; i < ns; ++i)
-        HIP_OK(hipMemcpyAsync(RR[i], s->RR + (size_t)i * nc, nc * 8, hipMemcpyDeviceToHost, s->stream));
-    HIP_OK(hipMemcpyAsync(deltaTChem, s->dtcOut, nc * 8, hipMemcpyDeviceToHost, s->stream));
-    std::vector<int> st(nc);
-    HIP_OK(hipMemcpyAsync(st.data(), s->status, nc * 4, hipMemcpyDeviceToHost, s->stream));
-    if (Tend) HIP_OK(hipMemcpyAsync(Tend, s->Tend, nc * 8, hipMemcpyDeviceToHost, s->stream));
-    if (nAttempts) HIP_OK(hipMemcpyAsync(nAttempts, s->nAtt, nc * 4, hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipMemcpyAsync(RR[i], d.RR + (size_t)i * nc, nc * 8, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(deltaTChem, d.dtcOut, nc * 8, hipMemcpyDeviceToHost, st));
+    std::vector<int> stat(nc);
+    HIP_OK(hipMemcpyAsync(stat.data(), d.status, nc * 4, hipMemcpyDeviceToHost, st));
+    if (Tend) HIP_OK(hipMemcpyAsync(Tend, d.Tend, nc * 8, hipMemcpyDeviceToHost, st));
+    if (nAttempts) HIP_OK(hipMemcpyAsync(nAttempts, d.nAtt, nc * 4, hipMemcpyDeviceToHost, st));
     unsigned long long bits = 0;
-    HIP_OK(hipMemcpyAsync(&bits, s->dtMin, 8, hipMemcpyDeviceToHost, s->stream));
-    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipMemcpyAsync(&bits, d.dtMin, 8, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
     double dmin;
     std::memcpy(&dmin, &bits, 8);
     dmin = std::min(dmin, OF_GREAT);
     if (deltaTMin) *deltaTMin = std::min(dmin, 2 * deltaT);
     bool bad = false;
     for (size_t c = 0; c < nc; ++c) {
-        if (cellStatus) cellStatus[c] = st[c];
-        if (st[c] & (CELL_MAXSTEPS | CELL_NONFINITE)) bad = true;
+        if (cellStatus) cellStatus[c] = stat[c];
+        if (stat[c] & (CELL_MAXSTEPS | CELL_NONFINITE)) bad = true;
     }
     return bad ? FCS_E_CELLS : FCS_OK;
 }
@@ -612,15 +766,13 @@ int fcs_solve(fcs_solver* s, int64_t nCells, double deltaT, const double* T0, co
 int fcs_batch_create(fcs_solver* s, int64_t nCells, fcs_batch** out)
 {
     if (!s || !out || nCells < 1 || nCells > 0x7fffffff) return FCS_E_ARG;
+    if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
     fcs_batch* b = new (std::nothrow) fcs_batch();
     if (!b) return FCS_E_NOMEM;
     b->s = s;
     b->nc = nCells;
-    const size_t nc = (size_t)nCells, ns = (size_t)s->mech->ns;
-    int rc = grow(b->T0, nc) | grow(b->p0, nc) | grow(b->rho0, nc) | grow(b->Y0, nc * ns) | grow(b->dtcIn, nc) |
-             grow(b->dtcOut, nc) | grow(b->RR, nc * ns) | grow(b->Tend, nc) | grow(b->status, nc) |
-             grow(b->nAtt, nc) | grow(b->dtMin, 1) | grow(b->counter, 1);
-    if (rc != FCS_OK || hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+    if (b->cells.alloc((size_t)nCells, s->mech->ns) != FCS_OK || hipEventCreate(&b->e0) != hipSuccess ||
+        hipEventCreate(&b->e1) != hipSuccess) {
         fcs_batch_destroy(b);
         return FCS_E_NOMEM;
     }
@@ -631,10 +783,7 @@ int fcs_batch_create(fcs_solver* s, int64_t nCells, fcs_batch** out)
 int fcs_batch_destroy(fcs_batch* b)
 {
     if (!b) return FCS_E_ARG;
-    for (void* p : {(void*)b->T0, (void*)b->p0, (void*)b->rho0, (void*)b->Y0, (void*)b->dtcIn, (void*)b->dtcOut,
-                    (void*)b->RR, (void*)b->Tend, (void*)b->status, (void*)b->nAtt, (void*)b->dtMin,
-                    (void*)b->counter})
-        if (p) (void)hipFree(p);
+    b->cells.release();
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     delete b;
@@ -647,11 +796,12 @@ int fcs_batch_upload(fcs_batch* b, const double* T0, const double* p0, const dou
     if (!b || !T0 || !p0 || !rho0 || !Y0 || !deltaTChem) return FCS_E_ARG;
     const size_t nc = (size_t)b->nc, ns = (size_t)b->s->mech->ns;
     hipStream_t st = b->s->stream;
-    HIP_OK(hipMemcpyAsync(b->T0, T0, nc * 8, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(b->p0, p0, nc * 8, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(b->rho0, rho0, nc * 8, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(b->Y0, Y0, nc * ns * 8, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(b->dtcIn, deltaTChem, nc * 8, hipMemcpyHostToDevice, st));
+    DevCells& d = b->cells;
+    HIP_OK(hipMemcpyAsync(d.T0, T0, nc * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.p0, p0, nc * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.rho0, rho0, nc * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.Y0, Y0, nc * ns * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d.dtcIn, deltaTChem, nc * 8, hipMemcpyHostToDevice, st));
     HIP_OK(hipStreamSynchronize(st));
     return FCS_OK;
 }
@@ -662,14 +812,11 @@ int fcs_batch_run(fcs_batch* b, double deltaT, float* elapsed_ms)
     fcs_solver* s = b->s;
     b->lastDt = deltaT;
     HIP_OK(hipEventRecord(b->e0, s->stream));
-    int rc = launch_integrate(s, (int)b->nc, deltaT, b->T0, b->p0, b->rho0, b->Y0, b->dtcIn, b->dtcOut, b->RR,
-                              b->status, b->Tend, b->nAtt, b->dtMin, b->counter);
+    int rc = run_cells(s, b->cells, (int)b->nc, deltaT);
     if (rc != FCS_OK) return rc;
     HIP_OK(hipEventRecord(b->e1, s->stream));
-    if (elapsed_ms) {
-        HIP_OK(hipEventSynchronize(b->e1));
-        HIP_OK(hipEventElapsedTime(elapsed_ms, b->e0, b->e1));
-    }
+    HIP_OK(hipEventSynchronize(b->e1));
+    if (elapsed_ms) HIP_OK(hipEventElapsedTime(elapsed_ms, b->e0, b->e1));
     return FCS_OK;
 }
 
@@ -679,13 +826,14 @@ int fcs_batch_download(fcs_batch* b, double* RR, double* deltaTChem, double* del
     if (!b) return FCS_E_ARG;
     const size_t nc = (size_t)b->nc, ns = (size_t)b->s->mech->ns;
     hipStream_t st = b->s->stream;
-    if (RR) HIP_OK(hipMemcpyAsync(RR, b->RR, nc * ns * 8, hipMemcpyDeviceToHost, st));
-    if (deltaTChem) HIP_OK(hipMemcpyAsync(deltaTChem, b->dtcOut, nc * 8, hipMemcpyDeviceToHost, st));
-    if (cellStatus) HIP_OK(hipMemcpyAsync(cellStatus, b->status, nc * 4, hipMemcpyDeviceToHost, st));
-    if (Tend) HIP_OK(hipMemcpyAsync(Tend, b->Tend, nc * 8, hipMemcpyDeviceToHost, st));
-    if (nAttempts) HIP_OK(hipMemcpyAsync(nAttempts, b->nAtt, nc * 4, hipMemcpyDeviceToHost, st));
+    DevCells& d = b->cells;
+    if (RR) HIP_OK(hipMemcpyAsync(RR, d.RR, nc * ns * 8, hipMemcpyDeviceToHost, st));
+    if (deltaTChem) HIP_OK(hipMemcpyAsync(deltaTChem, d.dtcOut, nc * 8, hipMemcpyDeviceToHost, st));
+    if (cellStatus) HIP_OK(hipMemcpyAsync(cellStatus, d.status, nc * 4, hipMemcpyDeviceToHost, st));
+    if (Tend) HIP_OK(hipMemcpyAsync(Tend, d.Tend, nc * 8, hipMemcpyDeviceToHost, st));
+    if (nAttempts) HIP_OK(hipMemcpyAsync(nAttempts, d.nAtt, nc * 4, hipMemcpyDeviceToHost, st));
     unsigned long long bits = 0;
-    HIP_OK(hipMemcpyAsync(&bits, b->dtMin, 8, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&bits, d.dtMin, 8, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (deltaTMin) {
         double dmin;
@@ -740,6 +888,36 @@ int fcs_eval_jacobian(fcs_solver* s, int64_t nS, const double* p, double* YT, do
         HIP_OK(hipMemcpy(YT, dy, nv * 8, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(dYTdt, df, nv * 8, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(Jy, dJ, nm * 8, hipMemcpyDeviceToHost));
+        return FCS_OK;
+    });
+}
+
+int fcs_eval_attempt(fcs_solver* s, int64_t nS, const double* p, const double* dx, double* YT, double* YTnew,
+                     double* err)
+{
+    if (!s || nS < 1 || nS > 0x7fffffff || !p || !dx || !YT || !YTnew || !err) return FCS_E_ARG;
+    if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
+    const int n = s->mech->ns + 1;
+    const size_t nv = (size_t)n * nS;
+    return with_device((3 * nS + 2 * nv) * 8, [&](char* base) -> int {
+        double* dp = (double*)base;
+        double* ddx = dp + nS;
+        double* de = ddx + nS;
+        double* dy = de + nS;
+        double* dn = dy + nv;
+        HIP_OK(hipMemcpy(dp, p, nS * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(ddx, dx, nS * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dy, YT, nv * 8, hipMemcpyHostToDevice));
+        DevMech M = s->mech->dm;
+        DevOpts o = dev_opts(s->opts);
+        int ns32 = (int)nS;
+        void* args[] = {&M, &o, &ns32, &dp, &ddx, &dy, &dn, &de};
+        HIP_OK(hipLaunchKernel(s->ks->attempt[s->opts.solver], dim3((unsigned)nS), dim3(64), args, s->ldsBytes,
+                               s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        HIP_OK(hipMemcpy(YT, dy, nv * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(YTnew, dn, nv * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(err, de, nS * 8, hipMemcpyDeviceToHost));
         return FCS_OK;
     });
 }

@@ -22,12 +22,16 @@ EXPORTS = [
     "fcs_device_count", "fcs_version", "fcs_mech_create", "fcs_mech_destroy", "fcs_solver_create",
     "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_batch_create", "fcs_batch_destroy",
     "fcs_batch_upload", "fcs_batch_run", "fcs_batch_download", "fcs_eval_derivatives",
-    "fcs_eval_jacobian", "fcs_lu_solve",
+    "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
 ]
 
 
 class FcsError(RuntimeError):
-    pass
+    """A C-ABI call returned an error code (``code``, one of FCS_E_*)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -59,6 +63,7 @@ def load(path: str | None = None):
     L.fcs_eval_derivatives.argtypes = [_vp, C.c_int64, _dp, _dp, _dp]
     L.fcs_eval_jacobian.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp]
     L.fcs_lu_solve.argtypes = [_vp, C.c_int64, _dp, _dp]
+    L.fcs_eval_attempt.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp]
     for name in EXPORTS:
         getattr(L, name)
         if name != "fcs_version":
@@ -69,7 +74,7 @@ def load(path: str | None = None):
 
 def check(rc, what, allow=()):
     if rc != FCS_OK and rc not in allow:
-        raise FcsError(f"{what} failed with code {rc}")
+        raise FcsError(f"{what} failed with code {rc}", rc)
     return rc
 
 
@@ -150,6 +155,18 @@ class Solver:
         J = np.zeros((nS, self.n, self.n))
         check(load().fcs_eval_jacobian(self._h, nS, dptr(p), dptr(YT), dptr(f), dptr(J)), "fcs_eval_jacobian")
         return f, J, YT
+
+    def attempt(self, p, YT, dx):
+        """One Rosenbrock attempt per state: returns (err, YTnew, clipped YT)."""
+        YT = np.array(YT, np.float64, order="C").reshape(self.n, -1)
+        nS = YT.shape[1]
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(p, np.float64), (nS,)))
+        dx = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, np.float64), (nS,)))
+        YTn = np.zeros_like(YT)
+        err = np.zeros(nS)
+        check(load().fcs_eval_attempt(self._h, nS, dptr(p), dptr(dx), dptr(YT), dptr(YTn), dptr(err)),
+              "fcs_eval_attempt")
+        return err, YTn, YT
 
     def lu_solve(self, A, b):
         A = np.ascontiguousarray(A, np.float64).reshape(-1, self.n, self.n)

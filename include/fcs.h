@@ -193,6 +193,15 @@ int fcs_eval_derivatives(fcs_solver* solver, int64_t nStates, const double* p,
  * YT [n*nStates] -> f(y) [n*nStates] and Jy row-major [nStates][n][n] */
 int fcs_eval_jacobian(fcs_solver* solver, int64_t nStates, const double* p,
                       double* YT, double* dYTdt, double* Jy);
+/* One Rosenbrock attempt per state with step dx[s]: OptRodas34::Rodas34Solve
+ * (src/ODE/OptRodas34/OptRodas34.C:291-879) or
+ * OptRosenbrock34::Rosenbrock34Solve (src/ODE/OptRosenbrock34/
+ * OptRosenbrock34.C:281-677), by the solver's option.  YT [n*nStates] start
+ * states (clipped >= 0 on output, as the Jacobian evaluation clips them),
+ * YTnew [n*nStates] candidate states, err [nStates] error norms.
+ * FCS_E_UNSUPPORTED for OptSeulex. */
+int fcs_eval_attempt(fcs_solver* solver, int64_t nStates, const double* p,
+                     const double* dx, double* YT, double* YTnew, double* err);
 /* LUsolver::Block4LUDecompose + xSolve (src/LUsolver/LUsolver.C:3058-3171):
  * A [nSys][n][n] row-major, b [nSys][n] -> x in b.  n = n_species + 1. */
 int fcs_lu_solve(fcs_solver* solver, int64_t nSys, const double* A, double* b);

@@ -798,6 +798,9 @@ static int rosenbrock_integrate(const orc_mech* m, const fcs_solver_opts* o, dou
         ++*nAtt;
         if (!(Err == Err)) return FCS_CELL_NONFINITE;
         while (Err > 1) {
+            /* guard absent from the reference (its rejection loop is unbounded):
+             * at most 100 * maxSteps attempts per cell */
+            if (*nAtt > 100 * o->maxSteps) return FCS_CELL_MAXSTEPS;
             logrec(log, cap, len, x, dx, Err, 0, 0);
             const double sc = SAFE * pow(Err, -ALPHA_DEC);
             dx *= sc > MIN_SCALE ? sc : MIN_SCALE;
@@ -805,7 +808,6 @@ static int rosenbrock_integrate(const orc_mech* m, const fcs_solver_opts* o, dou
                                                     : ros34_attempt(m, o, p, dx, w);
             ++*nAtt;
             if (!(Err == Err)) return FCS_CELL_NONFINITE;
-            if (*nAtt > 100 * o->maxSteps) return FCS_CELL_MAXSTEPS;
         }
         logrec(log, cap, len, x, dx, Err, 1, 0);
         x += dx;
@@ -1083,6 +1085,25 @@ int orc_integrate_cell(const orc_mech* m, const fcs_solver_opts* o, double p, do
     if (nAttempts) *nAttempts = na;
     work_free(&w);
     return rc;
+}
+
+/* One Rosenbrock attempt from y (clipped in place by the Jacobian) with step
+ * dx: src/ODE/OptRodas34/OptRodas34.C:291-879 (Rodas34Solve) or
+ * src/ODE/OptRosenbrock34/OptRosenbrock34.C:281-677 (Rosenbrock34Solve).
+ * Writes the candidate state to yNew and returns the error norm; NaN for a
+ * solver without a single-attempt form (Seulex). */
+double orc_rosenbrock_attempt(const orc_mech* m, const fcs_solver_opts* o, double p, double dx,
+                              double* y, double* yNew)
+{
+    if (o->solver == FCS_SOLVER_SEULEX) return NAN;
+    work w;
+    work_init(&w, m->n);
+    for (int i = 0; i < m->n; ++i) w.y0[i] = y[i];
+    const double err = (o->solver == FCS_SOLVER_RODAS34) ? rodas34_attempt(m, o, p, dx, &w)
+                                                         : ros34_attempt(m, o, p, dx, &w);
+    for (int i = 0; i < m->n; ++i) { y[i] = w.y0[i]; yNew[i] = w.yT[i]; }
+    work_free(&w);
+    return err;
 }
 
 /* src/ChemistryModel/FastChemistryModel_transientSolve.H:3-104 */

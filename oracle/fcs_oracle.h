@@ -64,6 +64,12 @@ int orc_integrate_cell(const orc_mech* m, const fcs_solver_opts* o, double p,
                        int32_t* nAttempts, orc_step_rec* log, int32_t logCap,
                        int32_t* logLen);
 
+/* One Rosenbrock attempt (Rodas34Solve / Rosenbrock34Solve) from y[n]
+ * (clipped in place) with step dx: candidate state to yNew[n], returns the
+ * error norm (NaN for Seulex). */
+double orc_rosenbrock_attempt(const orc_mech* m, const fcs_solver_opts* o, double p,
+                              double dx, double* y, double* yNew);
+
 /* FastChemistryModel::solve(deltaT) over a batch, species-major arrays
  * Y0/RR [ns*nCells]; nThreads > 1 uses OpenMP with a dynamic schedule. */
 int orc_solve(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells,

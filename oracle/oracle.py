@@ -56,6 +56,9 @@ def lib():
         L.orc_integrate_cell.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_double, _dp,
                                          C.c_double, _dp, _ip, C.POINTER(StepRec), C.c_int32, _ip]
         L.orc_set_textbook_rosenbrock.argtypes = [C.c_int]
+        L.orc_rosenbrock_attempt.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_double, C.c_double,
+                                             _dp, _dp]
+        L.orc_rosenbrock_attempt.restype = C.c_double
         L.orc_solve.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, C.c_double,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
     return _lib
@@ -101,6 +104,15 @@ class Oracle:
         lib().orc_jacobian(self._h, float(p), dptr(y), dptr(f), dptr(J),
                            0 if jac == "exact" else 1, dptr(Jc) if want_jc else None)
         return (f, J, y, Jc) if want_jc else (f, J, y)
+
+    def attempt(self, p, YT, dx, **opts):
+        """One Rosenbrock attempt (Rodas34Solve / Rosenbrock34Solve): returns
+        (error norm, candidate state, clipped start state)."""
+        o = make_opts(**opts)
+        y = np.array(YT, np.float64)
+        yn = np.zeros(self.n)
+        err = lib().orc_rosenbrock_attempt(self._h, C.byref(o), float(p), float(dx), dptr(y), dptr(yn))
+        return err, yn, y
 
     def integrate_cell(self, p, YT, deltaT, dxTry, log_cap=0, **opts):
         o = make_opts(**opts)

