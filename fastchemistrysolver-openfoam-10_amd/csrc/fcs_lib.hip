@@ -303,6 +303,11 @@ struct DevCells {
     unsigned long long* dtMin = nullptr;
     unsigned int* hostCount = nullptr;       // pinned [kPoll]
     hipEvent_t ev[kPoll] = {};
+    // optional per-launch timing of the step kernel (event pair per launch)
+    bool profile = false;
+    std::vector<hipEvent_t> pev;
+    long long lastLaunches = 0;
+    double lastStepMs = 0.0;
 
     int alloc(size_t nc, int nsp)
     {
@@ -345,6 +350,8 @@ struct DevCells {
         if (hostCount) (void)hipHostFree(hostCount);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
+        for (auto& e : pev) (void)hipEventDestroy(e);
+        pev.clear();
         base = nullptr;
         hostCount = nullptr;
         cap = 0;
@@ -693,7 +700,17 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt)
         unsigned int* cout = d.counts + (it + 1) % 3;
         unsigned int* czero = d.counts + (it + 2) % 3;
         void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
+        if (d.profile) {
+            while (d.pev.size() < 2 * (size_t)(it + 1)) {
+                hipEvent_t e;
+                HIP_OK(hipEventCreate(&e));
+                d.pev.push_back(e);
+            }
+            HIP_OK(hipEventRecord(d.pev[2 * it], st));
+        }
         HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st));
+        if (d.profile) HIP_OK(hipEventRecord(d.pev[2 * it + 1], st));
+        d.lastLaunches = it + 1;
         const int slot = (int)(it % DevCells::kPoll);
         HIP_OK(hipMemcpyAsync(d.hostCount + slot, cout, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIP_OK(hipEventRecord(d.ev[slot], st));
@@ -704,6 +721,15 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt)
             if (d.hostCount[bs] == 0u) break;
         }
         if (it > maxLaunches) return FCS_E_HIP;
+    }
+    d.lastStepMs = 0.0;
+    if (d.profile) {
+        HIP_OK(hipStreamSynchronize(st));
+        for (long long i = 0; i < d.lastLaunches; ++i) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, d.pev[2 * i], d.pev[2 * i + 1]));
+            d.lastStepMs += ms;
+        }
     }
     return FCS_OK;
 }
@@ -817,6 +843,21 @@ int fcs_batch_run(fcs_batch* b, double deltaT, float* elapsed_ms)
     HIP_OK(hipEventRecord(b->e1, s->stream));
     HIP_OK(hipEventSynchronize(b->e1));
     if (elapsed_ms) HIP_OK(hipEventElapsedTime(elapsed_ms, b->e0, b->e1));
+    return FCS_OK;
+}
+
+int fcs_batch_profile(fcs_batch* b, int enable)
+{
+    if (!b) return FCS_E_ARG;
+    b->cells.profile = enable != 0;
+    return FCS_OK;
+}
+
+int fcs_batch_stats(fcs_batch* b, int64_t* stepLaunches, double* stepKernelMs)
+{
+    if (!b) return FCS_E_ARG;
+    if (stepLaunches) *stepLaunches = b->cells.lastLaunches;
+    if (stepKernelMs) *stepKernelMs = b->cells.lastStepMs;
     return FCS_OK;
 }
 

@@ -21,8 +21,8 @@ _vp = C.c_void_p
 EXPORTS = [
     "fcs_device_count", "fcs_version", "fcs_mech_create", "fcs_mech_destroy", "fcs_solver_create",
     "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_batch_create", "fcs_batch_destroy",
-    "fcs_batch_upload", "fcs_batch_run", "fcs_batch_download", "fcs_eval_derivatives",
-    "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
+    "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_download",
+    "fcs_eval_derivatives", "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
 ]
 
 
@@ -60,6 +60,8 @@ def load(path: str | None = None):
     L.fcs_batch_upload.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
     L.fcs_batch_run.argtypes = [_vp, C.c_double, C.POINTER(C.c_float)]
     L.fcs_batch_download.argtypes = [_vp, _dp, _dp, _dp, _ip, _dp, _ip]
+    L.fcs_batch_profile.argtypes = [_vp, C.c_int]
+    L.fcs_batch_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.fcs_eval_derivatives.argtypes = [_vp, C.c_int64, _dp, _dp, _dp]
     L.fcs_eval_jacobian.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp]
     L.fcs_lu_solve.argtypes = [_vp, C.c_int64, _dp, _dp]
@@ -196,6 +198,16 @@ class Batch:
         ms = C.c_float(0.0)
         check(load().fcs_batch_run(self._h, float(deltaT), C.byref(ms) if timed else None), "fcs_batch_run")
         return ms.value
+
+    def profile(self, enable=True):
+        """Time every step launch of later runs with a HIP event pair."""
+        check(load().fcs_batch_profile(self._h, 1 if enable else 0), "fcs_batch_profile")
+
+    def stats(self):
+        """(step launches, summed step-kernel milliseconds) of the last run."""
+        n, ms = C.c_int64(0), C.c_double(0.0)
+        check(load().fcs_batch_stats(self._h, C.byref(n), C.byref(ms)), "fcs_batch_stats")
+        return n.value, ms.value
 
     def download(self):
         ns, nc = self.s.ns, self.nc

@@ -176,6 +176,12 @@ int fcs_batch_upload(fcs_batch* batch, const double* T0, const double* p0,
  * elapsed_ms is non-NULL the call waits and returns the time between HIP
  * events recorded on that stream around the kernel launches. */
 int fcs_batch_run(fcs_batch* batch, double deltaT, float* elapsed_ms);
+/* Profiling of fcs_batch_run (no counterpart in the reference): with
+ * enable != 0 each step launch is bracketed by a pair of HIP events on the
+ * solver's stream; fcs_batch_stats reports the launch count of the last run
+ * and the summed device duration of its step kernels (0 when disabled). */
+int fcs_batch_profile(fcs_batch* batch, int enable);
+int fcs_batch_stats(fcs_batch* batch, int64_t* stepLaunches, double* stepKernelMs);
 int fcs_batch_download(fcs_batch* batch, double* RR /* [ns*nCells] */,
                        double* deltaTChem, double* deltaTMin,
                        int32_t* cellStatus, double* Tend, int32_t* nAttempts);

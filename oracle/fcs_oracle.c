@@ -622,15 +622,20 @@ static const double SAFE = 0.9, ALPHA_INC = 0.2, ALPHA_DEC = 0.25, MIN_SCALE = 0
 static double err_norm(int n, const double* y0, const double* y, const double* e,
                        double absTol, double relTol)
 {
-    /* src/ODE/OptRodas34/OptRodas34.C:822-877 */
+    /* src/ODE/OptRodas34/OptRodas34.C:822-877.  Any NaN component makes the
+     * norm NaN, so the attempt reports the cell non-finite.  (The reference's
+     * AVX max keeps or drops a NaN depending on its vector lane and position,
+     * and its controller then accepts the step: a NaN Err is not > 1.) */
     double mx = 0.0;
+    int nan = 0;
     for (int i = 0; i < n; ++i) {
         const double a0 = fabs(y0[i]), a1 = fabs(y[i]);
         const double tol = absTol + relTol * (a1 > a0 ? a1 : a0);
         const double r = fabs(e[i]) / tol;
+        if (!(r == r)) nan = 1;
         mx = r > mx ? r : mx;
     }
-    return mx;
+    return nan ? NAN : mx;
 }
 
 /* Rodas3(4) coefficients: src/ODE/OptRodas34/OptRodas34.H:116-149 */
