@@ -118,41 +118,49 @@ __device__ __forceinline__ int first_lane(bool pred)
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // ---------------------------------------------------------------- LDS layout
+// The per-cell matrix lives in LDS, lane-interleaved column-major: element
+// (row i, column j) at J[j*64 + i], row i owned by lane i.  Column accesses
+// of one row index across the lanes are conflict-free; the pivot row of the
+// LU is read by all lanes at one address (broadcast).
 struct Lds {
     double* C;     // [64] concentrations
     double* expG;  // [64] exp(-G/RT)
     double* dBdT;  // [64] d(-G/RT)/dT
+    double* H;     // [64] species enthalpy (T-row column sums)
+    double* Q;     // [64] dY/dt * cp (T-row column sums)
     double* rq;    // [nrP] q (rates) or kf (Jacobian)
     double* rk;    // [nrP] kr (Jacobian)
     double* rdT;   // [nrP] dq/dT (Jacobian)
     double* rw;    // [nTBP] third-body partial (Jacobian)
-    double* J;     // [NP*64] Jacobian assembly / transposes; Seulex keeps Jy here
-    double* P;     // [66] pivot row broadcast
-    int* ip;       // [64] integer scratch
+    double* J;     // [NP*64] Jacobian, then iteration matrix and its LU factors
+    double* J2;    // [NP*64] Seulex only: Jy kept across the extrapolation columns
+    int* ip;       // [64] lane holding each logical row after the LU
 };
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-__host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
+__host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB, bool keepJy = false)
 {
-    return 3 * 64 + 3 * (size_t)round_up(nr, 2) + (size_t)round_up(nTB > 0 ? nTB : 1, 2)
-           + (size_t)NP * 64 + 66 + 32;
+    return 5 * 64 + 3 * (size_t)round_up(nr, 2) + (size_t)round_up(nTB > 0 ? nTB : 1, 2)
+           + (size_t)NP * 64 * (keepJy ? 2 : 1) + 32;
 }
 
-__device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
+__device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, bool keepJy = false)
 {
     Lds L;
     const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
     L.C = base;
     L.expG = base + 64;
     L.dBdT = base + 128;
-    L.rq = base + 192;
+    L.H = base + 192;
+    L.Q = base + 256;
+    L.rq = base + 320;
     L.rk = L.rq + nrP;
     L.rdT = L.rk + nrP;
     L.rw = L.rdT + nrP;
     L.J = L.rw + nTBP;
-    L.P = L.J + (size_t)NP * 64;
-    L.ip = (int*)(L.P + 66);
+    L.J2 = L.J + (size_t)NP * 64;
+    L.ip = (int*)(L.J2 + (keepJy ? (size_t)NP * 64 : 0));
     return L;
 }
 
@@ -233,7 +241,18 @@ __device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, int r, int 
     // third-body concentration M = sum_j eff_rj C_j
     const int tb = M.rtb[r];
     double Mc = 0.0;
-    for (int j = 0; j < M.ns; ++j) Mc += M.effT[(size_t)j * M.nTB + tb] * L.C[j];
+    for (int j0 = 0; j0 < M.ns; j0 += 8) {   // batches of 8 independent loads
+        double e[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j0 + u < M.ns) {
+                e[u] = M.effT[(size_t)(j0 + u) * M.nTB + tb];
+                c[u] = L.C[j0 + u];
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j0 + u < M.ns) Mc += e[u] * c[u];
+    }
     if (t == RXN_TB) {
         k.kf = kk * Mc;
         k.dkfdT = dk * Mc;
@@ -388,9 +407,20 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
     wave_sync();
     double om = 0.0;
     if (sp) {
-        for (int e = 0; e < M.maxDeg; ++e) {
-            const double nu = M.sp_nu[e * WAVE + lane];
-            if (nu != 0.0) om += nu * L.rq[M.sp_r[e * WAVE + lane]];
+        for (int e0 = 0; e0 < M.maxDeg; e0 += 8) {   // batches of 8 list entries
+            double nu[8], q[8];
+            int r[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool in = e0 + u < M.maxDeg;
+                nu[u] = in ? M.sp_nu[(e0 + u) * WAVE + lane] : 0.0;
+                r[u] = in ? M.sp_r[(e0 + u) * WAVE + lane] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = nu[u] != 0.0 ? L.rq[r[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (nu[u] != 0.0) om += nu[u] * q[u];
         }
     }
     const double Yd = sp ? om * (M.W[lane] * invrho) : 0.0;
@@ -408,14 +438,15 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
 //     = Jc_ij rho/W_j - (MW/W_j) sum_k Jc_ik C_k
 // (O(n^2) instead of the reference's dense O(n^3) product,
 // Jacobian.C:2814-2857); fast keeps the diagonal dC/dY only
-// (Jacobian.C:4570-4841).  y is clipped in place; returns f(y) for this lane
-// and the lane's row of Jy in a[].
+// (Jacobian.C:4570-4841).  y is clipped in place; Jy is left in L.J (row i
+// at lane i, columns 0..NP-1, temperature column NP-1); returns f(y) for
+// this lane.
 template <int NP>
-__device__ inline double jacobian(const DevMech& M, const Lds& L, double p, double lnp, int jacType, double& y,
-                                  double (&a)[NP])
+__device__ inline double jacobian(const DevMech& M, const Lds& L, double p, double lnp, int jacType, double& y)
 {
     const int lane = lane_id();
-    const bool sp = lane < M.ns;
+    const int ns = M.ns;
+    const bool sp = lane < ns;
     if (sp) y = y > 0.0 ? y : 0.0;
     const double T = clampT(M, readlane_d(y, NP - 1));
     const double lnT = log(T), iT = 1.0 / T;
@@ -432,12 +463,12 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     L.C[lane] = Cl;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     L.dBdT[lane] = th.dBdT;
-#pragma unroll
-    for (int j = 0; j < NP; ++j) L.J[j * WAVE + lane] = 0.0;
+    double* Jrow = L.J + lane;   // element j of this lane's row at Jrow[j*64]
+    for (int j = 0; j < NP; ++j) Jrow[j * WAVE] = 0.0;
     wave_sync();
     reaction_round<true>(M, L, T, lnT, iT, lnp);
     wave_sync();
-    // species rows: sparse partials into the lane's column of L.J
+    // species rows: sparse partials dq/dC into this lane's row
     double om = 0.0, jT = 0.0;
     if (sp) {
         for (int e = 0; e < M.maxDeg; ++e) {
@@ -461,115 +492,134 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             for (int q = 0; q < nl; ++q) {
                 double d = kf;
                 for (int b = 0; b < nl; ++b) if (b != q) d *= c[b];
-                L.J[s[q] * WAVE + lane] += nu * d;
+                Jrow[s[q] * WAVE] += nu * d;
             }
             if (kr != 0.0) {
                 for (int q = 0; q < nrh; ++q) {
                     double d = kr;
                     for (int b = 0; b < nrh; ++b) if (b != q) d *= c[3 + b];
-                    L.J[s[3 + q] * WAVE + lane] -= nu * d;
+                    Jrow[s[3 + q] * WAVE] -= nu * d;
                 }
             }
         }
-    }
-    wave_sync();
-#pragma unroll
-    for (int j = 0; j < NP - 1; ++j) a[j] = L.J[j * WAVE + lane];
-    a[NP - 1] = jT;
-    // third-body / falloff partials: nu * w_r * eff_rj on every column
-    if (sp) {
+        // third-body / falloff partials: nu * w_r * eff_rj on every column
+        // (species-major efficiencies: the lanes' loads of one column share a row)
         for (int e = 0; e < M.maxDegTB; ++e) {
             const double nu = M.sptb_nu[e * WAVE + lane];
             if (nu == 0.0) continue;
             const int t = M.sptb_t[e * WAVE + lane];
             const double coef = nu * L.rw[t];
-            const double* er = M.eff + (size_t)t * 64;
+            for (int j0 = 0; j0 < ns; j0 += 8) {
+                double v[8], ef[8];
 #pragma unroll
-            for (int j = 0; j < NP - 1; ++j) a[j] += coef * er[j];
+                for (int u = 0; u < 8; ++u)
+                    if (j0 + u < ns) {
+                        v[u] = Jrow[(j0 + u) * WAVE];
+                        ef[u] = M.effT[(size_t)(j0 + u) * M.nTB + t];
+                    }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (j0 + u < ns) Jrow[(j0 + u) * WAVE] = v[u] + coef * ef[u];
+            }
         }
     }
-    // dC/dY conversion of the species rows
+    // dC/dY conversion of the species rows, T column
     const double WiByRho = sp ? M.W[lane] * invrho : 0.0;
     const double Yd = sp ? om * WiByRho : 0.0;
-    double sC = 0.0;
-#pragma unroll
-    for (int j = 0; j < NP - 1; ++j) sC += a[j] * L.C[j];
     if (sp) {
-        if (jacType == 0) {
+        double sC = 0.0;
+        for (int j0 = 0; j0 < ns; j0 += 8) {
+            double v[8], c[8];
 #pragma unroll
-            for (int j = 0; j < NP - 1; ++j) {
-                const double iWj = M.invW[j];
-                a[j] = WiByRho * (a[j] * (rho * iWj) - (MW * iWj) * sC) + (MW * iWj) * Yd;
-            }
-        } else {
+            for (int u = 0; u < 8; ++u)
+                if (j0 + u < ns) {
+                    v[u] = Jrow[(j0 + u) * WAVE];
+                    c[u] = L.C[j0 + u];
+                }
 #pragma unroll
-            for (int j = 0; j < NP - 1; ++j) {
-                const double iWj = M.invW[j];
-                a[j] = WiByRho * (a[j] * (rho * iWj)) + (MW * iWj) * Yd;
-            }
+            for (int u = 0; u < 8; ++u)
+                if (j0 + u < ns) sC += v[u] * c[u];
         }
-        a[NP - 1] = WiByRho * (jT - sC * iT) + iT * Yd;
+        const double sCx = jacType == 0 ? sC : 0.0;   // fast: diagonal dC/dY only
+        for (int j0 = 0; j0 < ns; j0 += 8) {
+            double v[8], iw[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j0 + u < ns) {
+                    v[u] = Jrow[(j0 + u) * WAVE];
+                    iw[u] = M.invW[j0 + u];
+                }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j0 + u < ns) {
+                    const double iWj = iw[u];
+                    Jrow[(j0 + u) * WAVE] = jacType == 0
+                        ? WiByRho * (v[u] * (rho * iWj) - (MW * iWj) * sCx) + (MW * iWj) * Yd
+                        : WiByRho * (v[u] * (rho * iWj)) + (MW * iWj) * Yd;
+                }
+        }
+        Jrow[(NP - 1) * WAVE] = WiByRho * (jT - sC * iT) + iT * Yd;
     }
-    // temperature row (src/Reaction/Jacobian.C:3743-3791)
+    // temperature row (src/Reaction/Jacobian.C:3743-3791): lane j < NP forms
+    // column j's sum over the species rows and writes row NP-1, column j
     const double dTdt = -wave_sum(sp ? Yd * th.ha : 0.0) / cpM;
+    L.H[lane] = sp ? th.ha : 0.0;
+    L.Q[lane] = sp ? Yd * th.cp : 0.0;
     wave_sync();
-    if (sp) {
-#pragma unroll
-        for (int j = 0; j < NP - 1; ++j) L.J[j * WAVE + lane] = a[j] * th.ha;
-        L.J[(NP - 1) * WAVE + lane] = Yd * th.cp + a[NP - 1] * th.ha;
-    } else {
-#pragma unroll
-        for (int j = 0; j < NP; ++j) L.J[j * WAVE + lane] = 0.0;
-    }
-    wave_sync();
-    // lane j sums column j over the species lanes; the start is rotated by
-    // the lane index so that the lanes hit different LDS banks
-    double colsum = 0.0;
+    // (the sums read species rows only, so row NP-1 can be written at once)
     if (lane < NP) {
-        int i = lane % M.ns;
-        for (int q = 0; q < M.ns; ++q) {
-            colsum += L.J[lane * WAVE + i];
-            i = (i + 1 == M.ns) ? 0 : i + 1;
-        }
-    }
-    double rowT;
-    if (lane < NP - 1) rowT = sp ? (-colsum - th.cp * dTdt) / cpM : 0.0;
-    else rowT = (-colsum - dTdt * dcpMdT) / cpM;
-    wave_sync();
+        const bool tcol = lane == NP - 1;
+        double colsum = 0.0;
+        const int i0 = lane % ns;   // rotated start: the lanes hit different banks
+        for (int q0 = 0; q0 < ns; q0 += 8) {
+            double v[8], h[8], qq[8];
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
-        const double v = readlane_d(rowT, j);
-        if (lane == NP - 1) a[j] = v;
+            for (int u = 0; u < 8; ++u)
+                if (q0 + u < ns) {
+                    int i = i0 + q0 + u;
+                    i = i >= ns ? i - ns : i;
+                    v[u] = L.J[lane * WAVE + i];
+                    h[u] = L.H[i];
+                    qq[u] = tcol ? L.Q[i] : 0.0;
+                }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (q0 + u < ns) colsum += v[u] * h[u] + qq[u];
+        }
+        const double rowT = !tcol ? (sp ? (-colsum - th.cp * dTdt) / cpM : 0.0)
+                                  : (-colsum - dTdt * dcpMdT) / cpM;
+        L.J[lane * WAVE + (NP - 1)] = rowT;
     }
+    wave_sync();
     return sp ? Yd : (lane == NP - 1 ? dTdt : 0.0);
 }
 
 // ---------------------------------------------------------------- LU
 // Block-4 LU with pivoting restricted to each 4-row diagonal block
-// (src/LUsolver/LUsolver.C:106-246 full blocks, :503-551 / :553-650 tails),
-// on the row-per-lane register matrix.  Rows are permuted virtually (pos =
-// logical row held by the lane) during the factorisation and physically at
-// the end, so that lane k holds row k of L\U.  `src` is the lane each lane
-// copied its row from, used to permute right-hand sides.
-struct LuState { double invD; int src; };
+// (src/LUsolver/LUsolver.C:106-246 full blocks, :503-551 / :553-650 tails)
+// on the LDS matrix.  Rows never move: pos is the logical row a lane holds,
+// swapped as the reference swaps rows, and ip[r] is the lane holding
+// logical row r when the factorisation ends.  Right-looking elimination:
+// the pivot row is read by all lanes at one address.
+struct LuState { double invD; int pos; };
 
-// |value| of this column entry in the lane that holds logical row r.
+// |value| of column-k entry in the lane that holds logical row r.
 __device__ __forceinline__ double cand(double ak, int pos, int r)
 {
     return fabs(readlane_d(ak, first_lane(pos == r)));
 }
 
 template <int NP>
-__device__ inline LuState lu_factor(const Lds& L, double (&a)[NP])
+__device__ inline LuState lu_factor(const Lds& L)
 {
     const int lane = lane_id();
-    // pos: logical row held by this lane (lanes >= NP hold no row)
-    int pos = lane;
     const bool row = lane < NP;
+    double* Jrow = L.J + lane;
+    int pos = lane;          // lanes >= NP hold no row
     double invD = 0.0;
-#pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int k0 = k & ~3, d = k & 3;
+        double ak = Jrow[k * WAVE];
         // --- pivot choice (uniform) ---
         int pick = k;
         if (d == 0) {
@@ -577,24 +627,24 @@ __device__ inline LuState lu_factor(const Lds& L, double (&a)[NP])
             int im = 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double v = cand(a[k], pos, k0 + q);
+                const double v = cand(ak, pos, k0 + q);
                 if (temp < v) { im = q; temp = v; }
             }
             pick = k0 + im;
         } else if (d == 1) {
-            double t = cand(a[k], pos, k0 + 1);
+            double t = cand(ak, pos, k0 + 1);
             int im = 1;
-            const double v2 = cand(a[k], pos, k0 + 2);
+            const double v2 = cand(ak, pos, k0 + 2);
             if (t < v2) {
                 im = 2;
-                t = readlane_d(a[k], first_lane(pos == k0 + 2));   // signed, LUsolver.C:177
+                t = readlane_d(ak, first_lane(pos == k0 + 2));   // signed, LUsolver.C:177
             }
-            const double v3 = cand(a[k], pos, k0 + 3);
+            const double v3 = cand(ak, pos, k0 + 3);
             if (t < v3) im = 3;
             pick = k0 + im;
         } else if (d == 2) {
-            const double v2 = cand(a[k], pos, k0 + 2);
-            const double v3 = cand(a[k], pos, k0 + 3);
+            const double v2 = cand(ak, pos, k0 + 2);
+            const double v3 = cand(ak, pos, k0 + 3);
             if (v2 < v3) pick = k0 + 3;
         }
         if (pick != k) {
@@ -602,56 +652,58 @@ __device__ inline LuState lu_factor(const Lds& L, double (&a)[NP])
             else if (pos == pick) pos = k;
         }
         const int pl = first_lane(pos == k);
-        // --- pivot row to LDS (one lane), broadcast reads ---
-        if (lane == pl) {
-            if (a[k] == 0.0) a[k] = PIVOT_EPS;
-            invD = 1.0 / a[k];
-#pragma unroll
-            for (int j = k; j < NP; ++j) L.P[j] = a[j];
-            L.P[NP] = invD;
+        double piv = readlane_d(ak, pl);
+        if (piv == 0.0) {                        // LUsolver.C:155
+            piv = PIVOT_EPS;
+            if (lane == pl) Jrow[k * WAVE] = PIVOT_EPS;
         }
-        wave_sync();
-        const double inv = L.P[NP];
+        const double inv = 1.0 / piv;
+        if (lane == pl) invD = inv;
+        const double* prow = L.J + pl;           // pivot row, broadcast reads
         if (row && pos > k) {
-            const double l = a[k] * inv;
-            a[k] = l;
+            const double l = ak * inv;
+            Jrow[k * WAVE] = l;
+            // batches of 8: all loads of a batch are issued before its stores
+            for (int j0 = k + 1; j0 < NP; j0 += 8) {
+                double v[8], pv[8];
 #pragma unroll
-            for (int j = k + 1; j < NP; ++j) a[j] -= l * L.P[j];
+                for (int u = 0; u < 8; ++u)
+                    if (j0 + u < NP) {
+                        v[u] = Jrow[(j0 + u) * WAVE];
+                        pv[u] = prow[(j0 + u) * WAVE];
+                    }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (j0 + u < NP) Jrow[(j0 + u) * WAVE] = v[u] - l * pv[u];
+            }
         }
         wave_sync();
     }
-    // physical permutation: lane k receives logical row k
     if (row) L.ip[pos] = lane;
     wave_sync();
-    LuState st;
-    st.src = (lane < NP) ? L.ip[lane] : lane;
-    wave_sync();
-#pragma unroll
-    for (int j = 0; j < NP; ++j) a[j] = __shfl(a[j], st.src, WAVE);
-    st.invD = __shfl(invD, st.src, WAVE);
-    return st;
+    return LuState{invD, pos};
 }
 
-// Solves (L U) x = P b in place; b is this lane's element in the original
-// row order; returns x for this lane's column (src/LUsolver/LUsolver.C:
+// Solves (L U) x = P b; b is this lane's element in the original row order
+// (lane i = row i); returns x_i for lane i (src/LUsolver/LUsolver.C:
 // 3922-4111: swaps, unit-L forward, U backward scaled by invD).
 template <int NP>
-__device__ inline double lu_solve(const double (&a)[NP], const LuState& st, double b)
+__device__ inline double lu_solve(const Lds& L, const LuState& st, double b)
 {
     const int lane = lane_id();
-    b = __shfl(b, st.src, WAVE);
-#pragma unroll
-    for (int k = 0; k < NP - 1; ++k) {
-        const double xk = readlane_d(b, k);
-        if (lane > k) b -= a[k] * xk;
+    const double* Jrow = L.J + lane;
+    for (int r = 0; r < NP - 1; ++r) {
+        const int src = L.ip[r];
+        const double yr = readlane_d(b, src);
+        if (st.pos > r && lane < NP) b -= Jrow[r * WAVE] * yr;
     }
     double x = 0.0;
-#pragma unroll
-    for (int k = NP - 1; k >= 0; --k) {
-        if (lane == k) b *= st.invD;
-        const double xk = readlane_d(b, k);
-        if (lane < k) b -= a[k] * xk;
-        if (lane == k) x = xk;
+    for (int r = NP - 1; r >= 0; --r) {
+        const int src = L.ip[r];
+        if (lane == src) b *= st.invD;
+        const double xr = readlane_d(b, src);
+        if (st.pos < r) b -= Jrow[r * WAVE] * xr;
+        if (lane == r) x = xr;
     }
     return x;
 }

@@ -54,15 +54,14 @@ __device__ inline double error_norm(const DevOpts& o, double y0, double yT, doub
     return bad ? __builtin_nan("") : mx;
 }
 
+// Iteration matrix diag*I - J in place in the LDS matrix.
 template <int NP>
-__device__ inline void form_matrix(double (&a)[NP], double diag)
+__device__ inline void form_matrix(const Lds& L, double diag)
 {
     const int lane = lane_id();
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-        a[j] = -a[j];
-        if (lane == j) a[j] += diag;
-    }
+    double* Jrow = L.J + lane;
+    for (int j = 0; j < NP; ++j) Jrow[j * WAVE] = (lane == j ? diag : 0.0) - Jrow[j * WAVE];
+    wave_sync();
 }
 
 // Rosenbrock tableau in the transformed (Hairer-Wanner) form the reference
@@ -132,11 +131,10 @@ template <int NP>
 __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOpts& o, const Tableau& tb,
                                      double p, double lnp, double dx, double& y0, double& yT)
 {
-    double a[NP];
     const double invdx = 1.0 / dx;
-    const double dfdx = jacobian<NP>(M, L, p, lnp, o.jac, y0, a);
-    form_matrix<NP>(a, 1.0 * invdx * tb.invGamma);
-    const LuState st = lu_factor<NP>(L, a);
+    const double dfdx = jacobian<NP>(M, L, p, lnp, o.jac, y0);
+    form_matrix<NP>(L, 1.0 * invdx * tb.invGamma);
+    const LuState st = lu_factor<NP>(L);
     double k[6] = {0, 0, 0, 0, 0, 0};
     double dydx = 0.0;
     for (int s = 0; s < tb.stages; ++s) {
@@ -154,7 +152,7 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
 #pragma unroll
             for (int j = 0; j < 5; ++j) rhs += (tb.C[s][j] * invdx) * k[j];
         }
-        const double ks = lu_solve<NP>(a, st, rhs);
+        const double ks = lu_solve<NP>(L, st, rhs);
 #pragma unroll
         for (int j = 0; j < 6; ++j)
             if (j == s) k[j] = ks;
@@ -258,18 +256,21 @@ constexpr SeulexConst make_seulex_const()
 }
 __constant__ const SeulexConst kSeulex = make_seulex_const();
 
-// Jy row of this lane <-> LDS (kept across seul() calls of one step).
+// Jy is kept in L.J2 across the seul() calls of one step; each seul() forms
+// its iteration matrix from it in L.J.
 template <int NP>
-__device__ __forceinline__ void jy_store(const Lds& L, const double (&a)[NP])
+__device__ __forceinline__ void jy_store(const Lds& L)
 {
-#pragma unroll
-    for (int j = 0; j < NP; ++j) L.J[j * WAVE + lane_id()] = a[j];
+    const int lane = lane_id();
+    for (int j = 0; j < NP; ++j) L.J2[j * WAVE + lane] = L.J[j * WAVE + lane];
+    wave_sync();
 }
 template <int NP>
-__device__ __forceinline__ void jy_load(const Lds& L, double (&a)[NP])
+__device__ __forceinline__ void jy_load(const Lds& L)
 {
-#pragma unroll
-    for (int j = 0; j < NP; ++j) a[j] = L.J[j * WAVE + lane_id()];
+    const int lane = lane_id();
+    for (int j = 0; j < NP; ++j) L.J[j * WAVE + lane] = L.J2[j * WAVE + lane];
+    wave_sync();
 }
 
 // seul(): src/ODE/OptSeulex/OptSeulex.C:562-706, written with a single RHS
@@ -280,11 +281,10 @@ __device__ bool seul(const DevMech& M, const Lds& L, double p, double lnp, doubl
                      int nSteps, double scale, double& theta, double& ySeq)
 {
     const int lane = lane_id();
-    double a[NP];
-    jy_load<NP>(L, a);
+    jy_load<NP>(L);
     const double invdx = nSteps / dxTot;
-    form_matrix<NP>(a, invdx);
-    const LuState st = lu_factor<NP>(L, a);
+    form_matrix<NP>(L, invdx);
+    const LuState st = lu_factor<NP>(L);
     double yT = 0.0, dy = 0.0;
     int nn = 0;
     bool thetaPhase = false;
@@ -293,7 +293,7 @@ __device__ bool seul(const DevMech& M, const Lds& L, double p, double lnp, doubl
         double ev = (nn == 0) ? y0s : yT;
         const double f = derivatives<NP>(M, L, p, lnp, ev);
         if (nn == 0) y0s = ev; else yT = ev;
-        dy = lu_solve<NP>(a, st, thetaPhase ? f - dy * invdx : f);
+        dy = lu_solve<NP>(L, st, thetaPhase ? f - dy * invdx : f);
         if (nn == 0) yT = y0s;
         if (thetaPhase) {
             const bool huge = wave_any(lane < NP && fabs(dy) > sqrt(OF_VGREAT));
@@ -363,11 +363,8 @@ __device__ int seulex_integrate(const DevMech& M, const Lds& L, const DevOpts& o
         bool firstk = true;
         while (firstk || reject) {
             if (needJac) {
-                double jy[NP];
-                (void)jacobian<NP>(M, L, p, lnp, o.jac, y, jy);
-                wave_sync();
-                jy_store<NP>(L, jy);
-                wave_sync();
+                (void)jacobian<NP>(M, L, p, lnp, o.jac, y);
+                jy_store<NP>(L);
                 jacUpdated = true;
                 needJac = false;
             }

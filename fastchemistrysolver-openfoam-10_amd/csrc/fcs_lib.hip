@@ -171,16 +171,14 @@ __global__ __launch_bounds__(64) void k_jacobian(DevMech M, int jacType, int nS,
     const int s = blockIdx.x, lane = lane_id(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
-    double a[NP];
-    const double d = jacobian<NP>(M, L, p[s], log(p[s]), jacType, y, a);
+    const double d = jacobian<NP>(M, L, p[s], log(p[s]), jacType, y);
     if (ext >= 0) {
         YT[(size_t)ext * nS + s] = y;
         f[(size_t)ext * nS + s] = d;
         double* row = J + ((size_t)s * n + ext) * n;
-#pragma unroll
         for (int j = 0; j < NP; ++j) {
             const int ej = j < ns ? j : (j == NP - 1 ? ns : -1);
-            if (ej >= 0) row[ej] = a[j];
+            if (ej >= 0) row[ej] = L.J[j * WAVE + lane];
         }
     }
 }
@@ -216,14 +214,14 @@ __global__ __launch_bounds__(64) void k_lu_solve(DevMech M, int nSys, const doub
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
     const int s = blockIdx.x, lane = lane_id(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
-    double a[NP];
-#pragma unroll
     for (int j = 0; j < NP; ++j) {
         const int ej = j < ns ? j : (j == NP - 1 ? ns : -1);
-        a[j] = (ext >= 0 && ej >= 0) ? A[((size_t)s * n + ext) * n + ej] : ((lane == j && lane < NP) ? 1.0 : 0.0);
+        L.J[j * WAVE + lane] =
+            (ext >= 0 && ej >= 0) ? A[((size_t)s * n + ext) * n + ej] : ((lane == j && lane < NP) ? 1.0 : 0.0);
     }
-    const LuState st = lu_factor<NP>(L, a);
-    const double x = lu_solve<NP>(a, st, ext >= 0 ? b[(size_t)s * n + ext] : 0.0);
+    wave_sync();
+    const LuState st = lu_factor<NP>(L);
+    const double x = lu_solve<NP>(L, st, ext >= 0 ? b[(size_t)s * n + ext] : 0.0);
     if (ext >= 0) b[(size_t)s * n + ext] = x;
 }
 
