@@ -75,7 +75,13 @@ struct DevMech {
     const double* sp_nu;      // [maxDeg*64] net stoichiometric coefficient (0 = pad)
     const int* sptb_t;        // [maxDegTB*64] eff row of third-body/falloff entries
     const double* sptb_nu;    // [maxDegTB*64]
+    const unsigned long long* rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
+    const unsigned long long* sp_pack;  // [maxDeg*64] rpack of each per-species list entry
 };
+
+__device__ __forceinline__ int pk_species(unsigned long long w, int q) { return (int)((w >> (8 * q)) & 255u); }
+__device__ __forceinline__ int pk_nl(unsigned long long w) { return (int)((w >> 48) & 255u); }
+__device__ __forceinline__ int pk_nr(unsigned long long w) { return (int)(w >> 56); }
 
 struct DevOpts {
     int solver, jac, maxSteps;
@@ -352,10 +358,11 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
     const double PbyRT = PSTD / (RU * T);
     for (int r = lane_id(); r < M.nr; r += WAVE) {
         const int t = M.rtype[r];
-        const int nl = M.rnl[r], nrh = M.rnr[r];
+        const unsigned long long pk = M.rpack[r];
+        const int nl = pk_nl(pk), nrh = pk_nr(pk);
         int s[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) s[q] = M.rsp[q * M.nr + r];
+        for (int q = 0; q < 6; ++q) s[q] = pk_species(pk, q);
         const RxnK k = rate_constant<JAC>(M, L, r, t, T, lnT, iT, lnp);
         double kr = 0.0, dkrdT = 0.0, invKc = 0.0;
         if (M.rrev[r]) {
@@ -468,58 +475,97 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     wave_sync();
     reaction_round<true>(M, L, T, lnT, iT, lnp);
     wave_sync();
-    // species rows: sparse partials dq/dC into this lane's row
+    // species rows: sparse partials dq/dC into this lane's row.  The list is
+    // walked in blocks of 4 entries: all loads of a block are issued before
+    // its (sequential, order-preserving) updates.
     double om = 0.0, jT = 0.0;
     if (sp) {
-        for (int e = 0; e < M.maxDeg; ++e) {
-            const double nu = M.sp_nu[e * WAVE + lane];
-            if (nu == 0.0) continue;
-            const int r = M.sp_r[e * WAVE + lane];
-            const int nl = M.rnl[r], nrh = M.rnr[r];
-            int s[6];
-            double c[6];
+        for (int e0 = 0; e0 < M.maxDeg; e0 += 4) {
+            double nu[4], kf[4], kr[4], rd[4], c[4][6];
+            unsigned long long pk[4];
+            int r[4];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                s[q] = M.rsp[q * M.nr + r];
-                c[q] = L.C[s[q]];
+            for (int u = 0; u < 4; ++u) {
+                const bool in = e0 + u < M.maxDeg;
+                nu[u] = in ? M.sp_nu[(e0 + u) * WAVE + lane] : 0.0;
+                r[u] = in ? M.sp_r[(e0 + u) * WAVE + lane] : 0;
+                pk[u] = in ? M.sp_pack[(e0 + u) * WAVE + lane] : 0ull;
             }
-            const double kf = L.rq[r], kr = L.rk[r];
-            double CF = 1.0, CR = 1.0;
-            for (int q = 0; q < nl; ++q) CF *= c[q];
-            for (int q = 0; q < nrh; ++q) CR *= c[3 + q];
-            om += nu * (kf * CF - kr * CR);
-            jT += nu * L.rdT[r];
-            for (int q = 0; q < nl; ++q) {
-                double d = kf;
-                for (int b = 0; b < nl; ++b) if (b != q) d *= c[b];
-                Jrow[s[q] * WAVE] += nu * d;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                kf[u] = L.rq[r[u]];
+                kr[u] = L.rk[r[u]];
+                rd[u] = L.rdT[r[u]];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) c[u][q] = L.C[pk_species(pk[u], q)];
             }
-            if (kr != 0.0) {
-                for (int q = 0; q < nrh; ++q) {
-                    double d = kr;
-                    for (int b = 0; b < nrh; ++b) if (b != q) d *= c[3 + b];
-                    Jrow[s[3 + q] * WAVE] -= nu * d;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (nu[u] == 0.0) continue;
+                // (slot loops fully unrolled with guards: register-resident c[][])
+                const int nl = pk_nl(pk[u]), nrh = pk_nr(pk[u]);
+                double CF = 1.0, CR = 1.0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (q < nl) CF *= c[u][q];
+                    if (q < nrh) CR *= c[u][3 + q];
+                }
+                om += nu[u] * (kf[u] * CF - kr[u] * CR);
+                jT += nu[u] * rd[u];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (q >= nl) continue;
+                    double d = kf[u];
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) if (b < nl && b != q) d *= c[u][b];
+                    Jrow[pk_species(pk[u], q) * WAVE] += nu[u] * d;
+                }
+                if (kr[u] != 0.0) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        if (q >= nrh) continue;
+                        double d = kr[u];
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) if (b < nrh && b != q) d *= c[u][3 + b];
+                        Jrow[pk_species(pk[u], 3 + q) * WAVE] -= nu[u] * d;
+                    }
                 }
             }
         }
         // third-body / falloff partials: nu * w_r * eff_rj on every column
-        // (species-major efficiencies: the lanes' loads of one column share a row)
-        for (int e = 0; e < M.maxDegTB; ++e) {
-            const double nu = M.sptb_nu[e * WAVE + lane];
-            if (nu == 0.0) continue;
-            const int t = M.sptb_t[e * WAVE + lane];
-            const double coef = nu * L.rw[t];
-            for (int j0 = 0; j0 < ns; j0 += 8) {
-                double v[8], ef[8];
+        // (species-major efficiencies: the lanes' loads of one column share a
+        // row).  Blocks of 4 entries x 4 columns; each element still takes
+        // the entries' contributions in list order.
+        for (int e0 = 0; e0 < M.maxDegTB; e0 += 4) {
+            double coef[4];
+            int t[4];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j0 + u < ns) {
-                        v[u] = Jrow[(j0 + u) * WAVE];
-                        ef[u] = M.effT[(size_t)(j0 + u) * M.nTB + t];
+            for (int u = 0; u < 4; ++u) {
+                const bool in = e0 + u < M.maxDegTB;
+                const double nu = in ? M.sptb_nu[(e0 + u) * WAVE + lane] : 0.0;
+                t[u] = in ? M.sptb_t[(e0 + u) * WAVE + lane] : 0;
+                coef[u] = nu != 0.0 ? nu * L.rw[t[u]] : 0.0;
+            }
+            const bool any = coef[0] != 0.0 || coef[1] != 0.0 || coef[2] != 0.0 || coef[3] != 0.0;
+            if (!any) continue;
+            for (int j0 = 0; j0 < ns; j0 += 4) {
+                double v[4], ef[4][4];
+#pragma unroll
+                for (int w = 0; w < 4; ++w)
+                    if (j0 + w < ns) {
+                        v[w] = Jrow[(j0 + w) * WAVE];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) ef[u][w] = M.effT[(size_t)(j0 + w) * M.nTB + t[u]];
                     }
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j0 + u < ns) Jrow[(j0 + u) * WAVE] = v[u] + coef * ef[u];
+                for (int w = 0; w < 4; ++w)
+                    if (j0 + w < ns) {
+                        double acc = v[w];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (coef[u] != 0.0) acc += coef[u] * ef[u][w];
+                        Jrow[(j0 + w) * WAVE] = acc;
+                    }
             }
         }
     }
@@ -601,7 +647,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 // swapped as the reference swaps rows, and ip[r] is the lane holding
 // logical row r when the factorisation ends.  Right-looking elimination:
 // the pivot row is read by all lanes at one address.
-struct LuState { double invD; int pos; };
+struct LuState { double invD; int pos; int ipl; };   // ipl: lane holding logical row `lane`
 
 // |value| of column-k entry in the lane that holds logical row r.
 __device__ __forceinline__ double cand(double ak, int pos, int r)
@@ -612,6 +658,7 @@ __device__ __forceinline__ double cand(double ak, int pos, int r)
 template <int NP>
 __device__ inline LuState lu_factor(const Lds& L)
 {
+    static_assert(NP % 4 == 0, "padded system size must be a multiple of 4");
     const int lane = lane_id();
     const bool row = lane < NP;
     double* Jrow = L.J + lane;
@@ -660,28 +707,31 @@ __device__ inline LuState lu_factor(const Lds& L)
         const double inv = 1.0 / piv;
         if (lane == pl) invD = inv;
         const double* prow = L.J + pl;           // pivot row, broadcast reads
-        if (row && pos > k) {
-            const double l = ak * inv;
-            Jrow[k * WAVE] = l;
-            // batches of 8: all loads of a batch are issued before its stores
-            for (int j0 = k + 1; j0 < NP; j0 += 8) {
-                double v[8], pv[8];
+        // Every lane runs the update (uniform control flow); only rows below
+        // the pivot (pos > k) take it, and only in columns j > k.  The column
+        // range starts at a multiple of 4 (NP is one), so every block of 4
+        // is whole: its 8 loads are issued before its 4 stores.
+        const bool elim = row && pos > k;
+        const double l = ak * inv;
+        if (elim) Jrow[k * WAVE] = l;
+        for (int j0 = (k + 1) & ~3; j0 < NP; j0 += 4) {
+            double v[4], pv[4];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j0 + u < NP) {
-                        v[u] = Jrow[(j0 + u) * WAVE];
-                        pv[u] = prow[(j0 + u) * WAVE];
-                    }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j0 + u < NP) Jrow[(j0 + u) * WAVE] = v[u] - l * pv[u];
+            for (int u = 0; u < 4; ++u) {
+                v[u] = Jrow[(j0 + u) * WAVE];
+                pv[u] = prow[(j0 + u) * WAVE];
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                Jrow[(j0 + u) * WAVE] = (elim && j0 + u > k) ? v[u] - l * pv[u] : v[u];
         }
         wave_sync();
     }
     if (row) L.ip[pos] = lane;
     wave_sync();
-    return LuState{invD, pos};
+    const int ipl = row ? L.ip[lane] : lane;
+    wave_sync();
+    return LuState{invD, pos, ipl};
 }
 
 // Solves (L U) x = P b; b is this lane's element in the original row order
@@ -692,17 +742,23 @@ __device__ inline double lu_solve(const Lds& L, const LuState& st, double b)
 {
     const int lane = lane_id();
     const double* Jrow = L.J + lane;
+#pragma unroll
     for (int r = 0; r < NP - 1; ++r) {
-        const int src = L.ip[r];
+        const int src = readlane_i(st.ipl, r);
         const double yr = readlane_d(b, src);
-        if (st.pos > r && lane < NP) b -= Jrow[r * WAVE] * yr;
+        const double lr = Jrow[r * WAVE];            // unconditional: prefetchable
+        const double nb = b - lr * yr;
+        b = (st.pos > r && lane < NP) ? nb : b;
     }
     double x = 0.0;
+#pragma unroll
     for (int r = NP - 1; r >= 0; --r) {
-        const int src = L.ip[r];
+        const int src = readlane_i(st.ipl, r);
         if (lane == src) b *= st.invD;
         const double xr = readlane_d(b, src);
-        if (st.pos < r) b -= Jrow[r * WAVE] * xr;
+        const double ur = Jrow[r * WAVE];
+        const double nb = b - ur * xr;
+        b = st.pos < r ? nb : b;
         if (lane == r) x = xr;
     }
     return x;

@@ -499,12 +499,23 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         maxDeg = std::max(maxDeg, (int)lst[s].size());
         maxDegTB = std::max(maxDegTB, (int)lstTB[s].size());
     }
+    // packed reaction species: slots 0-2 left, 3-5 right (bytes 0-5), nl, nr (bytes 6, 7)
+    std::vector<unsigned long long> rpack(std::max(nr, 1), 0ull);
+    for (int r = 0; r < nr; ++r) {
+        unsigned long long w = 0;
+        for (int q = 0; q < 6; ++q) w |= (unsigned long long)(rsp[q * nr + r] & 255) << (8 * q);
+        w |= (unsigned long long)rnl[r] << 48;
+        w |= (unsigned long long)rnr[r] << 56;
+        rpack[r] = w;
+    }
+    std::vector<unsigned long long> spPack((size_t)maxDeg * 64, 0ull);
     std::vector<int> spR((size_t)maxDeg * 64, 0), spTB((size_t)maxDegTB * 64, 0);
     std::vector<double> spNu((size_t)maxDeg * 64, 0.0), spTBnu((size_t)maxDegTB * 64, 0.0);
     for (int s = 0; s < ns; ++s) {
         for (size_t e = 0; e < lst[s].size(); ++e) {
             spR[e * 64 + s] = lst[s][e].first;
             spNu[e * 64 + s] = lst[s][e].second;
+            spPack[e * 64 + s] = rpack[lst[s][e].first];
         }
         for (size_t e = 0; e < lstTB[s].size(); ++e) {
             spTB[e * 64 + s] = lstTB[s][e].first;
@@ -529,6 +540,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     const size_t iTroe = vd(troe), iEff = vd(eff), iEffT = vd(effT);
     const size_t iPn = vi(plogN), iPp = vd(plogLnp), iPA = vd(plogLnA), iPb = vd(plogBeta), iPT = vd(plogTa);
     const size_t iSpR = vi(spR), iSpNu = vd(spNu), iSpTB = vi(spTB), iSpTBnu = vd(spTBnu);
+    auto vq = [&](const std::vector<unsigned long long>& v) { return add(v.data(), v.size() * 8); };
+    const size_t iRpack = vq(rpack), iSpPack = vq(spPack);
     std::vector<unsigned char> host(total, 0);
     for (const auto& p : pieces)
         if (p.bytes) std::memcpy(host.data() + p.off, p.src, p.bytes);
@@ -586,6 +599,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.sp_nu = (const double*)P(iSpNu);
     dm.sptb_t = (const int*)P(iSpTB);
     dm.sptb_nu = (const double*)P(iSpTBnu);
+    dm.rpack = (const unsigned long long*)P(iRpack);
+    dm.sp_pack = (const unsigned long long*)P(iSpPack);
     m->ns = ns;
     m->nr = nr;
     m->perm = perm;
