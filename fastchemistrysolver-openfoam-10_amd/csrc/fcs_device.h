@@ -139,19 +139,17 @@ struct Lds {
     double* rdT;   // [nrP] dq/dT (Jacobian)
     double* rw;    // [nTBP] third-body partial (Jacobian)
     double* J;     // [NP*64] Jacobian, then iteration matrix and its LU factors
-    double* J2;    // [NP*64] Seulex only: Jy kept across the extrapolation columns
     int* ip;       // [64] lane holding each logical row after the LU
 };
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-__host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB, bool keepJy = false)
+__host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
 {
-    return 5 * 64 + 3 * (size_t)round_up(nr, 2) + (size_t)round_up(nTB > 0 ? nTB : 1, 2)
-           + (size_t)NP * 64 * (keepJy ? 2 : 1) + 32;
+    return 5 * 64 + 3 * (size_t)round_up(nr, 2) + (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * 64 + 32;
 }
 
-__device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, bool keepJy = false)
+__device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
 {
     Lds L;
     const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
@@ -165,8 +163,7 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, bool keepJ
     L.rdT = L.rk + nrP;
     L.rw = L.rdT + nrP;
     L.J = L.rw + nTBP;
-    L.J2 = L.J + (size_t)NP * 64;
-    L.ip = (int*)(L.J2 + (keepJy ? (size_t)NP * 64 : 0));
+    L.ip = (int*)(L.J + (size_t)NP * 64);
     return L;
 }
 

@@ -256,20 +256,31 @@ constexpr SeulexConst make_seulex_const()
 }
 __constant__ const SeulexConst kSeulex = make_seulex_const();
 
-// Jy is kept in L.J2 across the seul() calls of one step; each seul() forms
-// its iteration matrix from it in L.J.
+// The Jacobian Jy of a Seulex cell persists across its steps (it is
+// re-evaluated only when theta > jacRedo, src/ODE/OptSeulex/OptSeulex.C:
+// 314-318, 483-490), so it lives in HBM between step launches, per cell, in
+// the lane-interleaved layout of the LDS matrix: element (i, j) at
+// JyG[j*64 + i].  Each seul() forms its iteration matrix from it.
 template <int NP>
-__device__ __forceinline__ void jy_store(const Lds& L)
+__device__ __forceinline__ void jy_store(const Lds& L, double* JyG)
 {
     const int lane = lane_id();
-    for (int j = 0; j < NP; ++j) L.J2[j * WAVE + lane] = L.J[j * WAVE + lane];
-    wave_sync();
+    for (int j = 0; j < NP; ++j) JyG[j * WAVE + lane] = L.J[j * WAVE + lane];
 }
+
+// Iteration matrix diag*I - Jy in the LDS matrix, Jy read from HBM.
 template <int NP>
-__device__ __forceinline__ void jy_load(const Lds& L)
+__device__ __forceinline__ void form_matrix_from(const Lds& L, const double* JyG, double diag)
 {
     const int lane = lane_id();
-    for (int j = 0; j < NP; ++j) L.J[j * WAVE + lane] = L.J2[j * WAVE + lane];
+    double* Jrow = L.J + lane;
+    for (int j0 = 0; j0 < NP; j0 += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = JyG[(j0 + u) * WAVE + lane];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Jrow[(j0 + u) * WAVE] = (lane == j0 + u ? diag : 0.0) - v[u];
+    }
     wave_sync();
 }
 
@@ -277,13 +288,12 @@ __device__ __forceinline__ void jy_load(const Lds& L)
 // evaluation site.  y0s (this lane's y0_) is clipped in place by the first
 // RHS evaluation; returns false when the theta divergence test fails.
 template <int NP>
-__device__ bool seul(const DevMech& M, const Lds& L, double p, double lnp, double& y0s, double dxTot, int k,
-                     int nSteps, double scale, double& theta, double& ySeq)
+__device__ bool seul(const DevMech& M, const Lds& L, const double* JyG, double p, double lnp, double& y0s,
+                     double dxTot, int k, int nSteps, double scale, double& theta, double& ySeq)
 {
     const int lane = lane_id();
-    jy_load<NP>(L);
     const double invdx = nSteps / dxTot;
-    form_matrix<NP>(L, invdx);
+    form_matrix_from<NP>(L, JyG, invdx);
     const LuState st = lu_factor<NP>(L);
     double yT = 0.0, dy = 0.0;
     int nn = 0;
@@ -317,172 +327,186 @@ __device__ bool seul(const DevMech& M, const Lds& L, double p, double lnp, doubl
     return true;
 }
 
-// ODESolve + SeulexSolve (src/ODE/OptSeulex/OptSeulex.C:171-559).
+// Seulex controller state of one cell between launches.
+enum { SX_FIRST = 1, SX_LAST = 2, SX_PREVREJECT = 4, SX_JACUPD = 8, SX_INSTEP = 16 };
+
+struct SeulexCell {
+    double x, dxTry, dxNew, dxTry0, theta;
+    int kTarg, flags, nStep, nAtt;
+};
+
+// One pass of the extrapolation loop of OptSeulex (ODESolve + SeulexSolve,
+// src/ODE/OptSeulex/OptSeulex.C:171-559): the body of SeulexSolve's
+// `while (firstk || step.reject)` loop, i.e. columns k = 0..kTarg+1 from the
+// step-start state, preceded by the step set-up when a new step begins and
+// followed by the step-size / order update when the pass is accepted.  The
+// controller variables of the reference (theta_, kTarg_, the stepState
+// flags, dxNew, the step counters) are carried by the caller between calls,
+// with the per-lane state y, the step-start state y0s and the scale vector
+// (k11).  temp_ and dxOpt_ are only read at columns set earlier in the same
+// pass (temp_[0] = great at every pass), so they stay local.  Returns
+// STEP_CONTINUE while the cell is in flight, else its final status.
 template <int NP>
-__device__ int seulex_integrate(const DevMech& M, const Lds& L, const DevOpts& o, double p, double xEnd,
-                                double& y, double& dxTryIO, int& nAtt)
+__device__ int seulex_pass(const DevMech& M, const Lds& L, const DevOpts& o, double p, double xEnd,
+                           double* JyG, double& y, double& y0s, double& scale, SeulexCell& c)
 {
     const SeulexConst& S = kSeulex;
     const int lane = lane_id();
     const double lnp = log(p);
     const double kF1 = 0.7, kF2 = 0.9, sF1 = 0.8, sF2 = 0.93, sF3 = 0.1, sF4 = 4, sF5 = 0.5;
     const double jacRedo = o.relTol < 1e-4 ? o.relTol : 1e-4;
-    const double logTol = -log10(o.relTol + o.absTol) * 0.6 + 0.5;
-    double theta = 2 * jacRedo;
-    int kTarg = 0;
+    const double xStart = 0.0;
+    if (!(c.flags & SX_INSTEP)) {
+        // ODESolve loop head (OptSeulex.C:198-207) and SeulexSolve set-up (:283-318)
+        c.dxTry0 = c.dxTry;
+        if ((c.x + c.dxTry - xEnd) * (c.x + c.dxTry - xStart) > 0) {
+            c.flags |= SX_LAST;
+            c.dxTry = xEnd - c.x;
+        }
+        y0s = y;
+        if (c.flags & (SX_FIRST | SX_PREVREJECT)) c.theta = 2 * jacRedo;
+        if (c.flags & SX_FIRST) {
+            const double logTol = -log10(o.relTol + o.absTol) * 0.6 + 0.5;
+            int kt = (int)logTol;
+            kt = kt < SX_KMAX - 1 ? kt : SX_KMAX - 1;
+            c.kTarg = kt > 1 ? kt : 1;
+        }
+        scale = o.absTol + o.relTol * fabs(y);
+        c.flags &= ~SX_JACUPD;
+        if (c.theta > jacRedo) {
+            (void)jacobian<NP>(M, L, p, lnp, o.jac, y);
+            jy_store<NP>(L, JyG);
+            c.flags |= SX_JACUPD;
+        }
+        c.dxNew = fabs(c.dxTry);
+        c.flags |= SX_INSTEP;
+    }
     double dxOpt[SX_IMAX], temp[SX_IMAX];
     double table[SX_KMAX];
 #pragma unroll
     for (int q = 0; q < SX_KMAX; ++q) table[q] = 0.0;
-    // stepState (src/ODE/OptSeulex/OptSeulex.H:75-95)
-    double dxTry = dxTryIO;
-    bool first = true, last = false, reject = false, prevReject = false;
-    const double xStart = 0.0;
-    double x = xStart;
-    const int attemptCap = 100 * o.maxSteps;
-    for (int nStep = 0; nStep < o.maxSteps; ++nStep) {
-        const double dxTry0 = dxTry;
-        reject = false;
-        if ((x + dxTry - xEnd) * (x + dxTry - xStart) > 0) { last = true; dxTry = xEnd - x; }
-        // ---- SeulexSolve ----
-        temp[0] = OF_GREAT;
-        double dx = dxTry;
-        double y0s = y;
-        dxOpt[0] = fabs(0.1 * dx);
-        if (first || prevReject) theta = 2 * jacRedo;
-        if (first) {
-            int kt = (int)logTol;
-            kt = kt < SX_KMAX - 1 ? kt : SX_KMAX - 1;
-            kTarg = kt > 1 ? kt : 1;
+    temp[0] = OF_GREAT;
+    const bool first = c.flags & SX_FIRST, last = c.flags & SX_LAST;
+    bool prevReject = c.flags & SX_PREVREJECT;
+    const double dx = c.dxNew;
+    bool reject = false;
+    double errOld = 0;
+    int k = 0;
+    for (k = 0; k <= c.kTarg + 1; ++k) {
+        double ySeq = 0.0;
+        const bool ok = seul<NP>(M, L, JyG, p, lnp, y0s, dx, k, S.nSeq[k], scale, c.theta, ySeq);
+        ++c.nAtt;
+        if (!ok) { reject = true; c.dxNew = fabs(dx) * sF5; break; }
+        if (k == 0) {
+            y = ySeq;
+            continue;
         }
-        double scale = o.absTol + o.relTol * fabs(y);
-        bool jacUpdated = false;
-        bool needJac = theta > jacRedo;
-        int k = 0;
-        double dxNew = fabs(dx);
-        bool firstk = true;
-        while (firstk || reject) {
-            if (needJac) {
-                (void)jacobian<NP>(M, L, p, lnp, o.jac, y);
-                jy_store<NP>(L);
-                jacUpdated = true;
-                needJac = false;
-            }
-            dx = dxNew;
-            firstk = false;
-            reject = false;
-            double errOld = 0;
-            for (k = 0; k <= kTarg + 1; ++k) {
-                double ySeq = 0.0;
-                const bool ok = seul<NP>(M, L, p, lnp, y0s, dx, k, S.nSeq[k], scale, theta, ySeq);
-                ++nAtt;
-                if (!ok) { reject = true; dxNew = fabs(dx) * sF5; break; }
-                if (k == 0) {
-                    y = ySeq;
-                    continue;
-                }
-                // table[k-1] = ySeq, then extrapolate (OptSeulex.C:711-731)
+        // table[k-1] = ySeq, then extrapolate (OptSeulex.C:711-731)
 #pragma unroll
-                for (int q = 0; q < SX_KMAX; ++q)
-                    if (q == k - 1) table[q] = ySeq;
+        for (int q = 0; q < SX_KMAX; ++q)
+            if (q == k - 1) table[q] = ySeq;
 #pragma unroll
-                for (int j = SX_KMAX - 1; j > 0; --j)
-                    if (j <= k - 1) table[j - 1] = table[j] + S.coeff[k][j] * (table[j] - table[j - 1]);
-                y = table[0] + S.coeff[k][0] * (table[0] - y);
-                scale = o.absTol + o.relTol * fabs(y0s);
-                const double dd = (y - table[0]) / scale;
-                const double err = sqrt(wave_sum(lane < NP ? dd * dd : 0.0) / (M.ns + 1));
-                if (!(err == err)) return CELL_NONFINITE;
-                if (err > 1 / OF_SMALL || (k > 1 && err >= errOld)) {
-                    reject = true;
-                    dxNew = fabs(dx) * sF5;
-                    break;
-                }
-                errOld = 4 * err < 1 ? 4 * err : 1;
-                const double expo = 1.0 / (k + 1);
-                const double facmin = pow(sF3, expo);
-                double fac;
-                if (err == 0) fac = 1.0 / facmin;
-                else {
-                    fac = sF2 / pow(err / sF1, expo);
-                    const double hi = 1 / facmin < fac ? 1 / facmin : fac;
-                    fac = facmin / sF4 > hi ? facmin / sF4 : hi;
-                }
-                dxOpt[k] = fabs(dx * fac);
-                temp[k] = S.cpu[k] / dxOpt[k];
-                if ((first || last) && err <= 1.0) break;
-                if (k == kTarg - 1 && !prevReject && !first && !last) {
-                    if (err <= 1.0) break;
-                    else if (err > S.nSeq[kTarg] * S.nSeq[kTarg + 1] * 4.0) {
-                        reject = true;
-                        kTarg = k;
-                        if (kTarg > 1 && temp[k - 1] < kF1 * temp[k]) kTarg--;
-                        dxNew = dxOpt[kTarg];
-                        break;
-                    }
-                }
-                if (k == kTarg) {
-                    if (err <= 1.0) break;
-                    else if (err > S.nSeq[k + 1] * 2.0) {
-                        reject = true;
-                        if (kTarg > 1 && temp[k - 1] < kF1 * temp[k]) kTarg--;
-                        dxNew = dxOpt[kTarg];
-                        break;
-                    }
-                }
-                if (k == kTarg + 1) {
-                    if (err > 1.0) {
-                        reject = true;
-                        if (kTarg > 1 && temp[kTarg - 1] < kF1 * temp[kTarg]) kTarg--;
-                        dxNew = dxOpt[kTarg];
-                    }
-                    break;
-                }
+        for (int j = SX_KMAX - 1; j > 0; --j)
+            if (j <= k - 1) table[j - 1] = table[j] + S.coeff[k][j] * (table[j] - table[j - 1]);
+        y = table[0] + S.coeff[k][0] * (table[0] - y);
+        scale = o.absTol + o.relTol * fabs(y0s);
+        const double dd = (y - table[0]) / scale;
+        const double err = sqrt(wave_sum(lane < NP ? dd * dd : 0.0) / (M.ns + 1));
+        if (!(err == err)) return CELL_NONFINITE;
+        if (err > 1 / OF_SMALL || (k > 1 && err >= errOld)) {
+            reject = true;
+            c.dxNew = fabs(dx) * sF5;
+            break;
+        }
+        errOld = 4 * err < 1 ? 4 * err : 1;
+        const double expo = 1.0 / (k + 1);
+        const double facmin = pow(sF3, expo);
+        double fac;
+        if (err == 0) fac = 1.0 / facmin;
+        else {
+            fac = sF2 / pow(err / sF1, expo);
+            const double hi = 1 / facmin < fac ? 1 / facmin : fac;
+            fac = facmin / sF4 > hi ? facmin / sF4 : hi;
+        }
+        dxOpt[k] = fabs(dx * fac);
+        temp[k] = S.cpu[k] / dxOpt[k];
+        if ((first || last) && err <= 1.0) break;
+        if (k == c.kTarg - 1 && !prevReject && !first && !last) {
+            if (err <= 1.0) break;
+            else if (err > S.nSeq[c.kTarg] * S.nSeq[c.kTarg + 1] * 4.0) {
+                reject = true;
+                c.kTarg = k;
+                if (c.kTarg > 1 && temp[k - 1] < kF1 * temp[k]) c.kTarg--;
+                c.dxNew = dxOpt[c.kTarg];
+                break;
             }
-            if (reject) {
-                prevReject = true;
-                if (!jacUpdated) {
-                    theta = 2.0 * jacRedo;
-                    if (theta > jacRedo && !jacUpdated) needJac = true;
-                }
+        }
+        if (k == c.kTarg) {
+            if (err <= 1.0) break;
+            else if (err > S.nSeq[k + 1] * 2.0) {
+                reject = true;
+                if (c.kTarg > 1 && temp[k - 1] < kF1 * temp[k]) c.kTarg--;
+                c.dxNew = dxOpt[c.kTarg];
+                break;
             }
-            if (nAtt > attemptCap) return CELL_MAXSTEPS;
         }
-        x += dx;
-        int kopt;
-        if (k == 1) kopt = 2;
-        else if (k <= kTarg) {
-            kopt = k;
-            if (temp[k - 1] < kF1 * temp[k]) kopt = k - 1;
-            else if (temp[k] < kF2 * temp[k - 1]) kopt = (k + 1 < SX_KMAX - 1) ? k + 1 : SX_KMAX - 1;
-        } else {
-            kopt = k - 1;
-            if (k > 2 && temp[k - 2] < kF1 * temp[k - 1]) kopt = k - 2;
-            if (temp[k] < kF2 * temp[kopt]) kopt = (k < SX_KMAX - 1) ? k : SX_KMAX - 1;
-        }
-        if (prevReject) {
-            kTarg = kopt < k ? kopt : k;
-            dxNew = fabs(dx) < dxOpt[kTarg] ? fabs(dx) : dxOpt[kTarg];
-            prevReject = false;
-        } else {
-            if (kopt <= k) dxNew = dxOpt[kopt];
-            else {
-                if (k < kTarg && temp[k] < kF2 * temp[k - 1]) dxNew = dxOpt[k] * S.cpu[kopt + 1] / S.cpu[k];
-                else dxNew = dxOpt[k] * S.cpu[kopt] / S.cpu[k];
+        if (k == c.kTarg + 1) {
+            if (err > 1.0) {
+                reject = true;
+                if (c.kTarg > 1 && temp[c.kTarg - 1] < kF1 * temp[c.kTarg]) c.kTarg--;
+                c.dxNew = dxOpt[c.kTarg];
             }
-            kTarg = kopt;
+            break;
         }
-        dxTry = dxNew;
-        // ---- end SeulexSolve ----
-        if ((x - xEnd) * (xEnd - xStart) >= 0) {
-            if (nStep > 0 && last) dxTry = dxTry0;
-            dxTryIO = dxTry;
-            return 0;
-        }
-        first = false;
-        if (reject) prevReject = true;
     }
-    return CELL_MAXSTEPS;
+    if (reject) {
+        c.flags |= SX_PREVREJECT;
+        if (!(c.flags & SX_JACUPD)) {
+            c.theta = 2.0 * jacRedo;
+            (void)jacobian<NP>(M, L, p, lnp, o.jac, y);
+            jy_store<NP>(L, JyG);
+            c.flags |= SX_JACUPD;
+        }
+        if (c.nAtt > 100 * o.maxSteps) return CELL_MAXSTEPS;
+        return STEP_CONTINUE;
+    }
+    if (c.nAtt > 100 * o.maxSteps) return CELL_MAXSTEPS;
+    // accepted pass: step-size and order update (OptSeulex.C:495-557)
+    c.flags &= ~SX_INSTEP;
+    c.x += dx;
+    int kopt;
+    if (k == 1) kopt = 2;
+    else if (k <= c.kTarg) {
+        kopt = k;
+        if (temp[k - 1] < kF1 * temp[k]) kopt = k - 1;
+        else if (temp[k] < kF2 * temp[k - 1]) kopt = (k + 1 < SX_KMAX - 1) ? k + 1 : SX_KMAX - 1;
+    } else {
+        kopt = k - 1;
+        if (k > 2 && temp[k - 2] < kF1 * temp[k - 1]) kopt = k - 2;
+        if (temp[k] < kF2 * temp[kopt]) kopt = (k < SX_KMAX - 1) ? k : SX_KMAX - 1;
+    }
+    double dxNew;
+    if (prevReject) {
+        c.kTarg = kopt < k ? kopt : k;
+        dxNew = fabs(dx) < dxOpt[c.kTarg] ? fabs(dx) : dxOpt[c.kTarg];
+        c.flags &= ~SX_PREVREJECT;
+    } else {
+        if (kopt <= k) dxNew = dxOpt[kopt];
+        else {
+            if (k < c.kTarg && temp[k] < kF2 * temp[k - 1]) dxNew = dxOpt[k] * S.cpu[kopt + 1] / S.cpu[k];
+            else dxNew = dxOpt[k] * S.cpu[kopt] / S.cpu[k];
+        }
+        c.kTarg = kopt;
+    }
+    c.dxTry = dxNew;
+    // ODESolve loop tail (OptSeulex.C:236-251)
+    if ((c.x - xEnd) * (xEnd - xStart) >= 0) {
+        if (c.nStep > 0 && last) c.dxTry = c.dxTry0;
+        return 0;
+    }
+    c.flags &= ~SX_FIRST;
+    if (++c.nStep >= o.maxSteps) return CELL_MAXSTEPS;
+    return STEP_CONTINUE;
 }
 
 }  // namespace fcs

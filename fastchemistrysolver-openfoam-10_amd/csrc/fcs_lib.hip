@@ -33,6 +33,10 @@ struct CellArrays {
     // integrator state
     double *Yw, *x, *dxTry, *dx, *dxTry0;
     int *flags, *nStep;
+    // Seulex only: step-start state and scale vector [ns+1][nc], theta_ and
+    // kTarg_ [nc], and the per-cell Jacobian [nc][NP*64] kept across steps
+    double *Y0s, *Scl, *theta, *Jy;
+    int* kTarg;
 };
 
 // Clears the active-list counters and the deltaTMin accumulator (a kernel
@@ -70,9 +74,13 @@ __global__ __launch_bounds__(256) void k_setup(DevOpts o, int ns, CellArrays a, 
     a.dxTry[c] = a.dtcIn[c];
     a.dx[c] = 0.0;
     a.dxTry0[c] = 0.0;
-    a.flags[c] = 0;
+    a.flags[c] = o.solver == SOLVER_SEULEX ? SX_FIRST : 0;
     a.nStep[c] = 0;
     a.nAtt[c] = 0;
+    if (o.solver == SOLVER_SEULEX) {
+        a.theta[c] = 0.0;    // set at the first step (OptSeulex.C:291-294)
+        a.kTarg[c] = 0;
+    }
     list[atomicAdd(count, 1u)] = c;
 }
 
@@ -141,6 +149,63 @@ __global__ __launch_bounds__(64) void k_step(DevMech M, DevOpts o, CellArrays a,
             }
         } else {
             finish_cell<NP>(M, o, a, c, st, y, dxTry, nAtt);
+        }
+    }
+}
+
+// One pass of the Seulex extrapolation loop for every cell of the active
+// list (same list protocol as k_step).  The per-cell Jacobian lives in
+// a.Jy between launches.
+template <int NP>
+__global__ __launch_bounds__(64) void k_step_seulex(DevMech M, DevOpts o, CellArrays a, const int* listIn,
+                                                    const unsigned int* countIn, int* listOut,
+                                                    unsigned int* countOut, unsigned int* countZero)
+{
+    extern __shared__ double lds_raw[];
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const int lane = lane_id();
+    const int ns = M.ns;
+    const size_t nc = (size_t)a.nc;
+    const int row = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
+    const unsigned int n = *countIn;
+    if (blockIdx.x == 0 && lane == 0) *countZero = 0u;
+    for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int c = listIn[i];
+        double y = row >= 0 ? a.Yw[row * nc + c] : 0.0;
+        double y0s = row >= 0 ? a.Y0s[row * nc + c] : 0.0;
+        double scale = row >= 0 ? a.Scl[row * nc + c] : o.absTol;
+        SeulexCell sc;
+        sc.x = a.x[c];
+        sc.dxTry = a.dxTry[c];
+        sc.dxNew = a.dx[c];
+        sc.dxTry0 = a.dxTry0[c];
+        sc.theta = a.theta[c];
+        sc.kTarg = a.kTarg[c];
+        sc.flags = a.flags[c];
+        sc.nStep = a.nStep[c];
+        sc.nAtt = a.nAtt[c];
+        double* JyG = a.Jy + (size_t)c * NP * WAVE;
+        const int st = seulex_pass<NP>(M, L, o, a.p0[c], a.dt, JyG, y, y0s, scale, sc);
+        if (st == STEP_CONTINUE) {
+            if (row >= 0) {
+                a.Yw[row * nc + c] = y;
+                a.Y0s[row * nc + c] = y0s;
+                a.Scl[row * nc + c] = scale;
+            }
+            if (lane == 0) {
+                a.x[c] = sc.x;
+                a.dxTry[c] = sc.dxTry;
+                a.dx[c] = sc.dxNew;
+                a.dxTry0[c] = sc.dxTry0;
+                a.theta[c] = sc.theta;
+                a.kTarg[c] = sc.kTarg;
+                a.flags[c] = sc.flags;
+                a.nStep[c] = sc.nStep;
+                a.nAtt[c] = sc.nAtt;
+                listOut[atomicAdd(countOut, 1u)] = c;
+            }
+        } else {
+            finish_cell<NP>(M, o, a, c, st, y, sc.dxTry, sc.nAtt);
         }
     }
 }
@@ -230,7 +295,7 @@ namespace {
 
 struct KernelSet {
     int NP;
-    const void* step[2];        // Rodas34, Rosenbrock34
+    const void* step[3];        // Rodas34, Rosenbrock34, Seulex
     const void* deriv;
     const void* jac;
     const void* lu;
@@ -239,7 +304,8 @@ struct KernelSet {
 
 #define FCS_KSET(N)                                                                                        \
     {N,                                                                                                    \
-     {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>},                       \
+     {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>,                        \
+      (const void*)k_step_seulex<N>},                                                                      \
      (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>,                \
      {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>}},
 const KernelSet kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET)};
@@ -295,8 +361,10 @@ struct DevCells {
     void* base = nullptr;
     double *T0 = nullptr, *p0 = nullptr, *rho0 = nullptr, *Y0 = nullptr, *dtcIn = nullptr, *dtcOut = nullptr,
            *RR = nullptr, *Tend = nullptr, *Yw = nullptr, *x = nullptr, *dxTry = nullptr, *dx = nullptr,
-           *dxTry0 = nullptr;
-    int *status = nullptr, *nAtt = nullptr, *flags = nullptr, *nStep = nullptr, *list0 = nullptr, *list1 = nullptr;
+           *dxTry0 = nullptr, *Y0s = nullptr, *Scl = nullptr, *theta = nullptr, *Jy = nullptr;
+    int *status = nullptr, *nAtt = nullptr, *flags = nullptr, *nStep = nullptr, *list0 = nullptr, *list1 = nullptr,
+        *kTarg = nullptr;
+    size_t jyPerCell = 0;                    // doubles of Jy per cell (Seulex), 0 otherwise
     unsigned int* counts = nullptr;          // [3] active-list lengths
     unsigned long long* dtMin = nullptr;
     unsigned int* hostCount = nullptr;       // pinned [kPoll]
@@ -307,12 +375,15 @@ struct DevCells {
     long long lastLaunches = 0;
     double lastStepMs = 0.0;
 
-    int alloc(size_t nc, int nsp)
+    // jyDoubles: per-cell Jacobian storage of the Seulex solver (0 for the
+    // Rosenbrock solvers).
+    int alloc(size_t nc, int nsp, size_t jyDoubles)
     {
         release();
         const size_t d = align256(nc * 8), i = align256(nc * 4);
         const size_t nsd = align256(nc * nsp * 8), nwd = align256(nc * (nsp + 1) * 8);
-        const size_t total = 10 * d + 2 * nsd + nwd + 6 * i + 256 + 256;
+        const size_t jyb = align256(nc * jyDoubles * 8);
+        const size_t total = 11 * d + 2 * nsd + 3 * nwd + 7 * i + jyb + 256 + 256;
         if (hipMalloc(&base, total) != hipSuccess) {
             base = nullptr;
             return FCS_E_NOMEM;
@@ -321,10 +392,13 @@ struct DevCells {
         auto take = [&](size_t bytes) { char* r = q; q += bytes; return r; };
         T0 = (double*)take(d); p0 = (double*)take(d); rho0 = (double*)take(d); dtcIn = (double*)take(d);
         dtcOut = (double*)take(d); Tend = (double*)take(d); x = (double*)take(d); dxTry = (double*)take(d);
-        dx = (double*)take(d); dxTry0 = (double*)take(d);
+        dx = (double*)take(d); dxTry0 = (double*)take(d); theta = (double*)take(d);
         Y0 = (double*)take(nsd); RR = (double*)take(nsd); Yw = (double*)take(nwd);
+        Y0s = (double*)take(nwd); Scl = (double*)take(nwd);
         status = (int*)take(i); nAtt = (int*)take(i); flags = (int*)take(i); nStep = (int*)take(i);
-        list0 = (int*)take(i); list1 = (int*)take(i);
+        list0 = (int*)take(i); list1 = (int*)take(i); kTarg = (int*)take(i);
+        Jy = jyDoubles ? (double*)take(jyb) : nullptr;
+        jyPerCell = jyDoubles;
         counts = (unsigned int*)take(256);
         dtMin = (unsigned long long*)take(256);
         if (hipHostMalloc((void**)&hostCount, kPoll * sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
@@ -362,6 +436,7 @@ struct DevCells {
         a.T0 = T0; a.p0 = p0; a.rho0 = rho0; a.Y0 = Y0; a.dtcIn = dtcIn;
         a.dtcOut = dtcOut; a.RR = RR; a.Tend = Tend; a.status = status; a.nAtt = nAtt; a.dtMinBits = dtMin;
         a.Yw = Yw; a.x = x; a.dxTry = dxTry; a.dx = dx; a.dxTry0 = dxTry0; a.flags = flags; a.nStep = nStep;
+        a.Y0s = Y0s; a.Scl = Scl; a.theta = theta; a.Jy = Jy; a.kTarg = kTarg;
         return a;
     }
 };
@@ -621,8 +696,7 @@ static int solver_grid(fcs_solver* s)
     int dev = 0, cus = 0, per = 0;
     HIP_OK(hipGetDevice(&dev));
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int k = s->opts.solver == FCS_SOLVER_ROSENBROCK34 ? 1 : 0;
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, s->ks->step[k], 64, s->ldsBytes));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, s->ks->step[s->opts.solver], 64, s->ldsBytes));
     s->grid = std::max(1, per) * std::max(1, cus);
     return FCS_OK;
 }
@@ -646,7 +720,8 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
         return FCS_E_HIP;
     }
     if (s->ldsBytes > 65536)
-        for (const void* f : {ks->step[0], ks->step[1], ks->deriv, ks->jac, ks->lu, ks->attempt[0], ks->attempt[1]})
+        for (const void* f : {ks->step[0], ks->step[1], ks->step[2], ks->deriv, ks->jac, ks->lu, ks->attempt[0],
+                              ks->attempt[1]})
             (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->ldsBytes);
     if (solver_grid(s) != FCS_OK) {
         (void)hipStreamDestroy(s->stream);
@@ -664,6 +739,12 @@ int fcs_solver_destroy(fcs_solver* s)
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return FCS_OK;
+}
+
+// Per-cell Jacobian storage of the Seulex solver (doubles), 0 otherwise.
+static size_t jy_doubles(const fcs_solver* s)
+{
+    return s->opts.solver == FCS_SOLVER_SEULEX ? (size_t)s->ks->NP * WAVE : 0;
 }
 
 static DevOpts dev_opts(const fcs_solver_opts& o)
@@ -688,7 +769,6 @@ static DevOpts dev_opts(const fcs_solver_opts& o)
 // steps plus the rejection cap), which bounds the launch count.
 static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt)
 {
-    if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
     hipStream_t st = s->stream;
     CellArrays a = d.view(nc, dt);
     DevMech M = s->mech->dm;
@@ -703,7 +783,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt)
         void* args[] = {&o, &ns, &a, &d.list0, &c0};
         HIP_OK(hipLaunchKernel((const void*)k_setup, dim3((nc + 255) / 256), dim3(256), args, 0, st));
     }
-    const void* f = s->ks->step[s->opts.solver == FCS_SOLVER_ROSENBROCK34 ? 1 : 0];
+    const void* f = s->ks->step[s->opts.solver];
     const int grid = std::max(1, std::min(s->grid, nc));
     const long long maxLaunches = 101LL * s->opts.maxSteps + 2 * DevCells::kPoll + 2;
     for (long long it = 0;; ++it) {
@@ -754,7 +834,6 @@ int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0,
     if (!s || nCells < 0 || nCells > 0x7fffffff || !(deltaT > 0) || (nCells > 0 && (!T0 || !p0 || !rho0 || !Y0 ||
                                                                                    !deltaTChem || !RR)))
         return FCS_E_ARG;
-    if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
     const int ns = s->mech->ns;
     const size_t nc = (size_t)nCells;
     if (nc == 0) {
@@ -762,7 +841,8 @@ int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0,
         return FCS_OK;
     }
     DevCells& d = s->cells;
-    if (nc > d.cap && d.alloc(nc, ns) != FCS_OK) return FCS_E_NOMEM;
+    const size_t jy = jy_doubles(s);
+    if ((nc > d.cap || jy > d.jyPerCell) && d.alloc(nc, ns, jy) != FCS_OK) return FCS_E_NOMEM;
     hipStream_t st = s->stream;
     HIP_OK(hipMemcpyAsync(d.T0, T0, nc * 8, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d.p0, p0, nc * 8, hipMemcpyHostToDevice, st));
@@ -805,12 +885,11 @@ int fcs_solve(fcs_solver* s, int64_t nCells, double deltaT, const double* T0, co
 int fcs_batch_create(fcs_solver* s, int64_t nCells, fcs_batch** out)
 {
     if (!s || !out || nCells < 1 || nCells > 0x7fffffff) return FCS_E_ARG;
-    if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
     fcs_batch* b = new (std::nothrow) fcs_batch();
     if (!b) return FCS_E_NOMEM;
     b->s = s;
     b->nc = nCells;
-    if (b->cells.alloc((size_t)nCells, s->mech->ns) != FCS_OK || hipEventCreate(&b->e0) != hipSuccess ||
+    if (b->cells.alloc((size_t)nCells, s->mech->ns, jy_doubles(s)) != FCS_OK || hipEventCreate(&b->e0) != hipSuccess ||
         hipEventCreate(&b->e1) != hipSuccess) {
         fcs_batch_destroy(b);
         return FCS_E_NOMEM;

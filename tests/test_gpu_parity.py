@@ -16,12 +16,6 @@ pytestmark = pytest.mark.gpu
 
 DY_ABS, DY_REL, DT_ABS = 1e-6, 1e-4, 1e-3
 
-SEULEX_PENDING = pytest.mark.skip(
-    reason="OptSeulex is not yet available on the GPU: its step-kernel form is pending (fcs_solve returns "
-           "FCS_E_UNSUPPORTED); the earlier single-kernel integrator was withdrawn after it stalled on the GPU "
-           "with no cause found")
-
-
 @pytest.fixture(scope="module")
 def fcslib():
     from fcs import lib
@@ -152,8 +146,7 @@ def test_lu_pivot_quirk_on_device(fcslib, mech0):
 
 
 # ------------------------------------------------------------ solve(deltaT)
-@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34",
-                                    pytest.param("OptSeulex", marks=SEULEX_PENDING)])
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34", "OptSeulex"])
 def test_solve_config0_all_cells(fcslib, mech0, solver):
     field = M.config_field(0, mech0)
     dt = M.CONFIG_DT[0]
@@ -171,7 +164,7 @@ def test_solve_config0_fast_jacobian(fcslib, mech0):
     assert_solve_parity(g, o, field, dt)
 
 
-@pytest.mark.parametrize("solver", ["OptRodas34", pytest.param("OptSeulex", marks=SEULEX_PENDING)])
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptSeulex"])
 def test_solve_config1_sample(fcslib, mech1, solver):
     field = M.config_field(1, mech1).subset(np.random.default_rng(1).choice(65536, 192, replace=False))
     dt = M.CONFIG_DT[1]
@@ -245,12 +238,28 @@ def test_maxsteps_and_nonfinite_are_reported(fcslib, mech0):
     np.testing.assert_array_equal(np.isnan(g3["RR"]).any(axis=0), np.arange(8) == 2)
 
 
-def test_seulex_reports_unsupported(fcslib, mech0):
+def test_seulex_batch_and_fast_jacobian(fcslib, mech0):
+    """OptSeulex through the device-resident batch (the per-cell Jacobian is
+    kept in HBM across steps) with jacobian fast, against the oracle and
+    against fcs_solve; the single-attempt entry has no Seulex form."""
     from fcs.abi import FCS_E_UNSUPPORTED
     from fcs.lib import FcsError
-    s = fcslib.Solver(fcslib.Mech(mech0), solver="OptSeulex")
+    field = M.config_field(0, mech0, n_cells=256)
+    dt = M.CONFIG_DT[0]
+    s = fcslib.Solver(fcslib.Mech(mech0), solver="OptSeulex", jacobian="fast")
+    g = s.solve(field, dt)
+    o = Oracle(mech0).solve(field, dt, threads=1, solver="OptSeulex", jacobian="fast")
+    assert g["rc"] == 0 and o["rc"] == 0
+    assert_solve_parity(g, o, field, dt)
+    # column counts follow the same decisions except where round-off tips one
+    assert np.mean(g["nAttempts"] == o["nAttempts"]) >= 0.9
+    b = fcslib.Batch(s, field)
+    b.run(dt)
+    r = b.download()
+    for k in ("RR", "deltaTChem", "Tend", "status", "nAttempts"):
+        np.testing.assert_array_equal(r[k], g[k])
     with pytest.raises(FcsError) as e:
-        s.solve(M.config_field(0, mech0, n_cells=8), 1e-5)
+        s.attempt(field.p[:4], np.vstack([field.Y[:, :4], field.T[None, :4]]), np.full(4, 1e-9))
     assert e.value.code == FCS_E_UNSUPPORTED
 
 
