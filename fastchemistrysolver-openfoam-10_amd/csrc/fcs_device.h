@@ -31,6 +31,11 @@ constexpr double OF_SMALL = 1.0e-15;    // OpenFOAM-10 doubleScalarSmall
 constexpr double OF_GREAT = 1.0e15;     // OpenFOAM-10 doubleScalarGreat
 constexpr double OF_VGREAT = 1.0e300;   // OpenFOAM-10 doubleScalarVGreat
 constexpr int WAVE = 64;
+// Column stride (doubles) of the per-cell matrix in LDS.  65 rather than 64:
+// a row-wise access (lane = row, one column) is conflict-free either way,
+// and a column-wise one (lane = column, one row) spreads over the banks
+// instead of hitting one bank with every lane.
+constexpr int JS = 65;
 
 enum { RXN_ARR = 0, RXN_TB = 1, RXN_LIND = 2, RXN_TROE = 3, RXN_PLOG = 4 };
 enum { SOLVER_RODAS34 = 0, SOLVER_ROS34 = 1, SOLVER_SEULEX = 2 };
@@ -43,7 +48,8 @@ struct DevMech {
     int ns, nr, NPm;          // species, reactions, padded size used for tables
     int nTB;                  // third-body + falloff rows
     int pmax;                 // PLOG points per row
-    int maxDeg, maxDegTB;     // per-species list lengths (padded)
+    int maxDeg;               // per-species list length, padded to a multiple of 16
+    int nTBpad;               // row stride of effT: nTB padded to a multiple of 128
     double TlowMin, ThighMax, TcMin, TcMax;
     const double* W;          // [64]  (0 beyond ns)
     const double* invW;       // [64]  (0 beyond ns)
@@ -64,21 +70,26 @@ struct DevMech {
     const double* rbeta0;
     const double* rTa0;
     const double* troe;       // [4*nTroe] (alpha, Tsss, Ts, Tss) per row
+    const double* rtroe;      // [4*nr] coefficient-major rtroe[q*nr + r], zero for non-Troe rows
     const double* eff;        // [nTB*64] row-major, zero beyond ns
-    const double* effT;       // [64*nTB] species-major for M sums
+    const double* effT;       // [64*nTBpad] species-major for M sums (zero padding)
     const int* plog_n;        // [nPlog]
     const double* plog_lnp;   // [nPlog*pmax]
     const double* plog_lnA;
     const double* plog_beta;
     const double* plog_Ta;
-    const int* sp_r;          // [maxDeg*64] lane-interleaved (reaction)
-    const double* sp_nu;      // [maxDeg*64] net stoichiometric coefficient (0 = pad)
-    const int* sptb_t;        // [maxDegTB*64] eff row of third-body/falloff entries
-    const double* sptb_nu;    // [maxDegTB*64]
+    const int* sp_rn;         // [maxDeg*64] lane-interleaved: reaction | (nu + 8) << 16, nu = net
+                              // stoichiometric coefficient (padding: reaction 0, nu 0)
+    int nTBz;                 // third-body/falloff (species, row) entries, padded to a multiple of 16
+    const int* tbz_row;       // [nTBz] species, ascending (padding repeats the last)
+    const int* tbz_tb;        // [nTBz] third-body row (padding: 0)
+    const double* tbz_nu;     // [nTBz] net stoichiometric coefficient (padding: 0)
     const unsigned long long* rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
     const unsigned long long* sp_pack;  // [maxDeg*64] rpack of each per-species list entry
 };
 
+__device__ __forceinline__ int rn_r(int w) { return w & 0xffff; }
+__device__ __forceinline__ int rn_nu(int w) { return (w >> 16) - 8; }
 __device__ __forceinline__ int pk_species(unsigned long long w, int q) { return (int)((w >> (8 * q)) & 255u); }
 __device__ __forceinline__ int pk_nl(unsigned long long w) { return (int)((w >> 48) & 255u); }
 __device__ __forceinline__ int pk_nr(unsigned long long w) { return (int)(w >> 56); }
@@ -101,19 +112,36 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
 
 __device__ __forceinline__ int readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-// Butterfly sum: every lane ends with the same bits (each step adds a
-// commutative pair).
+// Wave reductions that leave the same bits in every lane: within each
+// 16-lane row, DPP exchanges (quad xor 1, quad xor 2, half-row mirror, row
+// mirror) add commutative pairs, so all lanes of a row hold the row total;
+// the four row totals are then combined as (r0 + r1) + (r2 + r3) from
+// uniform (scalar) copies.  No LDS traffic.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, WAVE);
-    return v;
+    v += dpp_d<DPP_XOR1>(v);
+    v += dpp_d<DPP_XOR2>(v);
+    v += dpp_d<DPP_HALF_MIRROR>(v);
+    v += dpp_d<DPP_MIRROR>(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 __device__ __forceinline__ double wave_max(double v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, WAVE));
-    return v;
+    v = fmax(v, dpp_d<DPP_XOR1>(v));
+    v = fmax(v, dpp_d<DPP_XOR2>(v));
+    v = fmax(v, dpp_d<DPP_HALF_MIRROR>(v));
+    v = fmax(v, dpp_d<DPP_MIRROR>(v));
+    return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
 }
 __device__ __forceinline__ int first_lane(bool pred)
 {
@@ -125,45 +153,52 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // ---------------------------------------------------------------- LDS layout
 // The per-cell matrix lives in LDS, lane-interleaved column-major: element
-// (row i, column j) at J[j*64 + i], row i owned by lane i.  Column accesses
-// of one row index across the lanes are conflict-free; the pivot row of the
-// LU is read by all lanes at one address (broadcast).
+// (row i, column j) at J[j*JS + i], row i owned by lane i.  Column accesses
+// of one row index across the lanes are conflict-free; the pivot rows of the
+// LU are read by all lanes at one address (broadcast).
 struct Lds {
     double* C;     // [64] concentrations
     double* expG;  // [64] exp(-G/RT)
     double* dBdT;  // [64] d(-G/RT)/dT
-    double* H;     // [64] species enthalpy (T-row column sums)
-    double* Q;     // [64] dY/dt * cp (T-row column sums)
     double* rq;    // [nrP] q (rates) or kf (Jacobian)
     double* rk;    // [nrP] kr (Jacobian)
     double* rdT;   // [nrP] dq/dT (Jacobian)
+    double* H;     // [64] species enthalpy (T-row column sums), over rk once the
+    double* Q;     // [64] dY/dt * cp (T-row column sums)   sparse partials are in
     double* rw;    // [nTBP] third-body partial (Jacobian)
-    double* J;     // [NP*64] Jacobian, then iteration matrix and its LU factors
+    double* Mtb;   // [nTBP] third-body concentration M of each third-body/falloff row
+    double* J;     // [NP*JS] Jacobian, then iteration matrix and its LU factors
     int* ip;       // [64] lane holding each logical row after the LU
 };
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// LDS doubles per wavefront; four wavefronts of the 53-species mechanism fit
+// the 160 KiB of a CU (40,112 B each).
 __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
 {
-    return 5 * 64 + 3 * (size_t)round_up(nr, 2) + (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * 64 + 32;
+    const size_t nrP = round_up(nr, 2);
+    const size_t kq = 2 * nrP > 128 ? 2 * nrP : 128;   // rk + rdT, later H + Q
+    return 3 * 64 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32;
 }
 
 __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
 {
     Lds L;
     const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
+    const int kq = 2 * nrP > 128 ? 2 * nrP : 128;
     L.C = base;
     L.expG = base + 64;
     L.dBdT = base + 128;
-    L.H = base + 192;
-    L.Q = base + 256;
-    L.rq = base + 320;
+    L.rq = base + 192;
     L.rk = L.rq + nrP;
     L.rdT = L.rk + nrP;
-    L.rw = L.rdT + nrP;
-    L.J = L.rw + nTBP;
-    L.ip = (int*)(L.J + (size_t)NP * 64);
+    L.H = L.rk;
+    L.Q = L.rk + 64;
+    L.rw = L.rk + kq;
+    L.Mtb = L.rw + nTBP;
+    L.J = L.Mtb + nTBP;
+    L.ip = (int*)(L.J + (size_t)NP * JS);
     return L;
 }
 
@@ -172,13 +207,31 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
 // coefficient choice follows setPtrCoeffs (ThermoFunctionI.H:1-31).
 struct Thermo { double cp, ha, negG, dBdT, dcpdT; };
 
-__device__ inline Thermo species_thermo(const DevMech& M, int s, double T, double lnT, double iT)
+// Per-species table entries of lane s (both NASA-7 ranges), loaded before
+// the temperature is known so that the loads overlap the density sum.
+struct SpeciesData { double lo[7], hi[7], Tc, invW, W; };
+
+__device__ __forceinline__ SpeciesData load_species(const DevMech& M, int s)
 {
-    const bool low = (T < M.TcMin) ? true : (T > M.TcMax ? false : (T < M.Tc[s]));
-    const double* c = low ? M.clo : M.chi;
-    const double a0 = c[0 * 64 + s], a1 = c[1 * 64 + s], a2 = c[2 * 64 + s], a3 = c[3 * 64 + s],
-                 a4 = c[4 * 64 + s], a5 = c[5 * 64 + s], a6 = c[6 * 64 + s];
-    const double r = RU * M.invW[s];
+    SpeciesData d;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        d.lo[k] = M.clo[k * 64 + s];
+        d.hi[k] = M.chi[k * 64 + s];
+    }
+    d.Tc = M.Tc[s];
+    d.invW = M.invW[s];
+    d.W = M.W[s];
+    return d;
+}
+
+__device__ inline Thermo species_thermo(const DevMech& M, const SpeciesData& S, double T, double lnT, double iT)
+{
+    const bool low = (T < M.TcMin) ? true : (T > M.TcMax ? false : (T < S.Tc));
+    const double a0 = low ? S.lo[0] : S.hi[0], a1 = low ? S.lo[1] : S.hi[1], a2 = low ? S.lo[2] : S.hi[2],
+                 a3 = low ? S.lo[3] : S.hi[3], a4 = low ? S.lo[4] : S.hi[4], a5 = low ? S.lo[5] : S.hi[5],
+                 a6 = low ? S.lo[6] : S.hi[6];
+    const double r = RU * S.invW;
     Thermo t;
     t.cp = ((((a4 * T + a3) * T + a2) * T + a1) * T + a0) * r;
     t.ha = (((((a4 * 0.2 * T + a3 * 0.25) * T + a2 * (1.0 / 3.0)) * T + a1 * 0.5) * T + a0) * T + a5) * r;
@@ -198,14 +251,45 @@ __device__ __forceinline__ double clampT(const DevMech& M, double T)
 // ---------------------------------------------------------------- rates
 struct RxnK { double kf, dkfdT, dkfdM; };
 
+// Table entries of one reaction, loaded together and unconditionally (the
+// tables cover every reaction), so the loads of a round are all in flight
+// before the first use.
+struct RxnData {
+    int t, rev, tb;
+    unsigned long long pk;
+    double A, b, Ta, A0, b0, Ta0, alpha, Tsss, Ts, Tss, Mc;
+};
+
+__device__ __forceinline__ RxnData load_rxn(const DevMech& M, const Lds& L, int r)
+{
+    RxnData d;
+    d.t = M.rtype[r];
+    d.rev = M.rrev[r];
+    d.tb = M.rtb[r];
+    d.pk = M.rpack[r];
+    d.A = M.rA[r];
+    d.b = M.rbeta[r];
+    d.Ta = M.rTa[r];
+    d.A0 = M.rA0[r];
+    d.b0 = M.rbeta0[r];
+    d.Ta0 = M.rTa0[r];
+    d.alpha = M.rtroe[r];
+    d.Tsss = M.rtroe[M.nr + r];
+    d.Ts = M.rtroe[2 * M.nr + r];
+    d.Tss = M.rtroe[3 * M.nr + r];
+    d.Mc = L.Mtb[d.tb >= 0 ? d.tb : 0];
+    return d;
+}
+
 // Rate constant of reaction r (kf includes M for third-body and falloff
 // forms).  JAC selects the Jacobian-path variants, which also return
 // dkf/dT and dkf/dM (src/Reaction/ReactionRate.C:84-672,
 // src/Reaction/Jacobian.C:108-910).
 template <bool JAC>
-__device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, int r, int t, double T, double lnT,
-                                     double iT, double lnp)
+__device__ inline RxnK rate_constant(const DevMech& M, const RxnData& D, int r, double T, double lnT, double iT,
+                                     double lnp)
 {
+    const int t = D.t;
     RxnK k{0.0, 0.0, 0.0};
     if (t == RXN_PLOG) {
         const int row = M.raux[r];
@@ -234,28 +318,15 @@ __device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, int r, int 
         if (JAC) k.dkfdT = k.kf * dlk;
         return k;
     }
-    const double kk = M.rA[r] * exp(M.rbeta[r] * lnT - M.rTa[r] * iT);
-    const double dk = JAC ? kk * (M.rbeta[r] + M.rTa[r] * iT) * iT : 0.0;
+    const double kk = D.A * exp(D.b * lnT - D.Ta * iT);
+    const double dk = JAC ? kk * (D.b + D.Ta * iT) * iT : 0.0;
     if (t == RXN_ARR) {
         k.kf = kk;
         k.dkfdT = dk;
         return k;
     }
-    // third-body concentration M = sum_j eff_rj C_j
-    const int tb = M.rtb[r];
-    double Mc = 0.0;
-    for (int j0 = 0; j0 < M.ns; j0 += 8) {   // batches of 8 independent loads
-        double e[8], c[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (j0 + u < M.ns) {
-                e[u] = M.effT[(size_t)(j0 + u) * M.nTB + tb];
-                c[u] = L.C[j0 + u];
-            }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (j0 + u < M.ns) Mc += e[u] * c[u];
-    }
+    // third-body concentration M = sum_j eff_rj C_j (third_body_sums)
+    const double Mc = D.Mc;
     if (t == RXN_TB) {
         k.kf = kk * Mc;
         k.dkfdT = dk * Mc;
@@ -263,8 +334,8 @@ __device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, int r, int 
         return k;
     }
     const double kinf = kk, dkinf = dk;
-    const double k0 = M.rA0[r] * exp(M.rbeta0[r] * lnT - M.rTa0[r] * iT);
-    const double dk0 = JAC ? k0 * (M.rbeta0[r] + M.rTa0[r] * iT) * iT : 0.0;
+    const double k0 = D.A0 * exp(D.b0 * lnT - D.Ta0 * iT);
+    const double dk0 = JAC ? k0 * (D.b0 + D.Ta0 * iT) * iT : 0.0;
     if (t == RXN_LIND) {
         const double Pr = k0 * Mc / kinf;
         const double tmp = 1.0 / (1.0 + Pr);
@@ -277,8 +348,7 @@ __device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, int r, int 
         return k;
     }
     // Troe
-    const double* tr = M.troe + 4 * (size_t)M.raux[r];
-    const double alpha = tr[0], Tsss = tr[1], Ts = tr[2], Tss = tr[3];
+    const double alpha = D.alpha, Tsss = D.Tsss, Ts = D.Ts, Tss = D.Tss;
     const double e1 = exp(-T / Tsss), e2 = exp(-Tss * iT), e3 = exp(-T / Ts);
     const double Fcent = (1.0 - alpha) * e1 + alpha * e3 + e2;
     const double logFc = log10(Fcent > TROE_SMALL ? Fcent : TROE_SMALL);
@@ -327,61 +397,98 @@ __device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, int r, int 
     return k;
 }
 
-// Kc in concentration units (src/Reaction/ReactionFunctionI.H:1-52,
-// ChemicalEquilibriumFunctionI.H:14-25) and, with JAC, d ln Kc / dT
-// (src/Reaction/JacobianFunctionI.H:1150-1158).
-template <bool JAC>
-__device__ inline double equilibrium(const Lds& L, const int* s, int nl, int nrh, double PbyRT, double iT,
-                                     double* dlnKc)
+// Third-body concentrations M_t = sum_j eff_tj C_j of every third-body and
+// falloff row (src/Reaction/ReactionRate.C:178-399 forms the same sums per
+// row), one row per lane, two rows per lane pass; L.C must be complete.
+__device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 {
-    double num = 1.0, den = 1.0, sdb = 0.0;
-    for (int a = 0; a < nrh; ++a) { num *= L.expG[s[3 + a]]; if (JAC) sdb += L.dBdT[s[3 + a]]; }
-    for (int a = 0; a < nl; ++a) { den *= L.expG[s[a]]; if (JAC) sdb -= L.dBdT[s[a]]; }
-    const int dn = nrh - nl;
-    double pw = 1.0;
-    if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
-    if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
-    if (JAC) *dlnKc = sdb - dn * iT;
-    return (num / den) * pw;
+    const int lane = lane_id();
+    for (int t0 = 0; t0 < M.nTB; t0 += 2 * WAVE) {
+        const int ta = t0 + lane, tb = t0 + WAVE + lane;
+        const bool ina = ta < M.nTB, inb = tb < M.nTB;
+        double ma = 0.0, mb = 0.0;
+        for (int j0 = 0; j0 < M.ns; j0 += 16) {   // batches of 16 species
+            // unconditional loads (the table is zero-padded to 64 species and
+            // nTBpad rows, C is zero beyond ns), so all 48 are in flight at once
+            double ea[16], eb[16], c[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                c[u] = L.C[j0 + u];
+                ea[u] = M.effT[(size_t)(j0 + u) * M.nTBpad + ta];
+                eb[u] = M.effT[(size_t)(j0 + u) * M.nTBpad + tb];
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (j0 + u < M.ns) {
+                    ma += ea[u] * c[u];
+                    mb += eb[u] * c[u];
+                }
+        }
+        if (ina) L.Mtb[ta] = ma;
+        if (inb) L.Mtb[tb] = mb;
+    }
 }
 
 // Reaction round: lane l handles reactions l, l+64, ...
 //  !JAC: rq[r] = q_r
 //   JAC: rq[r] = kf, rk[r] = kr, rdT[r] = dq/dT, rw[tb] = dkf/dM CF - dkr/dM CR
+// Kc in concentration units follows src/Reaction/ReactionFunctionI.H:1-52
+// and ChemicalEquilibriumFunctionI.H:14-25, d ln Kc / dT
+// src/Reaction/JacobianFunctionI.H:1150-1158.  The species of all six slots
+// are gathered unconditionally (unused slots name species 0) and the
+// products take the used slots in slot order.
 template <bool JAC>
 __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, double lnT, double iT,
                                       double lnp)
 {
     const double PbyRT = PSTD / (RU * T);
     for (int r = lane_id(); r < M.nr; r += WAVE) {
-        const int t = M.rtype[r];
-        const unsigned long long pk = M.rpack[r];
+        const RxnData D = load_rxn(M, L, r);
+        const unsigned long long pk = D.pk;
         const int nl = pk_nl(pk), nrh = pk_nr(pk);
-        int s[6];
+        double c[6], g[6], db[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) s[q] = pk_species(pk, q);
-        const RxnK k = rate_constant<JAC>(M, L, r, t, T, lnT, iT, lnp);
+        for (int q = 0; q < 6; ++q) {
+            const int sq = pk_species(pk, q);
+            c[q] = L.C[sq];
+            g[q] = L.expG[sq];
+            db[q] = JAC ? L.dBdT[sq] : 0.0;
+        }
+        const RxnK k = rate_constant<JAC>(M, D, r, T, lnT, iT, lnp);
         double kr = 0.0, dkrdT = 0.0, invKc = 0.0;
-        if (M.rrev[r]) {
-            double dlnKc = 0.0;
-            double Kc = equilibrium<JAC>(L, s, nl, nrh, PbyRT, iT, &dlnKc);
+        if (D.rev) {
+            double num = 1.0, den = 1.0, sdb = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (a < nrh) { num *= g[3 + a]; if (JAC) sdb += db[3 + a]; }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (a < nl) { den *= g[a]; if (JAC) sdb -= db[a]; }
+            const int dn = nrh - nl;
+            double pw = 1.0;
+            if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
+            if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
+            double Kc = (num / den) * pw;
             const bool floored = !(Kc > KC_FLOOR);
             Kc = floored ? KC_FLOOR : Kc;
             invKc = 1.0 / Kc;
             kr = k.kf * invKc;
-            if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * dlnKc);
+            if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * (sdb - dn * iT));
         }
         double CF = 1.0, CR = 1.0;
-        for (int a = 0; a < nl; ++a) CF *= L.C[s[a]];
-        for (int a = 0; a < nrh; ++a) CR *= L.C[s[3 + a]];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a < nl) CF *= c[a];
+            if (a < nrh) CR *= c[3 + a];
+        }
         if (!JAC) {
             L.rq[r] = k.kf * CF - kr * CR;
         } else {
             L.rq[r] = k.kf;
             L.rk[r] = kr;
             L.rdT[r] = k.dkfdT * CF - dkrdT * CR;
-            if (t == RXN_TB || t == RXN_LIND || t == RXN_TROE)
-                L.rw[M.rtb[r]] = k.dkfdM * CF - (k.dkfdM * invKc) * CR;
+            if (D.t == RXN_TB || D.t == RXN_LIND || D.t == RXN_TROE)
+                L.rw[D.tb] = k.dkfdM * CF - (k.dkfdM * invKc) * CR;
         }
     }
 }
@@ -395,39 +502,40 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
 {
     const int lane = lane_id();
     const bool sp = lane < M.ns;
+    const SpeciesData S = load_species(M, lane);      // tables are padded to 64 lanes
     if (sp) y = y > 0.0 ? y : 0.0;
     const double T = readlane_d(y, NP - 1);
     const double RuTByP = RU * T / p;                 // unclamped T for the density
-    const double invW = sp ? M.invW[lane] : 0.0;
+    const double invW = sp ? S.invW : 0.0;
     const double invrho = wave_sum(sp ? y * invW * RuTByP : 0.0);
     const double rho = 1.0 / invrho;
     const double Tc = clampT(M, T), lnT = log(Tc), iT = 1.0 / Tc;
     Thermo th{0, 0, 0, 0, 0};
-    if (sp) th = species_thermo(M, lane, Tc, lnT, iT);
+    if (sp) th = species_thermo(M, S, Tc, lnT, iT);
     L.C[lane] = sp ? rho * invW * y : 0.0;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
+    wave_sync();
+    third_body_sums(M, L);
     wave_sync();
     reaction_round<false>(M, L, Tc, lnT, iT, lnp);
     wave_sync();
     double om = 0.0;
     if (sp) {
-        for (int e0 = 0; e0 < M.maxDeg; e0 += 8) {   // batches of 8 list entries
-            double nu[8], q[8];
-            int r[8];
+        for (int e0 = 0; e0 < M.maxDeg; e0 += 16) {   // unconditional batches of 16 (maxDeg % 16 == 0)
+            int w[16];
+            double q[16];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const bool in = e0 + u < M.maxDeg;
-                nu[u] = in ? M.sp_nu[(e0 + u) * WAVE + lane] : 0.0;
-                r[u] = in ? M.sp_r[(e0 + u) * WAVE + lane] : 0;
+            for (int u = 0; u < 16; ++u) w[u] = M.sp_rn[(e0 + u) * WAVE + lane];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) q[u] = L.rq[rn_r(w[u])];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int nu = rn_nu(w[u]);
+                if (nu != 0) om += (double)nu * q[u];
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) q[u] = nu[u] != 0.0 ? L.rq[r[u]] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (nu[u] != 0.0) om += nu[u] * q[u];
         }
     }
-    const double Yd = sp ? om * (M.W[lane] * invrho) : 0.0;
+    const double Yd = sp ? om * (S.W * invrho) : 0.0;
     const double cpM = wave_sum(sp ? y * th.cp : 0.0);
     const double hY = wave_sum(sp ? th.ha * Yd : 0.0);
     wave_sync();
@@ -451,12 +559,13 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     const int lane = lane_id();
     const int ns = M.ns;
     const bool sp = lane < ns;
+    const SpeciesData S = load_species(M, lane);      // tables are padded to 64 lanes
     if (sp) y = y > 0.0 ? y : 0.0;
     const double T = clampT(M, readlane_d(y, NP - 1));
     const double lnT = log(T), iT = 1.0 / T;
-    const double invW = sp ? M.invW[lane] : 0.0;
+    const double invW = sp ? S.invW : 0.0;
     Thermo th{0, 0, 0, 0, 0};
-    if (sp) th = species_thermo(M, lane, T, lnT, iT);
+    if (sp) th = species_thermo(M, S, T, lnT, iT);
     const double RuTByP = RU * T / p;
     const double invrho = wave_sum(sp ? RuTByP * (y * invW) : 0.0);
     const double MW = 1.0 / wave_sum(sp ? y * invW : 0.0);
@@ -467,10 +576,43 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     L.C[lane] = Cl;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     L.dBdT[lane] = th.dBdT;
-    double* Jrow = L.J + lane;   // element j of this lane's row at Jrow[j*64]
-    for (int j = 0; j < NP; ++j) Jrow[j * WAVE] = 0.0;
+    double* Jrow = L.J + lane;   // element j of this lane's row at Jrow[j*JS]
+    for (int j = 0; j < NP; ++j) Jrow[j * JS] = 0.0;
+    wave_sync();
+    third_body_sums(M, L);
     wave_sync();
     reaction_round<true>(M, L, T, lnT, iT, lnp);
+    wave_sync();
+    // third-body / falloff partials: element (i, j) = sum_r nu_ir w_r eff_rj
+    // over the third-body/falloff entries of species i, before the sparse
+    // partials are added.  Column-wise over a flat entry list sorted by row
+    // (uniform loop, 16 entries per batch, coalesced efficiency rows): lane j
+    // forms row i's sum and stores it once.
+    if (M.nTBz > 0) {
+        double acc = 0.0;
+        int cur = M.tbz_row[0];
+        for (int k0 = 0; k0 < M.nTBz; k0 += 16) {
+            int rw_[16];
+            double cf[16], ef[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {   // unconditional: nTBz % 16 == 0
+                const int t = M.tbz_tb[k0 + u];
+                rw_[u] = M.tbz_row[k0 + u];
+                cf[u] = M.tbz_nu[k0 + u] * L.rw[t];
+                ef[u] = M.eff[t * WAVE + lane];
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                if (rw_[u] != cur) {
+                    if (sp) L.J[lane * JS + cur] = acc;
+                    acc = 0.0;
+                    cur = rw_[u];
+                }
+                acc += cf[u] * ef[u];
+            }
+        }
+        if (sp) L.J[lane * JS + cur] = acc;
+    }
     wave_sync();
     // species rows: sparse partials dq/dC into this lane's row.  The list is
     // walked in blocks of 4 entries: all loads of a block are issued before
@@ -483,10 +625,10 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             int r[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const bool in = e0 + u < M.maxDeg;
-                nu[u] = in ? M.sp_nu[(e0 + u) * WAVE + lane] : 0.0;
-                r[u] = in ? M.sp_r[(e0 + u) * WAVE + lane] : 0;
-                pk[u] = in ? M.sp_pack[(e0 + u) * WAVE + lane] : 0ull;
+                const int w = M.sp_rn[(e0 + u) * WAVE + lane];
+                nu[u] = (double)rn_nu(w);
+                r[u] = rn_r(w);
+                pk[u] = M.sp_pack[(e0 + u) * WAVE + lane];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -515,7 +657,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
                     double d = kf[u];
 #pragma unroll
                     for (int b = 0; b < 3; ++b) if (b < nl && b != q) d *= c[u][b];
-                    Jrow[pk_species(pk[u], q) * WAVE] += nu[u] * d;
+                    Jrow[pk_species(pk[u], q) * JS] += nu[u] * d;
                 }
                 if (kr[u] != 0.0) {
 #pragma unroll
@@ -524,50 +666,14 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
                         double d = kr[u];
 #pragma unroll
                         for (int b = 0; b < 3; ++b) if (b < nrh && b != q) d *= c[u][3 + b];
-                        Jrow[pk_species(pk[u], 3 + q) * WAVE] -= nu[u] * d;
+                        Jrow[pk_species(pk[u], 3 + q) * JS] -= nu[u] * d;
                     }
                 }
             }
         }
-        // third-body / falloff partials: nu * w_r * eff_rj on every column
-        // (species-major efficiencies: the lanes' loads of one column share a
-        // row).  Blocks of 4 entries x 4 columns; each element still takes
-        // the entries' contributions in list order.
-        for (int e0 = 0; e0 < M.maxDegTB; e0 += 4) {
-            double coef[4];
-            int t[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const bool in = e0 + u < M.maxDegTB;
-                const double nu = in ? M.sptb_nu[(e0 + u) * WAVE + lane] : 0.0;
-                t[u] = in ? M.sptb_t[(e0 + u) * WAVE + lane] : 0;
-                coef[u] = nu != 0.0 ? nu * L.rw[t[u]] : 0.0;
-            }
-            const bool any = coef[0] != 0.0 || coef[1] != 0.0 || coef[2] != 0.0 || coef[3] != 0.0;
-            if (!any) continue;
-            for (int j0 = 0; j0 < ns; j0 += 4) {
-                double v[4], ef[4][4];
-#pragma unroll
-                for (int w = 0; w < 4; ++w)
-                    if (j0 + w < ns) {
-                        v[w] = Jrow[(j0 + w) * WAVE];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) ef[u][w] = M.effT[(size_t)(j0 + w) * M.nTB + t[u]];
-                    }
-#pragma unroll
-                for (int w = 0; w < 4; ++w)
-                    if (j0 + w < ns) {
-                        double acc = v[w];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            if (coef[u] != 0.0) acc += coef[u] * ef[u][w];
-                        Jrow[(j0 + w) * WAVE] = acc;
-                    }
-            }
-        }
     }
     // dC/dY conversion of the species rows, T column
-    const double WiByRho = sp ? M.W[lane] * invrho : 0.0;
+    const double WiByRho = sp ? S.W * invrho : 0.0;
     const double Yd = sp ? om * WiByRho : 0.0;
     if (sp) {
         double sC = 0.0;
@@ -576,7 +682,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (j0 + u < ns) {
-                    v[u] = Jrow[(j0 + u) * WAVE];
+                    v[u] = Jrow[(j0 + u) * JS];
                     c[u] = L.C[j0 + u];
                 }
 #pragma unroll
@@ -589,19 +695,19 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (j0 + u < ns) {
-                    v[u] = Jrow[(j0 + u) * WAVE];
+                    v[u] = Jrow[(j0 + u) * JS];
                     iw[u] = M.invW[j0 + u];
                 }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (j0 + u < ns) {
                     const double iWj = iw[u];
-                    Jrow[(j0 + u) * WAVE] = jacType == 0
+                    Jrow[(j0 + u) * JS] = jacType == 0
                         ? WiByRho * (v[u] * (rho * iWj) - (MW * iWj) * sCx) + (MW * iWj) * Yd
                         : WiByRho * (v[u] * (rho * iWj)) + (MW * iWj) * Yd;
                 }
         }
-        Jrow[(NP - 1) * WAVE] = WiByRho * (jT - sC * iT) + iT * Yd;
+        Jrow[(NP - 1) * JS] = WiByRho * (jT - sC * iT) + iT * Yd;
     }
     // temperature row (src/Reaction/Jacobian.C:3743-3791): lane j < NP forms
     // column j's sum over the species rows and writes row NP-1, column j
@@ -613,15 +719,13 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     if (lane < NP) {
         const bool tcol = lane == NP - 1;
         double colsum = 0.0;
-        const int i0 = lane % ns;   // rotated start: the lanes hit different banks
         for (int q0 = 0; q0 < ns; q0 += 8) {
             double v[8], h[8], qq[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (q0 + u < ns) {
-                    int i = i0 + q0 + u;
-                    i = i >= ns ? i - ns : i;
-                    v[u] = L.J[lane * WAVE + i];
+                    const int i = q0 + u;
+                    v[u] = L.J[lane * JS + i];
                     h[u] = L.H[i];
                     qq[u] = tcol ? L.Q[i] : 0.0;
                 }
@@ -631,7 +735,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
         }
         const double rowT = !tcol ? (sp ? (-colsum - th.cp * dTdt) / cpM : 0.0)
                                   : (-colsum - dTdt * dcpMdT) / cpM;
-        L.J[lane * WAVE + (NP - 1)] = rowT;
+        L.J[lane * JS + (NP - 1)] = rowT;
     }
     wave_sync();
     return sp ? Yd : (lane == NP - 1 ? dTdt : 0.0);
@@ -661,66 +765,113 @@ __device__ inline LuState lu_factor(const Lds& L)
     double* Jrow = L.J + lane;
     int pos = lane;          // lanes >= NP hold no row
     double invD = 0.0;
-    for (int k = 0; k < NP; ++k) {
-        const int k0 = k & ~3, d = k & 3;
-        double ak = Jrow[k * WAVE];
-        // --- pivot choice (uniform) ---
-        int pick = k;
-        if (d == 0) {
-            double temp = 0.0;
-            int im = 0;
+    for (int k0 = 0; k0 < NP; k0 += 4) {
+        // ---- panel: columns k0..k0+3 of every row, in registers.  Each
+        // column takes the pivot choice, swap, eps replacement and
+        // elimination of the reference, restricted to the panel columns.
+        double pa[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double v = cand(ak, pos, k0 + q);
-                if (temp < v) { im = q; temp = v; }
+        for (int d = 0; d < 4; ++d) pa[d] = Jrow[(k0 + d) * JS];
+        int pl[4];           // lane holding logical row k0+d
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int k = k0 + d;
+            const double ak = pa[d];
+            int pick = k;
+            if (d == 0) {
+                double temp = 0.0;
+                int im = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double v = cand(ak, pos, k0 + q);
+                    if (temp < v) { im = q; temp = v; }
+                }
+                pick = k0 + im;
+            } else if (d == 1) {
+                double t = cand(ak, pos, k0 + 1);
+                int im = 1;
+                const double v2 = cand(ak, pos, k0 + 2);
+                if (t < v2) {
+                    im = 2;
+                    t = readlane_d(ak, first_lane(pos == k0 + 2));   // signed, LUsolver.C:177
+                }
+                const double v3 = cand(ak, pos, k0 + 3);
+                if (t < v3) im = 3;
+                pick = k0 + im;
+            } else if (d == 2) {
+                const double v2 = cand(ak, pos, k0 + 2);
+                const double v3 = cand(ak, pos, k0 + 3);
+                if (v2 < v3) pick = k0 + 3;
             }
-            pick = k0 + im;
-        } else if (d == 1) {
-            double t = cand(ak, pos, k0 + 1);
-            int im = 1;
-            const double v2 = cand(ak, pos, k0 + 2);
-            if (t < v2) {
-                im = 2;
-                t = readlane_d(ak, first_lane(pos == k0 + 2));   // signed, LUsolver.C:177
+            if (pick != k) {
+                if (pos == k) pos = pick;
+                else if (pos == pick) pos = k;
             }
-            const double v3 = cand(ak, pos, k0 + 3);
-            if (t < v3) im = 3;
-            pick = k0 + im;
-        } else if (d == 2) {
-            const double v2 = cand(ak, pos, k0 + 2);
-            const double v3 = cand(ak, pos, k0 + 3);
-            if (v2 < v3) pick = k0 + 3;
+            const int p = first_lane(pos == k);
+            double piv = readlane_d(ak, p);
+            if (piv == 0.0) {                        // LUsolver.C:155
+                piv = PIVOT_EPS;
+                if (lane == p) pa[d] = PIVOT_EPS;
+            }
+            const double inv = 1.0 / piv;
+            if (lane == p) invD = inv;
+            const bool elim = row && pos > k;
+            const double l = ak * inv;
+            if (elim) pa[d] = l;
+#pragma unroll
+            for (int e = d + 1; e < 4; ++e) {
+                const double pe = readlane_d(pa[e], p);
+                if (elim) pa[e] = pa[e] - l * pe;
+            }
+            pl[d] = p;
         }
-        if (pick != k) {
-            if (pos == k) pos = pick;
-            else if (pos == pick) pos = k;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) Jrow[(k0 + d) * JS] = pa[d];
+        // ---- U rows of the block, column-wise: lane j forms the pivot rows'
+        // entries of trailing column j with the operations (and order) the
+        // column-by-column elimination applies to them, and stores them.
+        const double l10 = readlane_d(pa[0], pl[1]);
+        const double l20 = readlane_d(pa[0], pl[2]), l21 = readlane_d(pa[1], pl[2]);
+        const double l30 = readlane_d(pa[0], pl[3]), l31 = readlane_d(pa[1], pl[3]),
+                     l32 = readlane_d(pa[2], pl[3]);
+        if (lane >= k0 + 4 && lane < NP) {
+            const double* cj = L.J + lane * JS;
+            const double u0 = cj[pl[0]], a1 = cj[pl[1]], a2 = cj[pl[2]], a3 = cj[pl[3]];
+            const double u1 = a1 - l10 * u0;
+            double u2 = a2 - l20 * u0;
+            u2 = u2 - l21 * u1;
+            double u3 = a3 - l30 * u0;
+            u3 = u3 - l31 * u1;
+            u3 = u3 - l32 * u2;
+            L.J[lane * JS + pl[1]] = u1;
+            L.J[lane * JS + pl[2]] = u2;
+            L.J[lane * JS + pl[3]] = u3;
         }
-        const int pl = first_lane(pos == k);
-        double piv = readlane_d(ak, pl);
-        if (piv == 0.0) {                        // LUsolver.C:155
-            piv = PIVOT_EPS;
-            if (lane == pl) Jrow[k * WAVE] = PIVOT_EPS;
-        }
-        const double inv = 1.0 / piv;
-        if (lane == pl) invD = inv;
-        const double* prow = L.J + pl;           // pivot row, broadcast reads
-        // Every lane runs the update (uniform control flow); only rows below
-        // the pivot (pos > k) take it, and only in columns j > k.  The column
-        // range starts at a multiple of 4 (NP is one), so every block of 4
-        // is whole: its 8 loads are issued before its 4 stores.
-        const bool elim = row && pos > k;
-        const double l = ak * inv;
-        if (elim) Jrow[k * WAVE] = l;
-        for (int j0 = (k + 1) & ~3; j0 < NP; j0 += 4) {
-            double v[4], pv[4];
+        wave_sync();
+        // ---- trailing update of the rows below the block: a_ij -= sum_q
+        // l_iq u_qj in the elimination's order.  The other rows take zero
+        // multipliers, which leaves them unchanged exactly.
+        const bool below = row && pos > k0 + 3;
+        const double m0 = below ? pa[0] : 0.0, m1 = below ? pa[1] : 0.0, m2 = below ? pa[2] : 0.0,
+                     m3 = below ? pa[3] : 0.0;
+        for (int j0 = k0 + 4; j0 < NP; j0 += 4) {
+            double v[4], u0[4], u1[4], u2[4], u3[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                v[u] = Jrow[(j0 + u) * WAVE];
-                pv[u] = prow[(j0 + u) * WAVE];
+                v[u] = Jrow[(j0 + u) * JS];
+                u0[u] = L.J[(j0 + u) * JS + pl[0]];
+                u1[u] = L.J[(j0 + u) * JS + pl[1]];
+                u2[u] = L.J[(j0 + u) * JS + pl[2]];
+                u3[u] = L.J[(j0 + u) * JS + pl[3]];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                Jrow[(j0 + u) * WAVE] = (elim && j0 + u > k) ? v[u] - l * pv[u] : v[u];
+            for (int u = 0; u < 4; ++u) {
+                double w = v[u] - m0 * u0[u];
+                w = w - m1 * u1[u];
+                w = w - m2 * u2[u];
+                w = w - m3 * u3[u];
+                Jrow[(j0 + u) * JS] = w;
+            }
         }
         wave_sync();
     }
@@ -743,7 +894,7 @@ __device__ inline double lu_solve(const Lds& L, const LuState& st, double b)
     for (int r = 0; r < NP - 1; ++r) {
         const int src = readlane_i(st.ipl, r);
         const double yr = readlane_d(b, src);
-        const double lr = Jrow[r * WAVE];            // unconditional: prefetchable
+        const double lr = Jrow[r * JS];              // unconditional: prefetchable
         const double nb = b - lr * yr;
         b = (st.pos > r && lane < NP) ? nb : b;
     }
@@ -753,7 +904,7 @@ __device__ inline double lu_solve(const Lds& L, const LuState& st, double b)
         const int src = readlane_i(st.ipl, r);
         if (lane == src) b *= st.invD;
         const double xr = readlane_d(b, src);
-        const double ur = Jrow[r * WAVE];
+        const double ur = Jrow[r * JS];
         const double nb = b - ur * xr;
         b = st.pos < r ? nb : b;
         if (lane == r) x = xr;

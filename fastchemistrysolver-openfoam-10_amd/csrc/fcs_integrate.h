@@ -60,7 +60,7 @@ __device__ inline void form_matrix(const Lds& L, double diag)
 {
     const int lane = lane_id();
     double* Jrow = L.J + lane;
-    for (int j = 0; j < NP; ++j) Jrow[j * WAVE] = (lane == j ? diag : 0.0) - Jrow[j * WAVE];
+    for (int j = 0; j < NP; ++j) Jrow[j * JS] = (lane == j ? diag : 0.0) - Jrow[j * JS];
     wave_sync();
 }
 
@@ -265,7 +265,7 @@ template <int NP>
 __device__ __forceinline__ void jy_store(const Lds& L, double* JyG)
 {
     const int lane = lane_id();
-    for (int j = 0; j < NP; ++j) JyG[j * WAVE + lane] = L.J[j * WAVE + lane];
+    for (int j = 0; j < NP; ++j) JyG[j * WAVE + lane] = L.J[j * JS + lane];
 }
 
 // Iteration matrix diag*I - Jy in the LDS matrix, Jy read from HBM.
@@ -279,7 +279,7 @@ __device__ __forceinline__ void form_matrix_from(const Lds& L, const double* JyG
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = JyG[(j0 + u) * WAVE + lane];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) Jrow[(j0 + u) * WAVE] = (lane == j0 + u ? diag : 0.0) - v[u];
+        for (int u = 0; u < 4; ++u) Jrow[(j0 + u) * JS] = (lane == j0 + u ? diag : 0.0) - v[u];
     }
     wave_sync();
 }

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(64) void k_jacobian(DevMech M, int jacType, int nS,
         double* row = J + ((size_t)s * n + ext) * n;
         for (int j = 0; j < NP; ++j) {
             const int ej = j < ns ? j : (j == NP - 1 ? ns : -1);
-            if (ej >= 0) row[ej] = L.J[j * WAVE + lane];
+            if (ej >= 0) row[ej] = L.J[j * JS + lane];
         }
     }
 }
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(64) void k_lu_solve(DevMech M, int nSys, const doub
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     for (int j = 0; j < NP; ++j) {
         const int ej = j < ns ? j : (j == NP - 1 ? ns : -1);
-        L.J[j * WAVE + lane] =
+        L.J[j * JS + lane] =
             (ext >= 0 && ej >= 0) ? A[((size_t)s * n + ext) * n + ej] : ((lane == j && lane < NP) ? 1.0 : 0.0);
     }
     wave_sync();
@@ -509,7 +509,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     }
     std::vector<int> rtype(nr), rrev(nr), rnl(nr), rnr(nr), rsp(6 * std::max(nr, 1), 0), rtb(nr, -1), raux(nr, -1);
     std::vector<double> rA(nr), rbeta(nr), rTa(nr), rA0(nr), rbeta0(nr), rTa0(nr);
-    std::vector<double> troe, eff, plogLnp, plogLnA, plogBeta, plogTa;
+    std::vector<double> troe, eff, plogLnp, plogLnA, plogBeta, plogTa, rtroe(4 * (size_t)std::max(nr, 1), 0.0);
     std::vector<int> plogN;
     int nTB = 0;
     for (int r = 0; r < nr; ++r) {
@@ -538,6 +538,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             if (!d->troe) return FCS_E_ARG;
             raux[r] = (int)troe.size() / 4;
             for (int q = 0; q < 4; ++q) troe.push_back(d->troe[4 * (size_t)f + q]);
+            for (int q = 0; q < 4; ++q) rtroe[(size_t)q * nr + r] = d->troe[4 * (size_t)f + q];
         }
         if (t == FCS_RXN_PLOG) {
             const int np = d->plog_n[f];
@@ -554,9 +555,10 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             }
         }
     }
-    std::vector<double> effT((size_t)64 * std::max(nTB, 1), 0.0);
+    const int nTBpad = std::max(128, (nTB + 127) / 128 * 128);
+    std::vector<double> effT((size_t)64 * nTBpad, 0.0);
     for (int t = 0; t < nTB; ++t)
-        for (int s = 0; s < 64; ++s) effT[(size_t)s * nTB + t] = eff[(size_t)t * 64 + s];
+        for (int s = 0; s < 64; ++s) effT[(size_t)s * nTBpad + t] = eff[(size_t)t * 64 + s];
     // ---- per-species lists (net stoichiometry), lane-interleaved
     std::vector<std::vector<std::pair<int, double>>> lst(ns), lstTB(ns);
     for (int r = 0; r < nr; ++r) {
@@ -569,11 +571,9 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             if (rtb[r] >= 0) lstTB[s].push_back({rtb[r], nu[s]});
         }
     }
-    int maxDeg = 1, maxDegTB = 1;
-    for (int s = 0; s < ns; ++s) {
-        maxDeg = std::max(maxDeg, (int)lst[s].size());
-        maxDegTB = std::max(maxDegTB, (int)lstTB[s].size());
-    }
+    int maxDeg = 1;
+    for (int s = 0; s < ns; ++s) maxDeg = std::max(maxDeg, (int)lst[s].size());
+    maxDeg = (maxDeg + 15) / 16 * 16;   // the device walks the lists in unconditional blocks of 16
     // packed reaction species: slots 0-2 left, 3-5 right (bytes 0-5), nl, nr (bytes 6, 7)
     std::vector<unsigned long long> rpack(std::max(nr, 1), 0ull);
     for (int r = 0; r < nr; ++r) {
@@ -584,17 +584,20 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         rpack[r] = w;
     }
     std::vector<unsigned long long> spPack((size_t)maxDeg * 64, 0ull);
-    std::vector<int> spR((size_t)maxDeg * 64, 0), spTB((size_t)maxDegTB * 64, 0);
-    std::vector<double> spNu((size_t)maxDeg * 64, 0.0), spTBnu((size_t)maxDegTB * 64, 0.0);
+    std::vector<int> spRN((size_t)maxDeg * 64, 8 << 16);   // padding: reaction 0, nu 0
+    std::vector<int> tbzRow, tbzTb;
+    std::vector<double> tbzNu;
     for (int s = 0; s < ns; ++s) {
         for (size_t e = 0; e < lst[s].size(); ++e) {
-            spR[e * 64 + s] = lst[s][e].first;
-            spNu[e * 64 + s] = lst[s][e].second;
+            const int nuI = (int)lst[s][e].second;
+            if (nuI != lst[s][e].second || nuI < -7 || nuI > 7 || lst[s][e].first > 0xffff) return FCS_E_ARG;
+            spRN[e * 64 + s] = lst[s][e].first | ((nuI + 8) << 16);
             spPack[e * 64 + s] = rpack[lst[s][e].first];
         }
         for (size_t e = 0; e < lstTB[s].size(); ++e) {
-            spTB[e * 64 + s] = lstTB[s][e].first;
-            spTBnu[e * 64 + s] = lstTB[s][e].second;
+            tbzRow.push_back(s);
+            tbzTb.push_back(lstTB[s][e].first);
+            tbzNu.push_back(lstTB[s][e].second);
         }
     }
     // ---- one device allocation
@@ -612,9 +615,17 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     const size_t iType = vi(rtype), iRev = vi(rrev), iNl = vi(rnl), iNr = vi(rnr), iSp = vi(rsp), iTb = vi(rtb),
                  iAux = vi(raux);
     const size_t iA = vd(rA), iB = vd(rbeta), iTa = vd(rTa), iA0 = vd(rA0), iB0 = vd(rbeta0), iTa0 = vd(rTa0);
-    const size_t iTroe = vd(troe), iEff = vd(eff), iEffT = vd(effT);
+    const size_t iTroe = vd(troe), iEff = vd(eff), iEffT = vd(effT), iRtroe = vd(rtroe);
     const size_t iPn = vi(plogN), iPp = vd(plogLnp), iPA = vd(plogLnA), iPb = vd(plogBeta), iPT = vd(plogTa);
-    const size_t iSpR = vi(spR), iSpNu = vd(spNu), iSpTB = vi(spTB), iSpTBnu = vd(spTBnu);
+    // flat third-body entry list padded to a multiple of 16 with zero entries on the last row
+    const int nTBz = (int)tbzRow.size();
+    while (tbzRow.size() % 16) {
+        tbzRow.push_back(nTBz ? tbzRow.back() : 0);
+        tbzTb.push_back(0);
+        tbzNu.push_back(0.0);
+    }
+    const size_t iSpRN = vi(spRN);
+    const size_t iTbzRow = vi(tbzRow), iTbzTb = vi(tbzTb), iTbzNu = vd(tbzNu);
     auto vq = [&](const std::vector<unsigned long long>& v) { return add(v.data(), v.size() * 8); };
     const size_t iRpack = vq(rpack), iSpPack = vq(spPack);
     std::vector<unsigned char> host(total, 0);
@@ -639,7 +650,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.nTB = nTB;
     dm.pmax = pmax;
     dm.maxDeg = maxDeg;
-    dm.maxDegTB = maxDegTB;
+    dm.nTBpad = nTBpad;
     dm.TlowMin = TlowMin;
     dm.ThighMax = ThighMax;
     dm.TcMin = TcMin;
@@ -663,6 +674,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.rbeta0 = (const double*)P(iB0);
     dm.rTa0 = (const double*)P(iTa0);
     dm.troe = (const double*)P(iTroe);
+    dm.rtroe = (const double*)P(iRtroe);
     dm.eff = (const double*)P(iEff);
     dm.effT = (const double*)P(iEffT);
     dm.plog_n = (const int*)P(iPn);
@@ -670,10 +682,11 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.plog_lnA = (const double*)P(iPA);
     dm.plog_beta = (const double*)P(iPb);
     dm.plog_Ta = (const double*)P(iPT);
-    dm.sp_r = (const int*)P(iSpR);
-    dm.sp_nu = (const double*)P(iSpNu);
-    dm.sptb_t = (const int*)P(iSpTB);
-    dm.sptb_nu = (const double*)P(iSpTBnu);
+    dm.sp_rn = (const int*)P(iSpRN);
+    dm.nTBz = nTBz > 0 ? (int)tbzRow.size() : 0;
+    dm.tbz_row = (const int*)P(iTbzRow);
+    dm.tbz_tb = (const int*)P(iTbzTb);
+    dm.tbz_nu = (const double*)P(iTbzNu);
     dm.rpack = (const unsigned long long*)P(iRpack);
     dm.sp_pack = (const unsigned long long*)P(iSpPack);
     m->ns = ns;
