@@ -86,8 +86,22 @@ struct DevMech {
     const double* tbz_nu;     // [nTBz] net stoichiometric coefficient (padding: 0)
     const unsigned long long* rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
     const unsigned long long* sp_pack;  // [maxDeg*64] rpack of each per-species list entry
+    int maxTerms;             // per-species Jacobian term list length, multiple of 16
+    const unsigned long long* sp_term;  // [maxTerms*64] lane-interleaved terms, see jterm_*
 };
 
+// Jacobian term of species row i: nu_ir * d q_r / d C_j for one reactant
+// (forward) or product (reverse) slot of reaction r.  Bits 0-15 r, 16-23
+// column j, 24-31 and 32-39 the other species of the slot's side (slot
+// order), 40-41 their count, 42 reverse, 48-55 the signed coefficient + 8
+// (negated for reverse terms; 0 = padding).
+__device__ __forceinline__ int jterm_r(unsigned long long w) { return (int)(w & 0xffffu); }
+__device__ __forceinline__ int jterm_col(unsigned long long w) { return (int)((w >> 16) & 255u); }
+__device__ __forceinline__ int jterm_o1(unsigned long long w) { return (int)((w >> 24) & 255u); }
+__device__ __forceinline__ int jterm_o2(unsigned long long w) { return (int)((w >> 32) & 255u); }
+__device__ __forceinline__ int jterm_no(unsigned long long w) { return (int)((w >> 40) & 3u); }
+__device__ __forceinline__ bool jterm_rev(unsigned long long w) { return (w >> 42) & 1u; }
+__device__ __forceinline__ int jterm_nu(unsigned long long w) { return (int)((w >> 48) & 255u) - 8; }
 __device__ __forceinline__ int rn_r(int w) { return w & 0xffff; }
 __device__ __forceinline__ int rn_nu(int w) { return (w >> 16) - 8; }
 __device__ __forceinline__ int pk_species(unsigned long long w, int q) { return (int)((w >> (8 * q)) & 255u); }
@@ -157,7 +171,7 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 // of one row index across the lanes are conflict-free; the pivot rows of the
 // LU are read by all lanes at one address (broadcast).
 struct Lds {
-    double* C;     // [64] concentrations
+    double* C;     // [65] concentrations, C[64] = 1
     double* expG;  // [64] exp(-G/RT)
     double* dBdT;  // [64] d(-G/RT)/dT
     double* rq;    // [nrP] q (rates) or kf (Jacobian)
@@ -179,7 +193,7 @@ __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
 {
     const size_t nrP = round_up(nr, 2);
     const size_t kq = 2 * nrP > 128 ? 2 * nrP : 128;   // rk + rdT, later H + Q
-    return 3 * 64 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32;
+    return 3 * 64 + 2 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32;
 }
 
 __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
@@ -187,10 +201,10 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
     Lds L;
     const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
     const int kq = 2 * nrP > 128 ? 2 * nrP : 128;
-    L.C = base;
-    L.expG = base + 64;
-    L.dBdT = base + 128;
-    L.rq = base + 192;
+    L.C = base;              // C[64] = 1.0: the "no other species" slot of the Jacobian terms
+    L.expG = base + 66;
+    L.dBdT = base + 130;
+    L.rq = base + 194;
     L.rk = L.rq + nrP;
     L.rdT = L.rk + nrP;
     L.H = L.rk;
@@ -437,59 +451,85 @@ __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 // src/Reaction/JacobianFunctionI.H:1150-1158.  The species of all six slots
 // are gathered unconditionally (unused slots name species 0) and the
 // products take the used slots in slot order.
+// One reaction of a round: rates from the preloaded record D and the
+// gathered slot values c (concentrations), g (exp(-G/RT)), db (d(-G/RT)/dT).
+template <bool JAC>
+__device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, const RxnData& D, int r,
+                                              const double* c, const double* g, const double* db, double T,
+                                              double lnT, double iT, double lnp, double PbyRT)
+{
+    const unsigned long long pk = D.pk;
+    const int nl = pk_nl(pk), nrh = pk_nr(pk);
+    const RxnK k = rate_constant<JAC>(M, D, r, T, lnT, iT, lnp);
+    double kr = 0.0, dkrdT = 0.0, invKc = 0.0;
+    if (D.rev) {
+        double num = 1.0, den = 1.0, sdb = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (a < nrh) { num *= g[3 + a]; if (JAC) sdb += db[3 + a]; }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (a < nl) { den *= g[a]; if (JAC) sdb -= db[a]; }
+        const int dn = nrh - nl;
+        double pw = 1.0;
+        if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
+        if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
+        double Kc = (num / den) * pw;
+        const bool floored = !(Kc > KC_FLOOR);
+        Kc = floored ? KC_FLOOR : Kc;
+        invKc = 1.0 / Kc;
+        kr = k.kf * invKc;
+        if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * (sdb - dn * iT));
+    }
+    double CF = 1.0, CR = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a < nl) CF *= c[a];
+        if (a < nrh) CR *= c[3 + a];
+    }
+    if (!JAC) {
+        L.rq[r] = k.kf * CF - kr * CR;
+    } else {
+        L.rq[r] = k.kf;
+        L.rk[r] = kr;
+        L.rdT[r] = k.dkfdT * CF - dkrdT * CR;
+        if (D.t == RXN_TB || D.t == RXN_LIND || D.t == RXN_TROE)
+            L.rw[D.tb] = k.dkfdM * CF - (k.dkfdM * invKc) * CR;
+    }
+}
+
+// Reaction round: lane l handles reactions l, l+64, ..., two per pass (their
+// table loads and gathers are all issued before either is evaluated).
+//  !JAC: rq[r] = q_r
+//   JAC: rq[r] = kf, rk[r] = kr, rdT[r] = dq/dT, rw[tb] = dkf/dM CF - dkr/dM CR
+// Kc in concentration units follows src/Reaction/ReactionFunctionI.H:1-52
+// and ChemicalEquilibriumFunctionI.H:14-25, d ln Kc / dT
+// src/Reaction/JacobianFunctionI.H:1150-1158.  The species of all six slots
+// are gathered unconditionally (unused slots name species 0) and the
+// products take the used slots in slot order.
 template <bool JAC>
 __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, double lnT, double iT,
                                       double lnp)
 {
     const double PbyRT = PSTD / (RU * T);
-    for (int r = lane_id(); r < M.nr; r += WAVE) {
-        const RxnData D = load_rxn(M, L, r);
-        const unsigned long long pk = D.pk;
-        const int nl = pk_nl(pk), nrh = pk_nr(pk);
-        double c[6], g[6], db[6];
+    for (int r0 = lane_id(); r0 < M.nr; r0 += 2 * WAVE) {
+        const int r1 = r0 + WAVE;
+        const bool has1 = r1 < M.nr;
+        const RxnData D0 = load_rxn(M, L, r0);
+        const RxnData D1 = load_rxn(M, L, has1 ? r1 : r0);
+        double c0[6], g0[6], db0[6], c1[6], g1[6], db1[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const int sq = pk_species(pk, q);
-            c[q] = L.C[sq];
-            g[q] = L.expG[sq];
-            db[q] = JAC ? L.dBdT[sq] : 0.0;
+            const int s0 = pk_species(D0.pk, q), s1 = pk_species(D1.pk, q);
+            c0[q] = L.C[s0];
+            g0[q] = L.expG[s0];
+            db0[q] = JAC ? L.dBdT[s0] : 0.0;
+            c1[q] = L.C[s1];
+            g1[q] = L.expG[s1];
+            db1[q] = JAC ? L.dBdT[s1] : 0.0;
         }
-        const RxnK k = rate_constant<JAC>(M, D, r, T, lnT, iT, lnp);
-        double kr = 0.0, dkrdT = 0.0, invKc = 0.0;
-        if (D.rev) {
-            double num = 1.0, den = 1.0, sdb = 0.0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (a < nrh) { num *= g[3 + a]; if (JAC) sdb += db[3 + a]; }
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (a < nl) { den *= g[a]; if (JAC) sdb -= db[a]; }
-            const int dn = nrh - nl;
-            double pw = 1.0;
-            if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
-            if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
-            double Kc = (num / den) * pw;
-            const bool floored = !(Kc > KC_FLOOR);
-            Kc = floored ? KC_FLOOR : Kc;
-            invKc = 1.0 / Kc;
-            kr = k.kf * invKc;
-            if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * (sdb - dn * iT));
-        }
-        double CF = 1.0, CR = 1.0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (a < nl) CF *= c[a];
-            if (a < nrh) CR *= c[3 + a];
-        }
-        if (!JAC) {
-            L.rq[r] = k.kf * CF - kr * CR;
-        } else {
-            L.rq[r] = k.kf;
-            L.rk[r] = kr;
-            L.rdT[r] = k.dkfdT * CF - dkrdT * CR;
-            if (D.t == RXN_TB || D.t == RXN_LIND || D.t == RXN_TROE)
-                L.rw[D.tb] = k.dkfdM * CF - (k.dkfdM * invKc) * CR;
-        }
+        reaction_eval<JAC>(M, L, D0, r0, c0, g0, db0, T, lnT, iT, lnp, PbyRT);
+        if (has1) reaction_eval<JAC>(M, L, D1, r1, c1, g1, db1, T, lnT, iT, lnp, PbyRT);
     }
 }
 
@@ -574,6 +614,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     const double dcpMdT = wave_sum(sp ? y * th.dcpdT : 0.0);
     const double Cl = sp ? rho * y * invW : 0.0;
     L.C[lane] = Cl;
+    if (lane == 0) L.C[64] = 1.0;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     L.dBdT[lane] = th.dBdT;
     double* Jrow = L.J + lane;   // element j of this lane's row at Jrow[j*JS]
@@ -583,14 +624,97 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     wave_sync();
     reaction_round<true>(M, L, T, lnT, iT, lnp);
     wave_sync();
-    // third-body / falloff partials: element (i, j) = sum_r nu_ir w_r eff_rj
-    // over the third-body/falloff entries of species i, before the sparse
-    // partials are added.  Column-wise over a flat entry list sorted by row
-    // (uniform loop, 16 entries per batch, coalesced efficiency rows): lane j
-    // forms row i's sum and stores it once.
+    // species rows, sparse partials: row i is a list of terms nu_ir dq_r/dC_j
+    // sorted by column (host order within a column follows the species'
+    // reaction list).  Each lane accumulates one column at a time in a
+    // register and stores it when the column changes (each element is
+    // written once, no read-modify-write).
+    if (sp) {
+        double acc = 0.0;
+        int cur = jterm_col(M.sp_term[lane]);
+        unsigned long long wn[16];   // next batch, loaded one batch ahead
+#pragma unroll
+        for (int u = 0; u < 16; ++u) wn[u] = M.sp_term[u * WAVE + lane];
+        for (int t0 = 0; t0 < M.maxTerms; t0 += 16) {
+            unsigned long long w[16];
+            double k[16], c1[16], c2[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) w[u] = wn[u];
+            const int tn = t0 + 16 < M.maxTerms ? t0 + 16 : t0;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) wn[u] = M.sp_term[(tn + u) * WAVE + lane];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int r = jterm_r(w[u]);
+                k[u] = jterm_rev(w[u]) ? L.rk[r] : L.rq[r];
+                c1[u] = L.C[jterm_o1(w[u])];   // 64 (= 1.0) when there is no other species
+                c2[u] = L.C[jterm_o2(w[u])];
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int nu = jterm_nu(w[u]);
+                if (nu == 0) continue;
+                const int col = jterm_col(w[u]);
+                if (col != cur) {
+                    Jrow[cur * JS] = acc;
+                    acc = 0.0;
+                    cur = col;
+                }
+                acc += (double)nu * ((k[u] * c1[u]) * c2[u]);
+            }
+        }
+        Jrow[cur * JS] = acc;
+    }
+    // production rate and d(omega)/dT of this lane's species (its reaction
+    // list, loads and gathers issued per block of 8 entries)
+    double om = 0.0, jT = 0.0;
+    if (sp) {
+        for (int e0 = 0; e0 < M.maxDeg; e0 += 8) {
+            double kf[8], kr[8], rd[8], c[8][6];
+            unsigned long long pk[8];
+            int wv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                wv[u] = M.sp_rn[(e0 + u) * WAVE + lane];
+                pk[u] = M.sp_pack[(e0 + u) * WAVE + lane];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int r = rn_r(wv[u]);
+                kf[u] = L.rq[r];
+                kr[u] = L.rk[r];
+                rd[u] = L.rdT[r];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) c[u][q] = L.C[pk_species(pk[u], q)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int nui = rn_nu(wv[u]);
+                if (nui == 0) continue;
+                const double nu = (double)nui;
+                const int nl = pk_nl(pk[u]), nrh = pk_nr(pk[u]);
+                double CF = 1.0, CR = 1.0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (q < nl) CF *= c[u][q];
+                    if (q < nrh) CR *= c[u][3 + q];
+                }
+                om += nu * (kf[u] * CF - kr[u] * CR);
+                jT += nu * rd[u];
+            }
+        }
+    }
+    wave_sync();
+    // third-body / falloff partials: element (i, j) gains sum_r nu_ir w_r
+    // eff_rj over the third-body/falloff entries of species i.  Column-wise
+    // over a flat entry list sorted by row (uniform loop, 16 entries per
+    // batch, coalesced efficiency rows): lane j forms row i's sum and adds it
+    // once; the row's old value is read when the row starts, so the read
+    // overlaps the row's sum.
     if (M.nTBz > 0) {
         double acc = 0.0;
         int cur = M.tbz_row[0];
+        double old = sp ? L.J[lane * JS + cur] : 0.0;
         for (int k0 = 0; k0 < M.nTBz; k0 += 16) {
             int rw_[16];
             double cf[16], ef[16];
@@ -604,74 +728,17 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 if (rw_[u] != cur) {
-                    if (sp) L.J[lane * JS + cur] = acc;
+                    if (sp) L.J[lane * JS + cur] = old + acc;
                     acc = 0.0;
                     cur = rw_[u];
+                    old = sp ? L.J[lane * JS + cur] : 0.0;
                 }
                 acc += cf[u] * ef[u];
             }
         }
-        if (sp) L.J[lane * JS + cur] = acc;
+        if (sp) L.J[lane * JS + cur] = old + acc;
     }
     wave_sync();
-    // species rows: sparse partials dq/dC into this lane's row.  The list is
-    // walked in blocks of 4 entries: all loads of a block are issued before
-    // its (sequential, order-preserving) updates.
-    double om = 0.0, jT = 0.0;
-    if (sp) {
-        for (int e0 = 0; e0 < M.maxDeg; e0 += 4) {
-            double nu[4], kf[4], kr[4], rd[4], c[4][6];
-            unsigned long long pk[4];
-            int r[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int w = M.sp_rn[(e0 + u) * WAVE + lane];
-                nu[u] = (double)rn_nu(w);
-                r[u] = rn_r(w);
-                pk[u] = M.sp_pack[(e0 + u) * WAVE + lane];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                kf[u] = L.rq[r[u]];
-                kr[u] = L.rk[r[u]];
-                rd[u] = L.rdT[r[u]];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) c[u][q] = L.C[pk_species(pk[u], q)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (nu[u] == 0.0) continue;
-                // (slot loops fully unrolled with guards: register-resident c[][])
-                const int nl = pk_nl(pk[u]), nrh = pk_nr(pk[u]);
-                double CF = 1.0, CR = 1.0;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    if (q < nl) CF *= c[u][q];
-                    if (q < nrh) CR *= c[u][3 + q];
-                }
-                om += nu[u] * (kf[u] * CF - kr[u] * CR);
-                jT += nu[u] * rd[u];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    if (q >= nl) continue;
-                    double d = kf[u];
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) if (b < nl && b != q) d *= c[u][b];
-                    Jrow[pk_species(pk[u], q) * JS] += nu[u] * d;
-                }
-                if (kr[u] != 0.0) {
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        if (q >= nrh) continue;
-                        double d = kr[u];
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) if (b < nrh && b != q) d *= c[u][3 + b];
-                        Jrow[pk_species(pk[u], 3 + q) * JS] -= nu[u] * d;
-                    }
-                }
-            }
-        }
-    }
     // dC/dY conversion of the species rows, T column
     const double WiByRho = sp ? S.W * invrho : 0.0;
     const double Yd = sp ? om * WiByRho : 0.0;
@@ -854,15 +921,40 @@ __device__ inline LuState lu_factor(const Lds& L)
         const bool below = row && pos > k0 + 3;
         const double m0 = below ? pa[0] : 0.0, m1 = below ? pa[1] : 0.0, m2 = below ? pa[2] : 0.0,
                      m3 = below ? pa[3] : 0.0;
+        // (software pipelined: the next block of 4 columns is loaded before
+        // the current one is updated; blocks are disjoint, so this is safe)
+        double nv[4], n0[4], n1[4], n2[4], n3[4];
+        if (k0 + 4 < NP) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = k0 + 4 + u;
+                nv[u] = Jrow[j * JS];
+                n0[u] = L.J[j * JS + pl[0]];
+                n1[u] = L.J[j * JS + pl[1]];
+                n2[u] = L.J[j * JS + pl[2]];
+                n3[u] = L.J[j * JS + pl[3]];
+            }
+        }
         for (int j0 = k0 + 4; j0 < NP; j0 += 4) {
             double v[4], u0[4], u1[4], u2[4], u3[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                v[u] = Jrow[(j0 + u) * JS];
-                u0[u] = L.J[(j0 + u) * JS + pl[0]];
-                u1[u] = L.J[(j0 + u) * JS + pl[1]];
-                u2[u] = L.J[(j0 + u) * JS + pl[2]];
-                u3[u] = L.J[(j0 + u) * JS + pl[3]];
+                v[u] = nv[u];
+                u0[u] = n0[u];
+                u1[u] = n1[u];
+                u2[u] = n2[u];
+                u3[u] = n3[u];
+            }
+            if (j0 + 4 < NP) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + 4 + u;
+                    nv[u] = Jrow[j * JS];
+                    n0[u] = L.J[j * JS + pl[0]];
+                    n1[u] = L.J[j * JS + pl[1]];
+                    n2[u] = L.J[j * JS + pl[2]];
+                    n3[u] = L.J[j * JS + pl[3]];
+                }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {

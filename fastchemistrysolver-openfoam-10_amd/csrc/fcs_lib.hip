@@ -617,6 +617,40 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     const size_t iA = vd(rA), iB = vd(rbeta), iTa = vd(rTa), iA0 = vd(rA0), iB0 = vd(rbeta0), iTa0 = vd(rTa0);
     const size_t iTroe = vd(troe), iEff = vd(eff), iEffT = vd(effT), iRtroe = vd(rtroe);
     const size_t iPn = vi(plogN), iPp = vd(plogLnp), iPA = vd(plogLnA), iPb = vd(plogBeta), iPT = vd(plogTa);
+    // per-species Jacobian term lists (nu_ir dq_r/dC_j per reactant / product
+    // slot), sorted by column j (stable: reaction-list order within a column)
+    std::vector<std::vector<unsigned long long>> terms(ns);
+    for (int s = 0; s < ns; ++s) {
+        for (const auto& e : lst[s]) {
+            const int r = e.first, nu = (int)e.second;
+            for (int side = 0; side < 2; ++side) {
+                if (side == 1 && !rrev[r]) continue;
+                const int cnt = side == 0 ? rnl[r] : rnr[r];
+                const int sgn = side == 0 ? nu : -nu;
+                for (int q = 0; q < cnt; ++q) {
+                    int oth[2] = {64, 64}, no = 0;   // 64: the LDS slot holding 1.0
+                    for (int b = 0; b < cnt; ++b)
+                        if (b != q) oth[no++] = rsp[(3 * side + b) * nr + r];
+                    const unsigned long long col = (unsigned long long)rsp[(3 * side + q) * nr + r];
+                    terms[s].push_back((unsigned long long)r | col << 16 | (unsigned long long)oth[0] << 24 |
+                                       (unsigned long long)oth[1] << 32 | (unsigned long long)no << 40 |
+                                       (unsigned long long)side << 42 | (unsigned long long)(sgn + 8) << 48);
+                }
+            }
+        }
+        std::stable_sort(terms[s].begin(), terms[s].end(), [](unsigned long long a, unsigned long long b) {
+            return ((a >> 16) & 255u) < ((b >> 16) & 255u);
+        });
+    }
+    int maxTerms = 16;
+    for (int s = 0; s < ns; ++s) maxTerms = std::max(maxTerms, (int)terms[s].size());
+    maxTerms = (maxTerms + 15) / 16 * 16;
+    std::vector<unsigned long long> spTerm((size_t)maxTerms * 64, 8ull << 48);   // padding: column 0, nu 0
+    for (int s = 0; s < ns; ++s) {
+        const unsigned long long lastCol = terms[s].empty() ? 0ull : (terms[s].back() >> 16) & 255u;
+        for (int e = 0; e < maxTerms; ++e)
+            spTerm[(size_t)e * 64 + s] = e < (int)terms[s].size() ? terms[s][e] : (lastCol << 16 | 8ull << 48);
+    }
     // flat third-body entry list padded to a multiple of 16 with zero entries on the last row
     const int nTBz = (int)tbzRow.size();
     while (tbzRow.size() % 16) {
@@ -624,7 +658,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         tbzTb.push_back(0);
         tbzNu.push_back(0.0);
     }
-    const size_t iSpRN = vi(spRN);
+    const size_t iSpRN = vi(spRN), iSpTerm = add(spTerm.data(), spTerm.size() * 8);
     const size_t iTbzRow = vi(tbzRow), iTbzTb = vi(tbzTb), iTbzNu = vd(tbzNu);
     auto vq = [&](const std::vector<unsigned long long>& v) { return add(v.data(), v.size() * 8); };
     const size_t iRpack = vq(rpack), iSpPack = vq(spPack);
@@ -683,6 +717,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.plog_beta = (const double*)P(iPb);
     dm.plog_Ta = (const double*)P(iPT);
     dm.sp_rn = (const int*)P(iSpRN);
+    dm.maxTerms = maxTerms;
+    dm.sp_term = (const unsigned long long*)P(iSpTerm);
     dm.nTBz = nTBz > 0 ? (int)tbzRow.size() : 0;
     dm.tbz_row = (const int*)P(iTbzRow);
     dm.tbz_tb = (const int*)P(iTbzTb);
