@@ -26,6 +26,7 @@ using namespace fcs;
 struct CellArrays {
     int nc;
     double dt;                       // flow time step deltaT
+    const double* dtCell;            // per-cell deltaT of the local-time-step solve, or null
     const double *T0, *p0, *rho0, *Y0, *dtcIn;
     double *dtcOut, *RR, *Tend;
     int *status, *nAtt;
@@ -86,7 +87,8 @@ __global__ __launch_bounds__(256) void k_setup(DevOpts o, int ns, CellArrays a, 
 
 // Writes the flow-step results of a finished cell (OptRodas34.C:114-117,
 // FastChemistryModel_transientSolve.H:59-67): RR = (max(0, y) rho0 - Y0 rho0)
-// / deltaT, the new deltaTChem (the input one when the integration failed)
+// / deltaT (or (max(0, y) - Y0) rho0 / deltaT_c for the local-time-step
+// solve), the new deltaTChem (the input one when the integration failed)
 // capped by deltaTChemMax, and the deltaTMin reduction.
 template <int NP>
 __device__ void finish_cell(const DevMech& M, const DevOpts& o, const CellArrays& a, int c, int st, double y,
@@ -98,7 +100,9 @@ __device__ void finish_cell(const DevMech& M, const DevOpts& o, const CellArrays
     if (lane < M.ns) {
         const double Yn = y > 0.0 ? y : 0.0;
         const double r0 = a.rho0[c];
-        a.RR[lane * nc + c] = (Yn * r0 - a.Y0[lane * nc + c] * r0) / a.dt;
+        const double Y0 = a.Y0[lane * nc + c];
+        // FastChemistryModel_transientSolve.H:64-67 / FastChemistryModel_localEulerSolve.H:58-61
+        a.RR[lane * nc + c] = a.dtCell ? (Yn - Y0) * r0 / a.dtCell[c] : (Yn * r0 - Y0 * r0) / a.dt;
     }
     if (lane == NP - 1) a.Tend[c] = y;
     if (lane == 0) {
@@ -133,7 +137,8 @@ __global__ __launch_bounds__(64) void k_step(DevMech M, DevOpts o, CellArrays a,
         double y = row >= 0 ? a.Yw[row * nc + c] : 0.0;
         double x = a.x[c], dxTry = a.dxTry[c], dx = a.dx[c], dxTry0 = a.dxTry0[c];
         int flags = a.flags[c], nStep = a.nStep[c], nAtt = a.nAtt[c];
-        const int st = rosenbrock_step<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, a.p0[c], a.dt, y,
+        const double xEnd = a.dtCell ? a.dtCell[c] : a.dt;
+        const int st = rosenbrock_step<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, a.p0[c], xEnd, y,
                                            x, dxTry, dx, dxTry0, flags, nStep, nAtt);
         if (st == STEP_CONTINUE) {
             if (row >= 0) a.Yw[row * nc + c] = y;
@@ -185,7 +190,7 @@ __global__ __launch_bounds__(64) void k_step_seulex(DevMech M, DevOpts o, CellAr
         sc.nStep = a.nStep[c];
         sc.nAtt = a.nAtt[c];
         double* JyG = a.Jy + (size_t)c * NP * WAVE;
-        const int st = seulex_pass<NP>(M, L, o, a.p0[c], a.dt, JyG, y, y0s, scale, sc);
+        const int st = seulex_pass<NP>(M, L, o, a.p0[c], a.dtCell ? a.dtCell[c] : a.dt, JyG, y, y0s, scale, sc);
         if (st == STEP_CONTINUE) {
             if (row >= 0) {
                 a.Yw[row * nc + c] = y;
@@ -360,6 +365,7 @@ struct DevCells {
     int ns = 0;
     void* base = nullptr;
     double *T0 = nullptr, *p0 = nullptr, *rho0 = nullptr, *Y0 = nullptr, *dtcIn = nullptr, *dtcOut = nullptr,
+           *dtField = nullptr,
            *RR = nullptr, *Tend = nullptr, *Yw = nullptr, *x = nullptr, *dxTry = nullptr, *dx = nullptr,
            *dxTry0 = nullptr, *Y0s = nullptr, *Scl = nullptr, *theta = nullptr, *Jy = nullptr;
     int *status = nullptr, *nAtt = nullptr, *flags = nullptr, *nStep = nullptr, *list0 = nullptr, *list1 = nullptr,
@@ -383,7 +389,7 @@ struct DevCells {
         const size_t d = align256(nc * 8), i = align256(nc * 4);
         const size_t nsd = align256(nc * nsp * 8), nwd = align256(nc * (nsp + 1) * 8);
         const size_t jyb = align256(nc * jyDoubles * 8);
-        const size_t total = 11 * d + 2 * nsd + 3 * nwd + 7 * i + jyb + 256 + 256;
+        const size_t total = 12 * d + 2 * nsd + 3 * nwd + 7 * i + jyb + 256 + 256;
         if (hipMalloc(&base, total) != hipSuccess) {
             base = nullptr;
             return FCS_E_NOMEM;
@@ -392,7 +398,7 @@ struct DevCells {
         auto take = [&](size_t bytes) { char* r = q; q += bytes; return r; };
         T0 = (double*)take(d); p0 = (double*)take(d); rho0 = (double*)take(d); dtcIn = (double*)take(d);
         dtcOut = (double*)take(d); Tend = (double*)take(d); x = (double*)take(d); dxTry = (double*)take(d);
-        dx = (double*)take(d); dxTry0 = (double*)take(d); theta = (double*)take(d);
+        dx = (double*)take(d); dxTry0 = (double*)take(d); theta = (double*)take(d); dtField = (double*)take(d);
         Y0 = (double*)take(nsd); RR = (double*)take(nsd); Yw = (double*)take(nwd);
         Y0s = (double*)take(nwd); Scl = (double*)take(nwd);
         status = (int*)take(i); nAtt = (int*)take(i); flags = (int*)take(i); nStep = (int*)take(i);
@@ -428,11 +434,12 @@ struct DevCells {
         hostCount = nullptr;
         cap = 0;
     }
-    CellArrays view(int nc, double dt) const
+    CellArrays view(int nc, double dt, bool lts) const
     {
         CellArrays a;
         a.nc = nc;
         a.dt = dt;
+        a.dtCell = lts ? dtField : nullptr;
         a.T0 = T0; a.p0 = p0; a.rho0 = rho0; a.Y0 = Y0; a.dtcIn = dtcIn;
         a.dtcOut = dtcOut; a.RR = RR; a.Tend = Tend; a.status = status; a.nAtt = nAtt; a.dtMinBits = dtMin;
         a.Yw = Yw; a.x = x; a.dxTry = dxTry; a.dx = dx; a.dxTry0 = dxTry0; a.flags = flags; a.nStep = nStep;
@@ -816,10 +823,10 @@ static DevOpts dev_opts(const fcs_solver_opts& o)
 // launch kPoll-1 places back, so the GPU always has step launches queued.
 // Every cell finishes within 101 * maxSteps attempts (maxSteps accepted
 // steps plus the rejection cap), which bounds the launch count.
-static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt)
+static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = false)
 {
     hipStream_t st = s->stream;
-    CellArrays a = d.view(nc, dt);
+    CellArrays a = d.view(nc, dt, lts);
     DevMech M = s->mech->dm;
     DevOpts o = dev_opts(s->opts);
     int ns = s->mech->ns;
@@ -876,17 +883,24 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt)
     return FCS_OK;
 }
 
-int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0, const double* p0,
-                 const double* rho0, const double* const* Y0, double* deltaTChem, double* const* RR,
-                 double* deltaTMin, int32_t* cellStatus, double* Tend, int32_t* nAttempts)
+// solve(deltaT) / solve(scalarField deltaT) over host arrays: dtField == null
+// selects the serial contract (FastChemistryModel_transientSolve.H:3-104,
+// deltaTMin capped by 2*deltaT), else the local-time-step one
+// (FastChemistryModel_localEulerSolve.H:3-101, per-cell deltaT, plain
+// minimum).
+static int solve_host(fcs_solver* s, int64_t nCells, double deltaT, const double* dtField, const double* T0,
+                      const double* p0, const double* rho0, const double* const* Y0, double* deltaTChem,
+                      double* const* RR, double* deltaTMin, int32_t* cellStatus, double* Tend, int32_t* nAttempts)
 {
-    if (!s || nCells < 0 || nCells > 0x7fffffff || !(deltaT > 0) || (nCells > 0 && (!T0 || !p0 || !rho0 || !Y0 ||
-                                                                                   !deltaTChem || !RR)))
+    if (!s || nCells < 0 || nCells > 0x7fffffff || (!dtField && !(deltaT > 0)) ||
+        (nCells > 0 && (!T0 || !p0 || !rho0 || !Y0 || !deltaTChem || !RR)))
         return FCS_E_ARG;
     const int ns = s->mech->ns;
     const size_t nc = (size_t)nCells;
+    for (size_t c = 0; dtField && c < nc; ++c)
+        if (!(dtField[c] > 0)) return FCS_E_ARG;
     if (nc == 0) {
-        if (deltaTMin) *deltaTMin = std::min(OF_GREAT, 2 * deltaT);
+        if (deltaTMin) *deltaTMin = dtField ? OF_GREAT : std::min(OF_GREAT, 2 * deltaT);
         return FCS_OK;
     }
     DevCells& d = s->cells;
@@ -897,9 +911,10 @@ int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0,
     HIP_OK(hipMemcpyAsync(d.p0, p0, nc * 8, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d.rho0, rho0, nc * 8, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d.dtcIn, deltaTChem, nc * 8, hipMemcpyHostToDevice, st));
+    if (dtField) HIP_OK(hipMemcpyAsync(d.dtField, dtField, nc * 8, hipMemcpyHostToDevice, st));
     for (int i = 0; i < ns; ++i)
         HIP_OK(hipMemcpyAsync(d.Y0 + (size_t)i * nc, Y0[i], nc * 8, hipMemcpyHostToDevice, st));
-    int rc = run_cells(s, d, (int)nc, deltaT);
+    int rc = run_cells(s, d, (int)nc, deltaT, dtField != nullptr);
     if (rc != FCS_OK) return rc;
     for (int i = 0; i < ns; ++i)
         HIP_OK(hipMemcpyAsync(RR[i], d.RR + (size_t)i * nc, nc * 8, hipMemcpyDeviceToHost, st));
@@ -914,7 +929,7 @@ int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0,
     double dmin;
     std::memcpy(&dmin, &bits, 8);
     dmin = std::min(dmin, OF_GREAT);
-    if (deltaTMin) *deltaTMin = std::min(dmin, 2 * deltaT);
+    if (deltaTMin) *deltaTMin = dtField ? dmin : std::min(dmin, 2 * deltaT);
     bool bad = false;
     for (size_t c = 0; c < nc; ++c) {
         if (cellStatus) cellStatus[c] = stat[c];
@@ -923,11 +938,28 @@ int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0,
     return bad ? FCS_E_CELLS : FCS_OK;
 }
 
+int fcs_solve_ex(fcs_solver* s, int64_t nCells, double deltaT, const double* T0, const double* p0,
+                 const double* rho0, const double* const* Y0, double* deltaTChem, double* const* RR,
+                 double* deltaTMin, int32_t* cellStatus, double* Tend, int32_t* nAttempts)
+{
+    return solve_host(s, nCells, deltaT, nullptr, T0, p0, rho0, Y0, deltaTChem, RR, deltaTMin, cellStatus, Tend,
+                      nAttempts);
+}
+
 int fcs_solve(fcs_solver* s, int64_t nCells, double deltaT, const double* T0, const double* p0, const double* rho0,
               const double* const* Y0, double* deltaTChem, double* const* RR, double* deltaTMin, int32_t* cellStatus)
 {
-    return fcs_solve_ex(s, nCells, deltaT, T0, p0, rho0, Y0, deltaTChem, RR, deltaTMin, cellStatus, nullptr,
-                        nullptr);
+    return solve_host(s, nCells, deltaT, nullptr, T0, p0, rho0, Y0, deltaTChem, RR, deltaTMin, cellStatus, nullptr,
+                      nullptr);
+}
+
+int fcs_solve_lts(fcs_solver* s, int64_t nCells, const double* deltaT, const double* T0, const double* p0,
+                  const double* rho0, const double* const* Y0, double* deltaTChem, double* const* RR,
+                  double* deltaTMin, int32_t* cellStatus, double* Tend, int32_t* nAttempts)
+{
+    if (!deltaT && nCells > 0) return FCS_E_ARG;
+    return solve_host(s, nCells, 0.0, deltaT, T0, p0, rho0, Y0, deltaTChem, RR, deltaTMin, cellStatus, Tend,
+                      nAttempts);
 }
 
 // ------------------------------------------------------------ batches

@@ -20,7 +20,7 @@ _vp = C.c_void_p
 
 EXPORTS = [
     "fcs_device_count", "fcs_version", "fcs_mech_create", "fcs_mech_destroy", "fcs_solver_create",
-    "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_batch_create", "fcs_batch_destroy",
+    "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_solve_lts", "fcs_batch_create", "fcs_batch_destroy",
     "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_download",
     "fcs_eval_derivatives", "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
 ]
@@ -55,6 +55,8 @@ def load(path: str | None = None):
     L.fcs_solve.argtypes = [_vp, C.c_int64, C.c_double, _dp, _dp, _dp, C.POINTER(_dp), _dp,
                             C.POINTER(_dp), _dp, _ip]
     L.fcs_solve_ex.argtypes = L.fcs_solve.argtypes + [_dp, _ip]
+    L.fcs_solve_lts.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp, C.POINTER(_dp), _dp,
+                                C.POINTER(_dp), _dp, _ip, _dp, _ip]
     L.fcs_batch_create.argtypes = [_vp, C.c_int64, C.POINTER(_vp)]
     L.fcs_batch_destroy.argtypes = [_vp]
     L.fcs_batch_upload.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
@@ -120,7 +122,8 @@ class Solver:
             _lib.fcs_solver_destroy(self._h)
             self._h = None
 
-    # ---- solve(deltaT) over host fields
+    # ---- solve(deltaT) over host fields (an array deltaT selects the
+    # local-time-step solve(scalarField deltaT), fcs_solve_lts)
     def solve(self, field, deltaT, rho=None):
         nc = field.n_cells
         T0 = np.ascontiguousarray(field.T, np.float64)
@@ -133,10 +136,18 @@ class Solver:
         st = np.zeros(nc, np.int32)
         Tend = np.zeros(nc)
         na = np.zeros(nc, np.int32)
-        rc = load().fcs_solve_ex(self._h, nc, float(deltaT), dptr(T0), dptr(p0), dptr(rho0),
-                                 row_pointers(Y0), dptr(dtc), row_pointers(RR), dptr(dmin), iptr(st),
-                                 dptr(Tend), iptr(na))
-        check(rc, "fcs_solve_ex", allow=(FCS_E_CELLS,))
+        if np.ndim(deltaT) == 0:
+            rc = load().fcs_solve_ex(self._h, nc, float(deltaT), dptr(T0), dptr(p0), dptr(rho0),
+                                     row_pointers(Y0), dptr(dtc), row_pointers(RR), dptr(dmin), iptr(st),
+                                     dptr(Tend), iptr(na))
+        else:
+            dtf = np.ascontiguousarray(deltaT, np.float64)
+            if dtf.shape != (nc,):
+                raise ValueError("deltaT field must have one entry per cell")
+            rc = load().fcs_solve_lts(self._h, nc, dptr(dtf), dptr(T0), dptr(p0), dptr(rho0),
+                                      row_pointers(Y0), dptr(dtc), row_pointers(RR), dptr(dmin), iptr(st),
+                                      dptr(Tend), iptr(na))
+        check(rc, "fcs_solve", allow=(FCS_E_CELLS,))
         return dict(rc=rc, RR=RR, deltaTChem=dtc, deltaTMin=float(dmin[0]), status=st, Tend=Tend,
                     nAttempts=na, rho0=rho0)
 
@@ -225,3 +236,20 @@ class Batch:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.fcs_batch_destroy(self._h)
             self._h = None
+
+
+def equilibrator(solver: "Solver", deltaT=4e-3):
+    """An ``equilibrate`` callable for the counterflow field generator: it
+    integrates a Field for deltaT (default 4 ms, long past the initial
+    transient of the synthetic 53-species mechanism) on the GPU and returns
+    the end states (T from Tend, Y = Y0 + RR deltaT / rho0, clipped and
+    renormalised)."""
+    from .mechanism import Field
+
+    def run(field):
+        r = solver.solve(field, deltaT)
+        if r["rc"] != FCS_OK:
+            raise FcsError(f"equilibration failed with code {r['rc']}", r["rc"])
+        Y = np.maximum(field.Y + r["RR"] * deltaT / r["rho0"][None, :], 0.0)
+        return Field(r["Tend"].copy(), field.p.copy(), Y / Y.sum(axis=0), field.deltaTChem.copy())
+    return run

@@ -345,6 +345,7 @@ class Field:
     p: np.ndarray        # [nc]
     Y: np.ndarray        # [ns, nc] species-major
     deltaTChem: np.ndarray
+    cls: np.ndarray | None = None   # cell class of generated fields (counterflow: 0 cold, 1 product, 2 flame)
 
     @property
     def n_cells(self):
@@ -356,7 +357,7 @@ class Field:
 
     def subset(self, idx):
         return Field(self.T[idx].copy(), self.p[idx].copy(), self.Y[:, idx].copy(),
-                     self.deltaTChem[idx].copy())
+                     self.deltaTChem[idx].copy(), None if self.cls is None else self.cls[idx].copy())
 
 
 def uniform_field(mech: Mechanism, n_cells, seed, T_range=(1000.0, 1500.0), p=101325.0,
@@ -382,6 +383,57 @@ def mixture_fraction_field(mech: Mechanism, n_cells, seed, T_peak=2200.0, T_end=
     return Field(T, np.full(n_cells, p), Y, np.full(n_cells, dtchem0))
 
 
+def counterflow_field(mech: Mechanism, n_side, seed, equilibrate, T_react=500.0, n_base=16,
+                      p=1.0e5, dtchem0=1e-7) -> Field:
+    """Synthetic 2-D counterflow field of n_side x n_side cells, shuffled.
+
+    Across the burner axis x in [0, 1] the cells fall into three bands: the
+    two cold inflow streams |x - 0.5| > 0.275 (45 %, T in [300, 480) K, below
+    T_react, oxidizer or fuel composition), the hot product layers
+    0.075 < |x - 0.5| <= 0.275 (40 %, near-equilibrium states), and the thin
+    flame front |x - 0.5| <= 0.075 (15 %, T from 900 K at the band edges to
+    1800 K at the centre, composition linear in the mixture fraction).
+
+    The product states come from ``equilibrate(field) -> field``, which must
+    integrate a small Field of hot mixtures long enough to bring them close
+    to chemical equilibrium (the GPU integrator does this at bench time);
+    each product cell is one of n_base such states with 1e-9 relative noise
+    on Y, so a 1 us flow step takes the integrator only a few sub-steps.
+    """
+    rng = np.random.default_rng(seed)
+    n = n_side * n_side
+    Yox = rng.dirichlet(np.ones(mech.n_species))
+    Yfu = rng.dirichlet(np.ones(mech.n_species))
+    x = (np.arange(n) % n_side + 0.5) / n_side
+    yy = (np.arange(n) // n_side + 0.5) / n_side
+    d = np.abs(x - 0.5)
+    cold = d > 0.275
+    flame = d <= 0.075
+    hot = ~cold & ~flame
+    Z = np.clip(x + 0.02 * np.sin(2 * np.pi * yy), 0.0, 1.0)
+    T = np.empty(n)
+    Y = np.empty((mech.n_species, n))
+    # cold streams
+    T[cold] = rng.uniform(300.0, 0.96 * T_react, cold.sum())
+    Y[:, cold] = np.where(x[cold] < 0.5, Yox[:, None], Yfu[:, None])
+    # flame front
+    T[flame] = 1800.0 - 900.0 * d[flame] / 0.075 + rng.uniform(-5.0, 5.0, flame.sum())
+    T[flame] = np.clip(T[flame], 900.0, 1800.0)
+    Y[:, flame] = (1.0 - Z[flame])[None, :] * Yox[:, None] + Z[flame][None, :] * Yfu[:, None]
+    # hot products: n_base equilibrated states
+    Zb = np.linspace(0.2, 0.8, n_base)
+    base = Field(np.linspace(1900.0, 2200.0, n_base), np.full(n_base, p),
+                 (1.0 - Zb)[None, :] * Yox[:, None] + Zb[None, :] * Yfu[:, None], np.full(n_base, dtchem0))
+    eq = equilibrate(base)
+    k = np.minimum((yy[hot] * n_base).astype(int), n_base - 1)
+    Yh = eq.Y[:, k] * (1.0 + 1e-9 * rng.standard_normal((mech.n_species, k.size)))
+    Y[:, hot] = np.maximum(Yh, 0.0) / np.maximum(Yh, 0.0).sum(axis=0)
+    T[hot] = eq.T[k]
+    cls = np.where(cold, 0, np.where(flame, 2, 1)).astype(np.int32)
+    order = rng.permutation(n)
+    return Field(T[order].copy(), np.full(n, p), Y[:, order].copy(), np.full(n, dtchem0), cls[order].copy())
+
+
 # Benchmark configurations (mechanism seed; field seed = mechanism seed + 1).
 def config_mechanism(cfg: int) -> Mechanism:
     if cfg == 0:
@@ -393,11 +445,22 @@ def config_mechanism(cfg: int) -> Mechanism:
     raise ValueError(cfg)
 
 
-def config_field(cfg: int, mech: Mechanism, n_cells=None) -> Field:
+def config_field(cfg: int, mech: Mechanism, n_cells=None, equilibrate=None) -> Field:
+    """Cell field of a benchmark configuration.  Configurations 2 and 3 use
+    the 512 x 512 counterflow field (n_cells must then be a square) and need
+    an ``equilibrate`` callable (see counterflow_field)."""
     if cfg == 0:
         return uniform_field(mech, n_cells or 1024, seed=1001)
     if cfg == 1:
         return mixture_fraction_field(mech, n_cells or 65536, seed=1054)
+    if cfg in (2, 3):
+        n = n_cells or 262144
+        side = int(round(np.sqrt(n)))
+        if side * side != n:
+            raise ValueError("configuration 2/3 fields are square: n_cells must be a square number")
+        if equilibrate is None:
+            raise ValueError("configuration 2/3 fields need an equilibrate callable")
+        return counterflow_field(mech, side, seed=1054, equilibrate=equilibrate)
     raise ValueError(cfg)
 
 

@@ -162,6 +162,21 @@ int fcs_solve_ex(fcs_solver* solver, int64_t nCells, double deltaT,
                  double* Tend, int32_t* nAttempts);
 
 /*
+ * fcs_solve_lts: FastChemistryModel::solve(const scalarField& deltaT), the
+ * local-time-step variant (src/ChemistryModel/FastChemistryModel.H:296,
+ * FastChemistryModel_localEulerSolve.H:3-101): cell c integrates over
+ * deltaT[c] > 0, RR = (Y - Y0) * rho0 / deltaT[c], and deltaTMin is the
+ * minimum over integrated cells of the new deltaTChem (great when none is
+ * integrated), without the 2*deltaT cap of fcs_solve.  Tend and nAttempts
+ * as in fcs_solve_ex (may be NULL).
+ */
+int fcs_solve_lts(fcs_solver* solver, int64_t nCells, const double* deltaT,
+                  const double* T0, const double* p0, const double* rho0,
+                  const double* const* Y0, double* deltaTChem,
+                  double* const* RR, double* deltaTMin, int32_t* cellStatus,
+                  double* Tend, int32_t* nAttempts);
+
+/*
  * Device-resident batch: inputs uploaded once, integrated in place in HBM.
  * Layout in HBM is structure-of-arrays, species-major: Y[i*nCells + c].
  */

@@ -1111,11 +1111,18 @@ double orc_rosenbrock_attempt(const orc_mech* m, const fcs_solver_opts* o, doubl
     return err;
 }
 
-/* src/ChemistryModel/FastChemistryModel_transientSolve.H:3-104 */
-int orc_solve(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells, double deltaT,
-              const double* T0, const double* p0, const double* rho0, const double* Y0,
-              double* deltaTChem, double* RR, double* deltaTMin, int32_t* cellStatus,
-              double* Tend, int32_t* nAttempts, int nThreads)
+/* Per-cell driver shared by the serial solve(deltaT)
+ * (src/ChemistryModel/FastChemistryModel_transientSolve.H:3-104) and the
+ * local-time-step solve(scalarField deltaT)
+ * (src/ChemistryModel/FastChemistryModel_localEulerSolve.H:3-101).  With
+ * dtField != NULL cell c integrates over dtField[c], RR is formed as
+ * (Y - Y0)*rho0/deltaT[c] (localEulerSolve.H:58-61) and deltaTMin is the
+ * plain minimum (:103); otherwise RR = (Y*rho0 - Y0*rho0)/deltaT
+ * (transientSolve.H:64-67) and deltaTMin = min(min, 2*deltaT) (:103). */
+static int solve_cells(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells, double deltaT,
+                       const double* dtField, const double* T0, const double* p0, const double* rho0,
+                       const double* Y0, double* deltaTChem, double* RR, double* deltaTMin,
+                       int32_t* cellStatus, double* Tend, int32_t* nAttempts, int nThreads)
 {
     const int ns = m->ns;
     double dmin = OF_GREAT;
@@ -1128,16 +1135,20 @@ int orc_solve(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells, doubl
         double y[512];
         int32_t st = 0, na = 0;
         const double T = T0[c];
+        const double dtc = dtField ? dtField[c] : deltaT;
         if (T > o->Treact) {
             for (int i = 0; i < ns; ++i) y[i] = Y0[(size_t)i * nCells + c];
             y[ns] = T;
             double dt = deltaTChem[c];
-            st = orc_integrate_cell(m, o, p0[c], y, deltaT, &dt, &na, NULL, 0, NULL);
+            st = orc_integrate_cell(m, o, p0[c], y, dtc, &dt, &na, NULL, 0, NULL);
             deltaTChem[c] = dt;
             dmin = dt < dmin ? dt : dmin;
             if (deltaTChem[c] > o->deltaTChemMax) deltaTChem[c] = o->deltaTChemMax;
-            for (int i = 0; i < ns; ++i)
-                RR[(size_t)i * nCells + c] = (y[i] * rho0[c] - Y0[(size_t)i * nCells + c] * rho0[c]) / deltaT;
+            for (int i = 0; i < ns; ++i) {
+                const double Y0i = Y0[(size_t)i * nCells + c];
+                RR[(size_t)i * nCells + c] = dtField ? (y[i] - Y0i) * rho0[c] / dtc
+                                                     : (y[i] * rho0[c] - Y0i * rho0[c]) / dtc;
+            }
             if (Tend) Tend[c] = y[ns];
         } else {
             st = FCS_CELL_SKIPPED;
@@ -1148,7 +1159,27 @@ int orc_solve(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells, doubl
         if (nAttempts) nAttempts[c] = na;
         if (st & (FCS_CELL_MAXSTEPS | FCS_CELL_NONFINITE)) anyFail |= 1;
     }
-    const double twodt = 2 * deltaT;
-    if (deltaTMin) *deltaTMin = dmin < twodt ? dmin : twodt;
+    if (deltaTMin) {
+        if (dtField) *deltaTMin = dmin;
+        else *deltaTMin = dmin < 2 * deltaT ? dmin : 2 * deltaT;
+    }
     return anyFail ? FCS_E_CELLS : FCS_OK;
+}
+
+int orc_solve(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells, double deltaT,
+              const double* T0, const double* p0, const double* rho0, const double* Y0,
+              double* deltaTChem, double* RR, double* deltaTMin, int32_t* cellStatus,
+              double* Tend, int32_t* nAttempts, int nThreads)
+{
+    return solve_cells(m, o, nCells, deltaT, NULL, T0, p0, rho0, Y0, deltaTChem, RR, deltaTMin,
+                       cellStatus, Tend, nAttempts, nThreads);
+}
+
+int orc_solve_lts(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells, const double* deltaT,
+                  const double* T0, const double* p0, const double* rho0, const double* Y0,
+                  double* deltaTChem, double* RR, double* deltaTMin, int32_t* cellStatus,
+                  double* Tend, int32_t* nAttempts, int nThreads)
+{
+    return solve_cells(m, o, nCells, 0.0, deltaT, T0, p0, rho0, Y0, deltaTChem, RR, deltaTMin,
+                       cellStatus, Tend, nAttempts, nThreads);
 }

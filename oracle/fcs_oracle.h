@@ -77,6 +77,12 @@ int orc_solve(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells,
               const double* rho0, const double* Y0, double* deltaTChem,
               double* RR, double* deltaTMin, int32_t* cellStatus,
               double* Tend, int32_t* nAttempts, int nThreads);
+/* local-time-step twin: src/ChemistryModel/FastChemistryModel_localEulerSolve.H:3-101 */
+int orc_solve_lts(const orc_mech* m, const fcs_solver_opts* o, int64_t nCells,
+              const double* deltaT, const double* T0, const double* p0,
+              const double* rho0, const double* Y0, double* deltaTChem,
+              double* RR, double* deltaTMin, int32_t* cellStatus,
+              double* Tend, int32_t* nAttempts, int nThreads);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,8 @@ def lib():
         L.orc_rosenbrock_attempt.restype = C.c_double
         L.orc_solve.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, C.c_double,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
+        L.orc_solve_lts.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, _dp,
+                                    _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
     return _lib
 
 
@@ -128,7 +130,8 @@ class Oracle:
         return dict(rc=rc, y=y, dxTry=float(dx[0]), nAttempts=int(na[0]), log=log)
 
     def solve(self, field, deltaT, threads=1, rho=None, **opts):
-        """FastChemistryModel::solve(deltaT) over a Field (species-major Y)."""
+        """FastChemistryModel::solve(deltaT) over a Field (species-major Y);
+        an array deltaT selects the local-time-step solve(scalarField)."""
         o = make_opts(**opts)
         nc = field.n_cells
         T0 = np.ascontiguousarray(field.T, np.float64)
@@ -141,9 +144,16 @@ class Oracle:
         st = np.zeros(nc, np.int32)
         Tend = np.zeros(nc)
         na = np.zeros(nc, np.int32)
-        rc = lib().orc_solve(self._h, C.byref(o), nc, float(deltaT), dptr(T0), dptr(p0),
-                             dptr(rho0), dptr(Y0), dptr(dtc), dptr(RR), dptr(dmin),
-                             iptr(st), dptr(Tend), iptr(na), int(threads))
+        if np.ndim(deltaT) == 0:
+            rc = lib().orc_solve(self._h, C.byref(o), nc, float(deltaT), dptr(T0), dptr(p0),
+                                 dptr(rho0), dptr(Y0), dptr(dtc), dptr(RR), dptr(dmin),
+                                 iptr(st), dptr(Tend), iptr(na), int(threads))
+        else:
+            dtf = np.ascontiguousarray(deltaT, np.float64)
+            assert dtf.shape == (nc,)
+            rc = lib().orc_solve_lts(self._h, C.byref(o), nc, dptr(dtf), dptr(T0), dptr(p0),
+                                     dptr(rho0), dptr(Y0), dptr(dtc), dptr(RR), dptr(dmin),
+                                     iptr(st), dptr(Tend), iptr(na), int(threads))
         return dict(rc=rc, RR=RR, deltaTChem=dtc, deltaTMin=float(dmin[0]), status=st,
                     Tend=Tend, nAttempts=na, rho0=rho0)
 

@@ -268,3 +268,14 @@ def test_empty_batch(fcslib, mech0):
     field = M.config_field(0, mech0, n_cells=8).subset(np.arange(0))
     g = s.solve(field, 1e-5)
     assert g["rc"] == 0 and g["deltaTMin"] == 2e-5
+
+
+def test_solve_lts_per_cell_deltaT(fcslib, mech0):
+    """Local-time-step solve(scalarField deltaT): per-cell flow steps, RR
+    formed with the cell's own deltaT, deltaTMin without the 2*deltaT cap."""
+    field = M.config_field(0, mech0, n_cells=256)
+    dt = np.random.default_rng(5).uniform(2e-6, 2e-5, 256)
+    g = fcslib.Solver(fcslib.Mech(mech0)).solve(field, dt)
+    o = Oracle(mech0).solve(field, dt, threads=1)
+    assert g["rc"] == 0 and o["rc"] == 0
+    assert_solve_parity(g, o, field, dt)
