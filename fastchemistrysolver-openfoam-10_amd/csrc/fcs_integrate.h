@@ -167,6 +167,20 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
     return error_norm<NP>(o, y0, yT, err);
 }
 
+// Step-size proposal after an accepted step of size dx with error norm Err
+// (src/ODE/OptRodas34/OptRodas34.C:276-285).
+__device__ __forceinline__ double accepted_dxTry(double Err, double dx)
+{
+    const double ratio = pow(MAX_SCALE / SAFE, -1.0 / ALPHA_INC);
+    if (Err > ratio) {
+        double f = SAFE * pow(Err, -ALPHA_INC);
+        f = f > MIN_SCALE ? f : MIN_SCALE;
+        f = f < MAX_SCALE ? f : MAX_SCALE;
+        return f * dx;
+    }
+    return SAFE * MAX_SCALE * dx;
+}
+
 // Step-controller state of one cell between launches (flags bits).
 enum { STEP_LAST = 1, STEP_RETRY = 2 };
 // Return value of a step that leaves the cell in flight.
@@ -180,12 +194,14 @@ constexpr int STEP_CONTINUE = -1;
 // exactly one attempt; it returns STEP_CONTINUE while the cell is in
 // flight, else its final status (0 when xEnd is reached).  Rejections are
 // capped at 100 * maxSteps attempts in total, a guard the reference does not
-// have (its rejection loop is unbounded).
+// have (its rejection loop is unbounded).  dxAcc receives the step size
+// when this call accepts a step (0 otherwise).
 template <int NP>
 __device__ int rosenbrock_step(const DevMech& M, const Lds& L, const DevOpts& o, const Tableau& tb, double p,
                                double xEnd, double& y0, double& x, double& dxTry, double& dx, double& dxTry0,
-                               int& flags, int& nStep, int& nAtt)
+                               int& flags, int& nStep, int& nAtt, double& dxAcc)
 {
+    dxAcc = 0.0;
     const double xStart = 0.0;
     if (!(flags & STEP_RETRY)) {
         dxTry0 = dxTry;
@@ -209,15 +225,8 @@ __device__ int rosenbrock_step(const DevMech& M, const Lds& L, const DevOpts& o,
     flags &= ~STEP_RETRY;
     x += dx;
     y0 = yT;
-    const double ratio = pow(MAX_SCALE / SAFE, -1.0 / ALPHA_INC);
-    if (Err > ratio) {
-        double f = SAFE * pow(Err, -ALPHA_INC);
-        f = f > MIN_SCALE ? f : MIN_SCALE;
-        f = f < MAX_SCALE ? f : MAX_SCALE;
-        dxTry = f * dx;
-    } else {
-        dxTry = SAFE * MAX_SCALE * dx;
-    }
+    dxAcc = dx;
+    dxTry = accepted_dxTry(Err, dx);
     if ((x - xEnd) * (xEnd - xStart) >= 0) {
         if (nStep > 0 && (flags & STEP_LAST)) dxTry = dxTry0;
         return 0;

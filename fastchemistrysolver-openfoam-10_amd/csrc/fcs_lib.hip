@@ -11,6 +11,7 @@
 
 #include "fcs.h"
 #include "fcs_integrate.h"
+#include "fcs_mlp_internal.h"
 
 using namespace fcs;
 
@@ -38,14 +39,16 @@ struct CellArrays {
     // kTarg_ [nc], and the per-cell Jacobian [nc][NP*64] kept across steps
     double *Y0s, *Scl, *theta, *Jy;
     int* kTarg;
+    double* dxFirst;                 // first accepted sub-step of the last run (0: none yet)
 };
 
 // Clears the active-list counters and the deltaTMin accumulator (a kernel
 // rather than memset nodes, so the sequence can be captured in a graph).
-__global__ void k_reset(unsigned int* counts, unsigned long long* dtMinBits)
+__global__ void k_reset(unsigned int* counts, unsigned long long* dtMinBits, unsigned long long* nAccepted)
 {
     counts[0] = counts[1] = counts[2] = 0u;
     *dtMinBits = 0x7ff0000000000000ull;  // +inf
+    nAccepted[0] = nAccepted[1] = 0ull;   // accepted, verified
 }
 
 // One thread per cell: cold cells are finished at once
@@ -76,6 +79,7 @@ __global__ __launch_bounds__(256) void k_setup(DevOpts o, int ns, CellArrays a, 
     a.dx[c] = 0.0;
     a.dxTry0[c] = 0.0;
     a.flags[c] = o.solver == SOLVER_SEULEX ? SX_FIRST : 0;
+    a.dxFirst[c] = 0.0;
     a.nStep[c] = 0;
     a.nAtt[c] = 0;
     if (o.solver == SOLVER_SEULEX) {
@@ -138,8 +142,11 @@ __global__ __launch_bounds__(64) void k_step(DevMech M, DevOpts o, CellArrays a,
         double x = a.x[c], dxTry = a.dxTry[c], dx = a.dx[c], dxTry0 = a.dxTry0[c];
         int flags = a.flags[c], nStep = a.nStep[c], nAtt = a.nAtt[c];
         const double xEnd = a.dtCell ? a.dtCell[c] : a.dt;
+        const int nStep0 = nStep;
+        double dxAcc = 0.0;
         const int st = rosenbrock_step<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, a.p0[c], xEnd, y,
-                                           x, dxTry, dx, dxTry0, flags, nStep, nAtt);
+                                           x, dxTry, dx, dxTry0, flags, nStep, nAtt, dxAcc);
+        if (lane == 0 && nStep0 == 0 && dxAcc > 0.0 && a.dxFirst[c] == 0.0) a.dxFirst[c] = dxAcc;
         if (st == STEP_CONTINUE) {
             if (row >= 0) a.Yw[row * nc + c] = y;
             if (lane == 0) {
@@ -211,6 +218,85 @@ __global__ __launch_bounds__(64) void k_step_seulex(DevMech M, DevOpts o, CellAr
             }
         } else {
             finish_cell<NP>(M, o, a, c, st, y, sc.dxTry, sc.nAtt);
+        }
+    }
+}
+
+// Network inputs of the cells of a list, one thread per entry: row i =
+// [T, log p, Y_0 .. Y_ns-1, log deltaT] in raw units (the network holds its
+// own normalisation).  Y is the clipped initial state k_setup prepared.
+__global__ void k_features(int ns, CellArrays a, const int* list, const unsigned int* count, float* X)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const int c = list[i];
+    const size_t nc = (size_t)a.nc;
+    float* x = X + (size_t)i * (ns + 3);
+    x[0] = (float)a.Yw[(size_t)ns * nc + c];
+    x[1] = (float)log(a.p0[c]);
+    for (int s = 0; s < ns; ++s) x[2 + s] = (float)a.Yw[(size_t)s * nc + c];
+    x[ns + 2] = (float)log(a.dtCell ? a.dtCell[c] : a.dt);
+}
+
+// Verification of the network's proposals, one wavefront per cell of the
+// list.  Row i of P holds [ln h, z_0 .. z_ns-1, z_T]: h is the proposed
+// first sub-step of an integration that starts by trying the whole flow
+// step, and the increments are tolerance-scaled and compressed,
+// d_k = sinh(z_k) (absTol + relTol |y0_k|).  Only cells whose proposed h is
+// at least half the flow step are verified: the proposed end state y0 + d
+// is accepted when one FP64 Rodas34 step over the whole flow step from y0
+// passes its own error test (norm <= 1) and agrees with the proposal to the
+// same tolerance (max-norm of |yT - yP| / (absTol + relTol max(|yT|, |yP|))
+// <= 1); the cell then finishes with the Rodas34 result and the step
+// proposal of an accepted step.  Every other cell joins the output list
+// unchanged, so the integrator then runs on it exactly as without the
+// network -- or, with seed != 0, starting from min(h, deltaT) instead of
+// deltaTChem.
+template <int NP>
+__global__ __launch_bounds__(64) void k_verify(DevMech M, DevOpts o, CellArrays a, const int* listIn,
+                                               const unsigned int* countIn, const float* P, int seed,
+                                               int* listOut, unsigned int* countOut,
+                                               unsigned long long* nAccepted)
+{
+    extern __shared__ double lds_raw[];
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const int lane = lane_id();
+    const int ns = M.ns;
+    const size_t nc = (size_t)a.nc;
+    const int row = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
+    const unsigned int n = *countIn;
+    for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int c = listIn[i];
+        const float* pr = P + (size_t)i * (ns + 2);
+        const double dt = a.dtCell ? a.dtCell[c] : a.dt;
+        double y0 = row >= 0 ? a.Yw[row * nc + c] : 0.0;
+        const double h = exp((double)pr[0]);
+        bool accept = false;
+        if (h >= 0.5 * dt) {   // uniform: one value per cell
+            if (lane == 0) atomicAdd(nAccepted + 1, 1ull);
+            const double yP = row >= 0 ? y0 + sinh((double)pr[1 + row]) * (o.absTol + o.relTol * fabs(y0)) : 0.0;
+            double yT = 0.0;
+            const double p = a.p0[c];
+            const double err = rosenbrock_attempt<NP>(M, L, o, kRodas34, p, log(p), dt, y0, yT);
+            double r = 0.0;
+            if (row >= 0) {
+                const double m = fabs(yT) > fabs(yP) ? fabs(yT) : fabs(yP);
+                r = fabs(yT - yP) / (o.absTol + o.relTol * m);
+            }
+            const bool bad = wave_any(!(r == r));
+            const double agree = wave_max(r);
+            accept = (err <= 1.0) && !bad && (agree <= 1.0);
+            if (accept) {
+                finish_cell<NP>(M, o, a, c, FCS_CELL_LEARNED, yT, accepted_dxTry(err, dt), 1);
+                if (lane == 0) atomicAdd(nAccepted, 1ull);
+            }
+        }
+        if (!accept && lane == 0) {
+            if (seed) {
+                const double hs = h < dt ? h : dt;
+                a.dxTry[c] = (hs > 0.0 && hs == hs) ? hs : a.dxTry[c];
+            }
+            listOut[atomicAdd(countOut, 1u)] = c;
         }
     }
 }
@@ -301,6 +387,7 @@ namespace {
 struct KernelSet {
     int NP;
     const void* step[3];        // Rodas34, Rosenbrock34, Seulex
+    const void* verify;
     const void* deriv;
     const void* jac;
     const void* lu;
@@ -311,6 +398,7 @@ struct KernelSet {
     {N,                                                                                                    \
      {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>,                        \
       (const void*)k_step_seulex<N>},                                                                      \
+     (const void*)k_verify<N>,                                                                             \
      (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>,                \
      {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>}},
 const KernelSet kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET)};
@@ -365,7 +453,7 @@ struct DevCells {
     int ns = 0;
     void* base = nullptr;
     double *T0 = nullptr, *p0 = nullptr, *rho0 = nullptr, *Y0 = nullptr, *dtcIn = nullptr, *dtcOut = nullptr,
-           *dtField = nullptr,
+           *dtField = nullptr, *dxFirst = nullptr,
            *RR = nullptr, *Tend = nullptr, *Yw = nullptr, *x = nullptr, *dxTry = nullptr, *dx = nullptr,
            *dxTry0 = nullptr, *Y0s = nullptr, *Scl = nullptr, *theta = nullptr, *Jy = nullptr;
     int *status = nullptr, *nAtt = nullptr, *flags = nullptr, *nStep = nullptr, *list0 = nullptr, *list1 = nullptr,
@@ -373,6 +461,10 @@ struct DevCells {
     size_t jyPerCell = 0;                    // doubles of Jy per cell (Seulex), 0 otherwise
     unsigned int* counts = nullptr;          // [3] active-list lengths
     unsigned long long* dtMin = nullptr;
+    unsigned long long* nAccepted = nullptr; // learned proposals accepted in the last run
+    float *Xf = nullptr, *Pf = nullptr;      // network inputs / outputs of the reacting cells
+    size_t fcap = 0;
+    long long lastPredicted = 0;
     unsigned int* hostCount = nullptr;       // pinned [kPoll]
     hipEvent_t ev[kPoll] = {};
     // optional per-launch timing of the step kernel (event pair per launch)
@@ -389,7 +481,7 @@ struct DevCells {
         const size_t d = align256(nc * 8), i = align256(nc * 4);
         const size_t nsd = align256(nc * nsp * 8), nwd = align256(nc * (nsp + 1) * 8);
         const size_t jyb = align256(nc * jyDoubles * 8);
-        const size_t total = 12 * d + 2 * nsd + 3 * nwd + 7 * i + jyb + 256 + 256;
+        const size_t total = 13 * d + 2 * nsd + 3 * nwd + 7 * i + jyb + 3 * 256;
         if (hipMalloc(&base, total) != hipSuccess) {
             base = nullptr;
             return FCS_E_NOMEM;
@@ -399,6 +491,7 @@ struct DevCells {
         T0 = (double*)take(d); p0 = (double*)take(d); rho0 = (double*)take(d); dtcIn = (double*)take(d);
         dtcOut = (double*)take(d); Tend = (double*)take(d); x = (double*)take(d); dxTry = (double*)take(d);
         dx = (double*)take(d); dxTry0 = (double*)take(d); theta = (double*)take(d); dtField = (double*)take(d);
+        dxFirst = (double*)take(d);
         Y0 = (double*)take(nsd); RR = (double*)take(nsd); Yw = (double*)take(nwd);
         Y0s = (double*)take(nwd); Scl = (double*)take(nwd);
         status = (int*)take(i); nAtt = (int*)take(i); flags = (int*)take(i); nStep = (int*)take(i);
@@ -407,6 +500,7 @@ struct DevCells {
         jyPerCell = jyDoubles;
         counts = (unsigned int*)take(256);
         dtMin = (unsigned long long*)take(256);
+        nAccepted = (unsigned long long*)take(256);
         if (hipHostMalloc((void**)&hostCount, kPoll * sizeof(unsigned int), hipHostMallocDefault) != hipSuccess) {
             hostCount = nullptr;
             release();
@@ -422,8 +516,22 @@ struct DevCells {
         ns = nsp;
         return FCS_OK;
     }
+    int ensure_features(size_t rows, int nsp)
+    {
+        if (rows <= fcap) return FCS_OK;
+        if (Xf) (void)hipFree(Xf), Xf = nullptr;
+        if (Pf) (void)hipFree(Pf), Pf = nullptr;
+        fcap = 0;
+        if (hipMalloc(&Xf, rows * (nsp + 3) * 4) != hipSuccess || hipMalloc(&Pf, rows * (nsp + 2) * 4) != hipSuccess)
+            return FCS_E_NOMEM;
+        fcap = rows;
+        return FCS_OK;
+    }
     void release()
     {
+        if (Xf) (void)hipFree(Xf), Xf = nullptr;
+        if (Pf) (void)hipFree(Pf), Pf = nullptr;
+        fcap = 0;
         if (base) (void)hipFree(base);
         if (hostCount) (void)hipHostFree(hostCount);
         for (auto& e : ev)
@@ -443,7 +551,7 @@ struct DevCells {
         a.T0 = T0; a.p0 = p0; a.rho0 = rho0; a.Y0 = Y0; a.dtcIn = dtcIn;
         a.dtcOut = dtcOut; a.RR = RR; a.Tend = Tend; a.status = status; a.nAtt = nAtt; a.dtMinBits = dtMin;
         a.Yw = Yw; a.x = x; a.dxTry = dxTry; a.dx = dx; a.dxTry0 = dxTry0; a.flags = flags; a.nStep = nStep;
-        a.Y0s = Y0s; a.Scl = Scl; a.theta = theta; a.Jy = Jy; a.kTarg = kTarg;
+        a.Y0s = Y0s; a.Scl = Scl; a.theta = theta; a.Jy = Jy; a.kTarg = kTarg; a.dxFirst = dxFirst;
         return a;
     }
 };
@@ -462,6 +570,8 @@ struct fcs_batch {
     fcs_solver* s = nullptr;
     int64_t nc = 0;
     DevCells cells;
+    fcs_mlp* mlp = nullptr;   // learned seed (fcs_batch_set_learned)
+    int learnedMode = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double lastDt = 0.0;
 };
@@ -823,7 +933,8 @@ static DevOpts dev_opts(const fcs_solver_opts& o)
 // launch kPoll-1 places back, so the GPU always has step launches queued.
 // Every cell finishes within 101 * maxSteps attempts (maxSteps accepted
 // steps plus the rejection cap), which bounds the launch count.
-static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = false)
+static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = false, fcs_mlp* mlp = nullptr,
+                     int learnedMode = 0)
 {
     hipStream_t st = s->stream;
     CellArrays a = d.view(nc, dt, lts);
@@ -831,7 +942,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
     DevOpts o = dev_opts(s->opts);
     int ns = s->mech->ns;
     {
-        void* args[] = {&d.counts, &d.dtMin};
+        void* args[] = {&d.counts, &d.dtMin, &d.nAccepted};
         HIP_OK(hipLaunchKernel((const void*)k_reset, dim3(1), dim3(1), args, 0, st));
     }
     if (nc > 0) {
@@ -841,8 +952,37 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
     }
     const void* f = s->ks->step[s->opts.solver];
     const int grid = std::max(1, std::min(s->grid, nc));
+    // learned seed: network proposals for the reacting cells, verified by one
+    // Rodas34 step each; the rejected ones form list 1 and the step loop
+    // starts there
+    long long it0 = 0;
+    d.lastPredicted = 0;
+    if (mlp && learnedMode && nc > 0) {
+        HIP_OK(hipMemcpyAsync(d.hostCount, d.counts, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        const unsigned int n0 = d.hostCount[0];
+        d.lastPredicted = n0;
+        if (n0 > 0) {
+            if (d.ensure_features(n0, ns) != FCS_OK) return FCS_E_NOMEM;
+            {
+                unsigned int* c0 = d.counts;
+                void* args[] = {&ns, &a, &d.list0, &c0, &d.Xf};
+                HIP_OK(hipLaunchKernel((const void*)k_features, dim3((n0 + 255) / 256), dim3(256), args, 0, st));
+            }
+            const int rc = fcs_mlp_forward_device(mlp, (int)n0, ns + 3, ns + 2, d.Xf, d.Pf, st);
+            if (rc != FCS_OK) return rc;
+            int seed = learnedMode == 2 ? 1 : 0;
+            unsigned int* c0 = d.counts;
+            unsigned int* c1 = d.counts + 1;
+            void* args[] = {&M, &o, &a, &d.list0, &c0, &d.Pf, &seed, &d.list1, &c1, &d.nAccepted};
+            HIP_OK(hipLaunchKernel(s->ks->verify, dim3(std::max(1, std::min(s->grid, (int)n0))), dim3(64), args,
+                                   s->ldsBytes, st));
+            it0 = 1;
+        }
+    }
     const long long maxLaunches = 101LL * s->opts.maxSteps + 2 * DevCells::kPoll + 2;
-    for (long long it = 0;; ++it) {
+    for (long long k = 0;; ++k) {
+        const long long it = k + it0;
         int* lin = (it & 1) ? d.list1 : d.list0;
         int* lout = (it & 1) ? d.list0 : d.list1;
         unsigned int* cin = d.counts + it % 3;
@@ -850,26 +990,26 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
         unsigned int* czero = d.counts + (it + 2) % 3;
         void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
         if (d.profile) {
-            while (d.pev.size() < 2 * (size_t)(it + 1)) {
+            while (d.pev.size() < 2 * (size_t)(k + 1)) {
                 hipEvent_t e;
                 HIP_OK(hipEventCreate(&e));
                 d.pev.push_back(e);
             }
-            HIP_OK(hipEventRecord(d.pev[2 * it], st));
+            HIP_OK(hipEventRecord(d.pev[2 * k], st));
         }
         HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st));
-        if (d.profile) HIP_OK(hipEventRecord(d.pev[2 * it + 1], st));
-        d.lastLaunches = it + 1;
-        const int slot = (int)(it % DevCells::kPoll);
+        if (d.profile) HIP_OK(hipEventRecord(d.pev[2 * k + 1], st));
+        d.lastLaunches = k + 1;
+        const int slot = (int)(k % DevCells::kPoll);
         HIP_OK(hipMemcpyAsync(d.hostCount + slot, cout, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIP_OK(hipEventRecord(d.ev[slot], st));
-        const long long back = it - (DevCells::kPoll - 1);
+        const long long back = k - (DevCells::kPoll - 1);
         if (back >= 0) {
             const int bs = (int)(back % DevCells::kPoll);
             HIP_OK(hipEventSynchronize(d.ev[bs]));
             if (d.hostCount[bs] == 0u) break;
         }
-        if (it > maxLaunches) return FCS_E_HIP;
+        if (k > maxLaunches) return FCS_E_HIP;
     }
     d.lastStepMs = 0.0;
     if (d.profile) {
@@ -1011,11 +1151,49 @@ int fcs_batch_run(fcs_batch* b, double deltaT, float* elapsed_ms)
     fcs_solver* s = b->s;
     b->lastDt = deltaT;
     HIP_OK(hipEventRecord(b->e0, s->stream));
-    int rc = run_cells(s, b->cells, (int)b->nc, deltaT);
+    int rc = run_cells(s, b->cells, (int)b->nc, deltaT, false, b->mlp, b->learnedMode);
     if (rc != FCS_OK) return rc;
     HIP_OK(hipEventRecord(b->e1, s->stream));
     HIP_OK(hipEventSynchronize(b->e1));
     if (elapsed_ms) HIP_OK(hipEventElapsedTime(elapsed_ms, b->e0, b->e1));
+    return FCS_OK;
+}
+
+int fcs_batch_set_learned(fcs_batch* b, fcs_mlp* mlp, int32_t mode)
+{
+    if (!b || mode < FCS_LEARNED_OFF || mode > FCS_LEARNED_SEED || (mode != FCS_LEARNED_OFF && !mlp)) return FCS_E_ARG;
+    if (mode != FCS_LEARNED_OFF) {
+        if (b->s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
+        const int ns = b->s->mech->ns;
+        // width check: a one-row forward pass on a scratch input
+        float* x = nullptr;
+        if (hipMalloc(&x, (size_t)(2 * ns + 5) * 4) != hipSuccess) return FCS_E_NOMEM;
+        HIP_OK(hipMemset(x, 0, (size_t)(2 * ns + 5) * 4));
+        const int rc = fcs_mlp_forward_device(mlp, 1, ns + 3, ns + 2, x, x + ns + 3, b->s->stream);
+        (void)hipStreamSynchronize(b->s->stream);
+        (void)hipFree(x);
+        if (rc != FCS_OK) return rc;
+    }
+    b->mlp = mode ? mlp : nullptr;
+    b->learnedMode = mode;
+    return FCS_OK;
+}
+
+int fcs_batch_learned_stats(fcs_batch* b, int64_t* predicted, int64_t* verified, int64_t* accepted)
+{
+    if (!b) return FCS_E_ARG;
+    unsigned long long acc[2] = {0, 0};
+    HIP_OK(hipMemcpy(acc, b->cells.nAccepted, 16, hipMemcpyDeviceToHost));
+    if (predicted) *predicted = b->cells.lastPredicted;
+    if (verified) *verified = b->learnedMode ? (int64_t)acc[1] : 0;
+    if (accepted) *accepted = b->learnedMode ? (int64_t)acc[0] : 0;
+    return FCS_OK;
+}
+
+int fcs_batch_first_steps(fcs_batch* b, double* dxFirst)
+{
+    if (!b || !dxFirst) return FCS_E_ARG;
+    HIP_OK(hipMemcpy(dxFirst, b->cells.dxFirst, (size_t)b->nc * 8, hipMemcpyDeviceToHost));
     return FCS_OK;
 }
 

@@ -8,7 +8,7 @@ import sys
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(PKG_ROOT, "libfcs.so")
-SRC = os.path.join(PKG_ROOT, "csrc", "fcs_lib.hip")
+SRCS = [os.path.join(PKG_ROOT, "csrc", f) for f in ("fcs_lib.hip", "fcs_mlp.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
@@ -19,9 +19,9 @@ HIP_FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-    """Compile csrc/fcs_lib.hip into libfcs.so (always recompiles)."""
+    """Compile csrc/*.hip into libfcs.so (always recompiles)."""
     cmd = [HIPCC, *HIP_FLAGS, "-I" + os.path.join(REPO, "include"),
-           "-I" + os.path.join(PKG_ROOT, "csrc"), "-o", LIB + ".tmp", SRC]
+           "-I" + os.path.join(PKG_ROOT, "csrc"), "-o", LIB + ".tmp", *SRCS]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

@@ -23,6 +23,9 @@ EXPORTS = [
     "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_solve_lts", "fcs_batch_create", "fcs_batch_destroy",
     "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_download",
     "fcs_eval_derivatives", "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
+    "fcs_batch_set_learned", "fcs_batch_learned_stats", "fcs_batch_first_steps",
+    "fcs_mlp_create", "fcs_mlp_destroy", "fcs_mlp_param_count", "fcs_mlp_get_params", "fcs_mlp_set_params",
+    "fcs_mlp_set_norm", "fcs_mlp_forward", "fcs_mlp_loss_grad", "fcs_mlp_train",
 ]
 
 
@@ -68,10 +71,24 @@ def load(path: str | None = None):
     L.fcs_eval_jacobian.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp]
     L.fcs_lu_solve.argtypes = [_vp, C.c_int64, _dp, _dp]
     L.fcs_eval_attempt.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp]
+    _fp = C.POINTER(C.c_float)
+    L.fcs_batch_set_learned.argtypes = [_vp, _vp, C.c_int32]
+    L.fcs_batch_learned_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.fcs_batch_first_steps.argtypes = [_vp, _dp]
+    L.fcs_mlp_create.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.c_uint64, C.POINTER(_vp)]
+    L.fcs_mlp_destroy.argtypes = [_vp]
+    L.fcs_mlp_param_count.argtypes = [_vp]
+    L.fcs_mlp_get_params.argtypes = [_vp, _fp]
+    L.fcs_mlp_set_params.argtypes = [_vp, _fp]
+    L.fcs_mlp_set_norm.argtypes = [_vp, _fp, _fp, _fp, _fp]
+    L.fcs_mlp_forward.argtypes = [_vp, C.c_int64, _fp, _fp]
+    L.fcs_mlp_loss_grad.argtypes = [_vp, C.c_int64, _fp, _fp, _fp, _fp]
+    L.fcs_mlp_train.argtypes = [_vp, C.c_int64, _fp, _fp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]
     for name in EXPORTS:
         getattr(L, name)
         if name != "fcs_version":
             getattr(L, name).restype = C.c_int
+    L.fcs_mlp_param_count.restype = C.c_int64
     _lib = L
     return L
 
@@ -220,6 +237,27 @@ class Batch:
         check(load().fcs_batch_stats(self._h, C.byref(n), C.byref(ms)), "fcs_batch_stats")
         return n.value, ms.value
 
+    def set_learned(self, mlp, mode="verify"):
+        """Learned seed on later runs: mode "off", "verify" (fallback cells
+        integrate exactly as without the network) or "seed" (fallback cells
+        start from the network's first sub-step)."""
+        m = {"off": 0, "verify": 1, "seed": 2}[mode]
+        check(load().fcs_batch_set_learned(self._h, None if mlp is None else mlp._h, m), "fcs_batch_set_learned")
+        self._mlp = mlp     # the batch keeps the network's pointer
+
+    def learned_stats(self):
+        """(cells given a proposal, proposals verified, proposals accepted)
+        of the last run."""
+        a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        check(load().fcs_batch_learned_stats(self._h, C.byref(a), C.byref(b), C.byref(c)),
+              "fcs_batch_learned_stats")
+        return a.value, b.value, c.value
+
+    def first_steps(self):
+        out = np.zeros(self.nc)
+        check(load().fcs_batch_first_steps(self._h, dptr(out)), "fcs_batch_first_steps")
+        return out
+
     def download(self):
         ns, nc = self.s.ns, self.nc
         RR = np.zeros((ns, nc))
@@ -236,6 +274,68 @@ class Batch:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.fcs_batch_destroy(self._h)
             self._h = None
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The learned seed's network on the GPU (fcs_mlp_*): BF16 MFMA forward
+    and backward kernels, FP32 master weights, Adam."""
+
+    def __init__(self, widths, seed=0):
+        self.widths = [int(w) for w in widths]
+        wa = (C.c_int32 * len(self.widths))(*self.widths)
+        h = _vp()
+        check(load().fcs_mlp_create(len(self.widths) - 1, wa, int(seed), C.byref(h)), "fcs_mlp_create")
+        self._h = h
+        self.n_params = int(load().fcs_mlp_param_count(h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fcs_mlp_destroy(self._h)
+            self._h = None
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        check(load().fcs_mlp_get_params(self._h, _fptr(p)), "fcs_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.shape == (self.n_params,)
+        check(load().fcs_mlp_set_params(self._h, _fptr(p)), "fcs_mlp_set_params")
+
+    def set_norm(self, in_mean, in_std, out_mean, out_std):
+        a, b, c, d = (_f32(x) for x in (in_mean, in_std, out_mean, out_std))
+        self.norm = (a, b, c, d)
+        check(load().fcs_mlp_set_norm(self._h, _fptr(a), _fptr(b), _fptr(c), _fptr(d)), "fcs_mlp_set_norm")
+
+    def forward(self, X):
+        X = _f32(X)
+        Y = np.zeros((X.shape[0], self.widths[-1]), np.float32)
+        check(load().fcs_mlp_forward(self._h, X.shape[0], _fptr(X), _fptr(Y)), "fcs_mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X, T = _f32(X), _f32(T)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        check(load().fcs_mlp_loss_grad(self._h, X.shape[0], _fptr(X), _fptr(T), _fptr(loss), _fptr(g)),
+              "fcs_mlp_loss_grad")
+        return float(loss[0]), g
+
+    def train(self, X, T, epochs, batch=256, lr=1e-3, seed=0):
+        X, T = _f32(X), _f32(T)
+        hist = np.zeros(epochs, np.float32)
+        check(load().fcs_mlp_train(self._h, X.shape[0], _fptr(X), _fptr(T), int(epochs), int(batch), float(lr),
+                                   int(seed), _fptr(hist)), "fcs_mlp_train")
+        return hist
 
 
 def equilibrator(solver: "Solver", deltaT=4e-3):

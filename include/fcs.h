@@ -44,6 +44,7 @@ extern "C" {
 #define FCS_CELL_SKIPPED       1   /* T0 <= Treact: RR = 0, not integrated      */
 #define FCS_CELL_MAXSTEPS      2   /* maxSteps exceeded (reference: FatalError) */
 #define FCS_CELL_NONFINITE     4   /* NaN/Inf in the state or error norm        */
+#define FCS_CELL_LEARNED       8   /* finished by a verified learned proposal   */
 
 /* ---- reaction rate forms (fcs_mech_desc.rxn_type) --------------------- */
 #define FCS_RXN_ARRHENIUS   0   /* (ir)reversibleArrhenius                      */
@@ -202,6 +203,37 @@ int fcs_batch_download(fcs_batch* batch, double* RR /* [ns*nCells] */,
                        int32_t* cellStatus, double* Tend, int32_t* nAttempts);
 
 /*
+ * Learned seed on a batch (no counterpart in the reference).  The network
+ * (fcs_mlp) maps [T, ln p, Y_0..Y_ns-1, ln deltaT] of each reacting cell to
+ * [ln h, z_0..z_ns-1, z_T]: h is the first accepted sub-step of an
+ * integration that starts by trying the whole flow step, and the end-state
+ * increments are d_k = sinh(z_k) * (absTol + relTol*|y0_k|).  With mode
+ * FCS_LEARNED_VERIFY, cells with h >= deltaT/2 get their proposed end state
+ * verified: it is accepted only if one FP64 Rodas34 step over the whole
+ * flow step from the initial state passes relTol / absTol and agrees with
+ * the proposal to the same tolerance, and the cell then finishes with that
+ * Rodas34 result.  Every other cell runs the plain integrator from its
+ * original state and deltaTChem, making the same accept/reject decisions as
+ * without the network.  FCS_LEARNED_SEED starts those cells from min(h,
+ * deltaT) instead of deltaTChem.  FCS_E_ARG for other network widths,
+ * FCS_E_UNSUPPORTED for OptSeulex.  The batch keeps the network pointer:
+ * destroy the batch first.
+ */
+#define FCS_LEARNED_OFF     0
+#define FCS_LEARNED_VERIFY  1
+#define FCS_LEARNED_SEED    2
+typedef struct fcs_mlp fcs_mlp;
+int fcs_batch_set_learned(fcs_batch* batch, fcs_mlp* mlp, int32_t mode);
+/* cells given a proposal in the last run, proposals verified (h >= deltaT/2)
+ * and proposals accepted (cellStatus FCS_CELL_LEARNED) */
+int fcs_batch_learned_stats(fcs_batch* batch, int64_t* predicted, int64_t* verified, int64_t* accepted);
+/* first accepted Rosenbrock sub-step of each cell in the last run (0 when the
+ * cell took none: skipped, finished by a learned proposal, or OptSeulex);
+ * with deltaTChem set to deltaT on input, the training target h of the
+ * network's first output */
+int fcs_batch_first_steps(fcs_batch* batch, double* dxFirst /* [nCells] */);
+
+/*
  * Kernel-level entry points over host arrays (batch of independent states,
  * species-major SoA, n = n_species + 1).  They run the same device code the
  * integrator runs and exist for parity checks and profiling.
@@ -226,6 +258,35 @@ int fcs_eval_attempt(fcs_solver* solver, int64_t nStates, const double* p,
 /* LUsolver::Block4LUDecompose + xSolve (src/LUsolver/LUsolver.C:3058-3171):
  * A [nSys][n][n] row-major, b [nSys][n] -> x in b.  n = n_species + 1. */
 int fcs_lu_solve(fcs_solver* solver, int64_t nSys, const double* A, double* b);
+
+/*
+ * Learned seed of the integrator: a fully connected network with SiLU
+ * hidden layers, evaluated and trained on the GPU with BF16 MFMA tiles and
+ * FP32 accumulation (FP32 master weights, Adam).  No counterpart in the
+ * reference: it proposes each cell's first chemical sub-step and its end
+ * state, which the FP64 Rodas34 integrator verifies (fcs_solve_learned).
+ * widths[0..nLayers] are the layer widths (input first).  Parameters are
+ * FP32 in a flat padded layout (per layer W [pad16(in)][pad16(out)] then
+ * b [pad16(out)], padding zero); inputs and outputs are normalised with
+ * the statistics of fcs_mlp_set_norm (default mean 0, std 1).
+ */
+int fcs_mlp_create(int32_t nLayers, const int32_t* widths, uint64_t seed, fcs_mlp** out);
+int fcs_mlp_destroy(fcs_mlp* mlp);
+int64_t fcs_mlp_param_count(const fcs_mlp* mlp);
+int fcs_mlp_get_params(fcs_mlp* mlp, float* params);
+int fcs_mlp_set_params(fcs_mlp* mlp, const float* params);
+int fcs_mlp_set_norm(fcs_mlp* mlp, const float* inMean, const float* inStd,
+                     const float* outMean, const float* outStd);
+/* X [n][widths[0]] raw inputs -> Y [n][widths[nLayers]] raw outputs */
+int fcs_mlp_forward(fcs_mlp* mlp, int64_t n, const float* X, float* Y);
+/* Mean-squared error of the normalised outputs against targets T (raw
+ * units) and its gradient with respect to every parameter (grad may be NULL). */
+int fcs_mlp_loss_grad(fcs_mlp* mlp, int64_t n, const float* X, const float* T,
+                      float* loss, float* grad);
+/* Adam on shuffled mini-batches; lossOut[e] = mean loss of epoch e (may be NULL). */
+int fcs_mlp_train(fcs_mlp* mlp, int64_t n, const float* X, const float* T,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                  float* lossOut);
 
 #ifdef __cplusplus
 }
