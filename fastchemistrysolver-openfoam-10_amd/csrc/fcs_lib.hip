@@ -465,6 +465,11 @@ struct DevCells {
     float *Xf = nullptr, *Pf = nullptr;      // network inputs / outputs of the reacting cells
     size_t fcap = 0;
     long long lastPredicted = 0;
+    // launch shape (autotuned or set by the caller): grid of the step kernel
+    // (0: occupancy x CUs) and step launches per captured hipGraph (0: eager)
+    int gridOverride = 0, graphLaunches = 0;
+    hipGraphExec_t graphExec = nullptr;
+    struct GraphKey { double dt; int nc, grid, lts, solver; } gkey{0.0, 0, 0, 0, 0};
     unsigned int* hostCount = nullptr;       // pinned [kPoll]
     hipEvent_t ev[kPoll] = {};
     // optional per-launch timing of the step kernel (event pair per launch)
@@ -527,8 +532,14 @@ struct DevCells {
         fcap = rows;
         return FCS_OK;
     }
+    void drop_graph()
+    {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        graphExec = nullptr;
+    }
     void release()
     {
+        drop_graph();
         if (Xf) (void)hipFree(Xf), Xf = nullptr;
         if (Pf) (void)hipFree(Pf), Pf = nullptr;
         fcap = 0;
@@ -951,7 +962,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
         HIP_OK(hipLaunchKernel((const void*)k_setup, dim3((nc + 255) / 256), dim3(256), args, 0, st));
     }
     const void* f = s->ks->step[s->opts.solver];
-    const int grid = std::max(1, std::min(s->grid, nc));
+    const int grid = std::max(1, std::min(d.gridOverride > 0 ? d.gridOverride : s->grid, nc));
     // learned seed: network proposals for the reacting cells, verified by one
     // Rodas34 step each; the rejected ones form list 1 and the step loop
     // starts there
@@ -981,13 +992,94 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
         }
     }
     const long long maxLaunches = 101LL * s->opts.maxSteps + 2 * DevCells::kPoll + 2;
+    // step launch number `it` reads list it%2 and counter it%3, appends to the
+    // other list and counter (it+1)%3, and clears counter (it+2)%3
+    auto launch_args = [&](long long it, int*& lin, unsigned int*& cin, int*& lout, unsigned int*& cout,
+                           unsigned int*& czero) {
+        lin = (it & 1) ? d.list1 : d.list0;
+        lout = (it & 1) ? d.list0 : d.list1;
+        cin = d.counts + it % 3;
+        cout = d.counts + (it + 1) % 3;
+        czero = d.counts + (it + 2) % 3;
+    };
+    d.lastStepMs = 0.0;
+    if (d.graphLaunches > 0) {
+        // hipGraph replays of G step launches (G a multiple of 6, so every
+        // replay starts at the same list / counter phase); a few eager
+        // launches first bring the phase to 0.  Launches after the list has
+        // emptied return at once.  The host polls the last launch's counter
+        // of the replay one replay back.
+        const int G = d.graphLaunches;
+        long long k = 0;
+        for (; (k + it0) % 6 != 0; ++k) {
+            int *lin, *lout;
+            unsigned int *cin, *cout, *czero;
+            launch_args(k + it0, lin, cin, lout, cout, czero);
+            void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
+            HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st));
+        }
+        const DevCells::GraphKey key{dt, nc, grid, lts ? 1 : 0, s->opts.solver};
+        if (!d.graphExec || std::memcmp(&key, &d.gkey, sizeof key) != 0) {
+            d.drop_graph();
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamSynchronize(st));
+            HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int j = 0; j < G; ++j) {
+                int *lin, *lout;
+                unsigned int *cin, *cout, *czero;
+                launch_args(j, lin, cin, lout, cout, czero);
+                void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
+                if (hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st) != hipSuccess) {
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    return FCS_E_HIP;
+                }
+            }
+            HIP_OK(hipStreamEndCapture(st, &g));
+            const hipError_t e = hipGraphInstantiate(&d.graphExec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                d.graphExec = nullptr;
+                return FCS_E_HIP;
+            }
+            d.gkey = key;
+        }
+        unsigned int* lastOut = d.counts + (G - 1 + 1) % 3;   // output counter of launch G-1
+        const long long maxReplays = maxLaunches / G + 2;
+        while (d.pev.size() < 2) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            d.pev.push_back(e);
+        }
+        for (long long r = 0;; ++r) {
+            if (d.profile) HIP_OK(hipEventRecord(d.pev[0], st));
+            HIP_OK(hipGraphLaunch(d.graphExec, st));
+            if (d.profile) {
+                HIP_OK(hipEventRecord(d.pev[1], st));
+                HIP_OK(hipEventSynchronize(d.pev[1]));
+                float ms = 0.f;
+                HIP_OK(hipEventElapsedTime(&ms, d.pev[0], d.pev[1]));
+                d.lastStepMs += ms;
+            }
+            d.lastLaunches = k + (r + 1) * G;
+            const int slot = (int)(r % 2);
+            HIP_OK(hipMemcpyAsync(d.hostCount + slot, lastOut, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipEventRecord(d.ev[slot], st));
+            if (r >= 1) {
+                const int bs = (int)((r - 1) % 2);
+                HIP_OK(hipEventSynchronize(d.ev[bs]));
+                if (d.hostCount[bs] == 0u) break;
+            }
+            if (r > maxReplays) return FCS_E_HIP;
+        }
+        HIP_OK(hipStreamSynchronize(st));
+        return FCS_OK;
+    }
     for (long long k = 0;; ++k) {
         const long long it = k + it0;
-        int* lin = (it & 1) ? d.list1 : d.list0;
-        int* lout = (it & 1) ? d.list0 : d.list1;
-        unsigned int* cin = d.counts + it % 3;
-        unsigned int* cout = d.counts + (it + 1) % 3;
-        unsigned int* czero = d.counts + (it + 2) % 3;
+        int *lin, *lout;
+        unsigned int *cin, *cout, *czero;
+        launch_args(it, lin, cin, lout, cout, czero);
         void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
         if (d.profile) {
             while (d.pev.size() < 2 * (size_t)(k + 1)) {
@@ -1156,6 +1248,61 @@ int fcs_batch_run(fcs_batch* b, double deltaT, float* elapsed_ms)
     HIP_OK(hipEventRecord(b->e1, s->stream));
     HIP_OK(hipEventSynchronize(b->e1));
     if (elapsed_ms) HIP_OK(hipEventElapsedTime(elapsed_ms, b->e0, b->e1));
+    return FCS_OK;
+}
+
+int fcs_batch_set_launch(fcs_batch* b, int32_t grid, int32_t graphLaunches)
+{
+    if (!b || grid < 0 || graphLaunches < 0 || graphLaunches % 6 != 0 || graphLaunches > 6000) return FCS_E_ARG;
+    b->cells.gridOverride = grid;
+    b->cells.graphLaunches = graphLaunches;
+    b->cells.drop_graph();
+    return FCS_OK;
+}
+
+int fcs_batch_autotune(fcs_batch* b, double deltaT, int32_t* grid, int32_t* graphLaunches)
+{
+    if (!b || !(deltaT > 0)) return FCS_E_ARG;
+    fcs_solver* s = b->s;
+    int dev = 0, cus = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per = std::max(1, s->grid / std::max(cus, 1));
+    float best = 0.f;
+    int bestGrid = 0, bestGraph = 0;
+    const bool wasProfile = b->cells.profile;
+    b->cells.profile = false;
+    for (int g : {0, 24})
+        for (int k = 1; k <= per; ++k) {
+            const int gr = k * cus;
+            b->cells.gridOverride = gr;
+            b->cells.graphLaunches = g;
+            b->cells.drop_graph();
+            float ms = 0.f;
+            int rc = run_cells(s, b->cells, (int)b->nc, deltaT, false, b->mlp, b->learnedMode);   // warm-up / capture
+            if (rc == FCS_OK) {
+                HIP_OK(hipEventRecord(b->e0, s->stream));
+                rc = run_cells(s, b->cells, (int)b->nc, deltaT, false, b->mlp, b->learnedMode);
+            }
+            if (rc != FCS_OK) {
+                b->cells.profile = wasProfile;
+                return rc;
+            }
+            HIP_OK(hipEventRecord(b->e1, s->stream));
+            HIP_OK(hipEventSynchronize(b->e1));
+            HIP_OK(hipEventElapsedTime(&ms, b->e0, b->e1));
+            if (bestGrid == 0 || ms < best) {
+                best = ms;
+                bestGrid = gr;
+                bestGraph = g;
+            }
+        }
+    b->cells.profile = wasProfile;
+    b->cells.gridOverride = bestGrid;
+    b->cells.graphLaunches = bestGraph;
+    b->cells.drop_graph();
+    if (grid) *grid = bestGrid;
+    if (graphLaunches) *graphLaunches = bestGraph;
     return FCS_OK;
 }
 

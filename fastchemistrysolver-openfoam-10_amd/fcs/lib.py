@@ -23,7 +23,8 @@ EXPORTS = [
     "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_solve_lts", "fcs_batch_create", "fcs_batch_destroy",
     "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_download",
     "fcs_eval_derivatives", "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
-    "fcs_batch_set_learned", "fcs_batch_learned_stats", "fcs_batch_first_steps",
+    "fcs_batch_set_learned", "fcs_batch_learned_stats", "fcs_batch_first_steps", "fcs_batch_set_launch",
+    "fcs_batch_autotune",
     "fcs_mlp_create", "fcs_mlp_destroy", "fcs_mlp_param_count", "fcs_mlp_get_params", "fcs_mlp_set_params",
     "fcs_mlp_set_norm", "fcs_mlp_forward", "fcs_mlp_loss_grad", "fcs_mlp_train",
 ]
@@ -75,6 +76,8 @@ def load(path: str | None = None):
     L.fcs_batch_set_learned.argtypes = [_vp, _vp, C.c_int32]
     L.fcs_batch_learned_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.fcs_batch_first_steps.argtypes = [_vp, _dp]
+    L.fcs_batch_set_launch.argtypes = [_vp, C.c_int32, C.c_int32]
+    L.fcs_batch_autotune.argtypes = [_vp, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.fcs_mlp_create.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.c_uint64, C.POINTER(_vp)]
     L.fcs_mlp_destroy.argtypes = [_vp]
     L.fcs_mlp_param_count.argtypes = [_vp]
@@ -236,6 +239,18 @@ class Batch:
         n, ms = C.c_int64(0), C.c_double(0.0)
         check(load().fcs_batch_stats(self._h, C.byref(n), C.byref(ms)), "fcs_batch_stats")
         return n.value, ms.value
+
+    def set_launch(self, grid=0, graph_launches=0):
+        """Step-kernel grid (0: occupancy x CUs) and step launches per
+        captured hipGraph (0: eager launches)."""
+        check(load().fcs_batch_set_launch(self._h, int(grid), int(graph_launches)), "fcs_batch_set_launch")
+
+    def autotune(self, deltaT):
+        """Times the candidate launch shapes on the uploaded cells and keeps
+        the fastest; returns (grid, graph_launches)."""
+        g, n = C.c_int32(0), C.c_int32(0)
+        check(load().fcs_batch_autotune(self._h, float(deltaT), C.byref(g), C.byref(n)), "fcs_batch_autotune")
+        return g.value, n.value
 
     def set_learned(self, mlp, mode="verify"):
         """Learned seed on later runs: mode "off", "verify" (fallback cells

@@ -203,6 +203,21 @@ int fcs_batch_download(fcs_batch* batch, double* RR /* [ns*nCells] */,
                        int32_t* cellStatus, double* Tend, int32_t* nAttempts);
 
 /*
+ * Launch shape of a batch's step kernel (no counterpart in the reference:
+ * the device work lists replace the MPI load balancer).  grid: workgroups
+ * of the step kernel (0: occupancy x CUs, the default).  graphLaunches: 0
+ * launches every step kernel from the host; a positive multiple of 6
+ * (at most 6000) captures that many step launches in a hipGraph, replayed
+ * until every cell has finished (launches after the lists have emptied
+ * return at once; results are identical to eager launches).
+ * fcs_batch_autotune times every grid of 1..occupancy workgroups per CU
+ * with eager launches and with 24-launch graphs on the batch's uploaded
+ * cells (two runs each), keeps the fastest and returns it.
+ */
+int fcs_batch_set_launch(fcs_batch* batch, int32_t grid, int32_t graphLaunches);
+int fcs_batch_autotune(fcs_batch* batch, double deltaT, int32_t* grid, int32_t* graphLaunches);
+
+/*
  * Learned seed on a batch (no counterpart in the reference).  The network
  * (fcs_mlp) maps [T, ln p, Y_0..Y_ns-1, ln deltaT] of each reacting cell to
  * [ln h, z_0..z_ns-1, z_T]: h is the first accepted sub-step of an

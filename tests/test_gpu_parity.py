@@ -279,3 +279,38 @@ def test_solve_lts_per_cell_deltaT(fcslib, mech0):
     o = Oracle(mech0).solve(field, dt, threads=1)
     assert g["rc"] == 0 and o["rc"] == 0
     assert_solve_parity(g, o, field, dt)
+
+
+def test_graph_replays_match_eager_launches(fcslib, mech1):
+    """Step launches captured in a hipGraph: every replay (with eager work
+    and downloads in between) gives bit-identical results to eager
+    launches; a new deltaT re-captures."""
+    field = M.config_field(1, mech1, n_cells=2048)
+    s = fcslib.Solver(fcslib.Mech(mech1))
+    ref = {}
+    for dt in (1e-6, 5e-7):
+        b = fcslib.Batch(s, field)
+        b.run(dt)
+        ref[dt] = b.download()
+    g = fcslib.Batch(s, field)
+    g.set_launch(0, 12)
+    for dt in (1e-6, 1e-6, 5e-7, 1e-6):
+        g.run(dt)
+        r = g.download()
+        for k in ("RR", "deltaTChem", "Tend", "status", "nAttempts"):
+            np.testing.assert_array_equal(r[k], ref[dt][k])
+        assert r["deltaTMin"] == ref[dt]["deltaTMin"]
+
+
+def test_autotuned_launch_shape_keeps_results(fcslib, mech0):
+    field = M.config_field(0, mech0, n_cells=512)
+    s = fcslib.Solver(fcslib.Mech(mech0))
+    b = fcslib.Batch(s, field)
+    b.run(1e-5)
+    r0 = b.download()
+    grid, graph = b.autotune(1e-5)
+    assert grid > 0 and graph in (0, 24)
+    b.run(1e-5)
+    r1 = b.download()
+    for k in ("RR", "deltaTChem", "Tend", "status", "nAttempts"):
+        np.testing.assert_array_equal(r1[k], r0[k])
