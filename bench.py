@@ -2,14 +2,25 @@
 """Benchmark of the gfx950 per-cell chemistry integrator.
 
 One step = FastChemistryModel::solve(deltaT) over the whole batch of cells
-with inputs resident in HBM: plain OptRodas34, FP64, the 53-species /
-325-reaction synthetic mechanism, 65,536 cells on a 1-D mixture-fraction
-profile, deltaT = 1e-6 s, absTol 1e-8, relTol 1e-4, initial chemical
-time step 1e-7 s.
+with inputs resident in HBM (fcs_batch_run), FP64 Rodas34, the
+53-species / 325-reaction synthetic mechanism, deltaT = 1e-6 s,
+absTol 1e-8, relTol 1e-4, initial chemical time step 1e-7 s.
 
-Prints ONE JSON line (rank 0).  The CPU baseline is the oracle (a scalar C
-restatement of the reference algorithm) timed on a bounded, seeded sample
-of the same cells on one host core.
+Default workload (the metric's): learned-seeded Rodas34 on the 512 x 512
+counterflow field (262,144 cells: 45 % cold cells below Treact = 500 K,
+40 % near-equilibrium products, 15 % flame front), step launches replayed
+from a hipGraph.  The network is trained first (not timed) by the
+library's own BF16-MFMA kernels on samples the plain GPU integrator
+produces from two other seeded counterflow fields.  The same field is also
+run with the plain integrator (no network), and the 65,536-cell
+mixture-fraction field of the plain-integrator configuration is run too;
+their rates are reported beside `value`.  The error of the plain and
+learned runs is measured against a relTol 1e-10 / absTol 1e-14 solve of a
+stratified sample of cells.
+
+Prints ONE JSON line.  The CPU baseline is the oracle (a scalar C
+restatement of the reference algorithm) timed on a bounded, stratified
+sample of the same field on one host core.
 """
 from __future__ import annotations
 
@@ -26,28 +37,114 @@ sys.path.insert(0, os.path.join(ROOT, "fastchemistrysolver-openfoam-10_amd"))
 
 METRIC = "cells·dt/s at 53-species mech (learned-seeded vs plain GPU Rodas34, ≤1e-4 err)"
 FP64_PEAK_TFLOPS = 78.6      # MI355X dense FP64 (vector and matrix)
-HBM_PEAK_GBS = 8000.0
+DT = 1e-6
+TREACT = 500.0
+GRAPH_LAUNCHES = 24
 
 
-def cpu_baseline(mech, field, dt, seconds=15.0):
-    """Times the oracle on one core over a seeded random sample of cells,
-    growing the sample until ~`seconds` of CPU work."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(mech, field, seconds=15.0):
+    """Times the oracle on one core over a stratified random sample of the
+    field's classes, growing the sample until ~`seconds` of CPU work, and
+    converts the per-class cost into the whole field's cells·dt/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     o = Oracle(mech)
     rng = np.random.default_rng(7)
-    order = rng.permutation(field.n_cells)
-    n, used, cells = 8, 0.0, 0
-    while used < seconds and cells < field.n_cells:
-        idx = order[cells:cells + n]
-        t0 = time.perf_counter()
-        o.solve(field.subset(idx), dt, threads=1)
-        used += time.perf_counter() - t0
-        cells += len(idx)
-        n = min(2 * n, 256)
-    return dict(value=cells / used, unit="cells·dt/s", cores=1, kind="port",
-                sample=f"{cells} of {field.n_cells} cells drawn uniformly at random (seed 7), "
-                       f"{used:.1f} s on one core, plain OptRodas34 oracle (C, -O3)")
+    classes = np.unique(field.cls)
+    cost, count = {}, {}
+    per = {c: rng.permutation(np.nonzero(field.cls == c)[0]) for c in classes}
+    used, n = 0.0, 4
+    done = {c: 0 for c in classes}
+    while used < seconds and any(done[c] < per[c].size for c in classes):
+        for c in classes:
+            idx = per[c][done[c]:done[c] + n]
+            if idx.size == 0:
+                continue
+            t0 = time.perf_counter()
+            o.solve(field.subset(idx), DT, threads=1, Treact=TREACT)
+            dt = time.perf_counter() - t0
+            cost[c] = cost.get(c, 0.0) + dt
+            count[c] = count.get(c, 0) + idx.size
+            done[c] += idx.size
+            used += dt
+        n = min(2 * n, 64)
+    per_cell = {c: cost[c] / count[c] for c in cost}
+    total = sum(per_cell[c] * (field.cls == c).sum() for c in per_cell)
+    desc = ", ".join(f"class {int(c)}: {count[c]} cells" for c in sorted(count))
+    return dict(value=field.n_cells / total, unit="cells·dt/s", cores=1, kind="port",
+                sample=f"stratified random sample of the {field.n_cells}-cell counterflow field ({desc}; "
+                       f"0 cold, 1 product, 2 flame front), {used:.1f} s on one core, plain OptRodas34 "
+                       f"oracle (C, -O3); per-class cost scaled to the field's class counts")
+
+
+def timed_runs(batch, steps, warmup, profile):
+    for _ in range(warmup):
+        batch.run(DT, timed=True)
+    batch.profile(profile)
+    launches, step_ms = 0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(DT, timed=True)
+        n, ms = batch.stats()
+        launches += n
+        step_ms += ms
+    wall = time.perf_counter() - t0
+    batch.profile(False)
+    return wall, launches, step_ms
+
+
+def stratified(field, n_per, seed=3):
+    rng = np.random.default_rng(seed)
+    idx = [rng.choice(np.nonzero(field.cls == c)[0], n_per, replace=False) for c in (1, 2)]
+    return np.concatenate(idx)
+
+
+def attempt(mech, gm):
+    """Two launches of one Rodas34 attempt (dx = 1e-9 s) on the first 1,024
+    cells of the mixture-fraction field: one wavefront per cell, the unit of
+    work of the step kernel, for hardware-counter collection."""
+    from fcs import lib
+    from fcs.mechanism import config_field
+    fld = config_field(1, mech, n_cells=1024)
+    s = lib.Solver(gm, solver="OptRodas34")
+    YT = np.vstack([fld.Y, fld.T[None, :]])
+    t0 = time.perf_counter()
+    for _ in range(2):
+        err, _, _ = s.attempt(fld.p, YT, np.full(fld.n_cells, 1e-9))
+    print(json.dumps({"workload": "2 x one Rodas34 attempt on 1,024 cells", "seconds": time.perf_counter() - t0,
+                      "mean_error_norm": float(err.mean())}), flush=True)
+
+
+def mixture(args, mech, gm):
+    """Plain Rodas34 on the 65,536-cell 1-D mixture-fraction field, with the
+    step kernel's roofline from an event pair around every step launch."""
+    from fcs import lib
+    from fcs.flops import attempt_flops
+    from fcs.mechanism import config_field
+    f1 = config_field(1, mech)
+    s1 = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4)
+    b1 = lib.Batch(s1, f1)
+    wall, launches, step_ms = timed_runs(b1, args.steps, args.warmup, profile=True)
+    r = b1.download()
+    fpa = attempt_flops(mech, "OptRodas34")
+    att = int(r["nAttempts"].sum())
+    avg_launch_s = 1e-3 * step_ms / max(launches, 1)
+    achieved = fpa * att * args.steps / max(launches, 1) / avg_launch_s / 1e12
+    line = {"metric": METRIC, "value": f1.n_cells / (wall / args.steps), "unit": "cells·dt/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "plain OptRodas34, 65,536-cell 1-D mixture-fraction field, 53-species/"
+                                   "325-reaction synthetic mechanism, deltaT 1e-6 s", "cells": f1.n_cells,
+                       "attempts_per_step": att, "failed_cells": int(((r["status"] & 6) != 0).sum())},
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                         "kernel": "k_step<56,Rodas34>", "flops_per_attempt": fpa, "launches": launches,
+                         "avg_launch_ms": 1e3 * avg_launch_s}}
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -55,47 +152,107 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=65536)
-    ap.add_argument("--solver", default="OptRodas34")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 65,536-cell plain run")
+    ap.add_argument("--train-side", type=int, default=256, help="side of each training field")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt"), default="counterflow",
+                    help="counterflow: learned-seeded vs plain Rodas34 on the 262,144-cell field (default); "
+                         "mixture: plain Rodas34 on the 65,536-cell mixture-fraction field only; "
+                         "attempt: two single-attempt launches on 1,024 cells (for counter collection)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); cells are independent, "
               "so the path shards by cell ranges with no collective on the data path.", file=sys.stderr)
         sys.exit(2)
 
-    from fcs import lib
-    from fcs.flops import attempt_flops, hbm_bytes_per_cell
-    from fcs.mechanism import config_field, config_mechanism
+    from fcs import learned, lib
+    from fcs.flops import attempt_flops
+    from fcs.mechanism import config_field, config_mechanism, counterflow_field
 
-    mech = config_mechanism(1)
-    field = config_field(1, mech, n_cells=args.cells)
-    dt = 1e-6
+    mech = config_mechanism(2)
     gm = lib.Mech(mech)
-    solver = lib.Solver(gm, solver=args.solver, jacobian="exact", absTol=1e-8, relTol=1e-4)
-    batch = lib.Batch(solver, field)
-
-    for _ in range(args.warmup):
-        batch.run(dt, timed=True)
-    batch.profile(True)     # HIP event pair around every step launch, on the solver's stream
-    launches, step_ms = 0, 0.0
+    if args.workload == "mixture":
+        return mixture(args, mech, gm)
+    if args.workload == "attempt":
+        return attempt(mech, gm)
+    solver = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
+    eq = lib.equilibrator(solver)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run(dt, timed=True)
-        n, ms = batch.stats()
-        launches += n
-        step_ms += ms
-    wall = time.perf_counter() - t0
-    out = batch.download()
-    nAtt = int(out["nAttempts"].sum())          # every step re-integrates the same inputs
-    bad = int(((out["status"] & 6) != 0).sum())
+    field = config_field(2, mech, equilibrate=eq)
+    log(f"field: {field.n_cells} cells in {time.perf_counter() - t0:.1f} s")
 
-    ms_step = 1e3 * wall / args.steps
-    value = args.cells / (wall / args.steps)
-    fpa = attempt_flops(mech, args.solver)
+    # ---- learned seed: training data from the plain integrator, training (untimed)
+    t0 = time.perf_counter()
+    Xs, Ts = [], []
+    for seed in (2001, 2002):
+        f = counterflow_field(mech, args.train_side, seed=seed, equilibrate=eq)
+        X, T = learned.samples(solver, f, DT)
+        Xs.append(X)
+        Ts.append(T)
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net, hist = learned.train(X, T, epochs=args.epochs, batch=512, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    log(f"training: {X.shape[0]} samples in {t_data:.1f} s, {args.epochs} epochs in {t_train:.1f} s, "
+        f"loss {hist[0]:.3g} -> {hist[-1]:.3g}")
+
+    fpa = attempt_flops(mech, "OptRodas34")
+
+    # ---- plain Rodas34 on the field
+    plain = lib.Batch(solver, field)
+    plain.set_launch(0, GRAPH_LAUNCHES)
+    wall_p, _, _ = timed_runs(plain, args.steps, args.warmup, profile=False)
+    rp = plain.download()
+    log(f"plain: {wall_p / args.steps * 1e3:.1f} ms per step")
+
+    # ---- learned-seeded Rodas34 on the same field (the measured workload)
+    lrn = lib.Batch(solver, field)
+    lrn.set_launch(0, GRAPH_LAUNCHES)
+    lrn.set_learned(net, "verify")
+    wall_l, _, _ = timed_runs(lrn, args.steps, args.warmup, profile=False)
+    rl = lrn.download()
+    npred, nver, nacc = lrn.learned_stats()
+    log(f"learned: {wall_l / args.steps * 1e3:.1f} ms per step, {nacc}/{npred} proposals accepted")
+
+    # roofline of the step kernel on the learned run: an event pair around
+    # every eager step launch on the solver's stream
+    lrn.set_launch(0, 0)
+    _, launches, step_ms = timed_runs(lrn, 1, 0, profile=True)
+    att_steps = int(rl["nAttempts"][(rl["status"] & 8) == 0].sum())
     avg_launch_s = 1e-3 * step_ms / max(launches, 1)
-    flops_launch = fpa * nAtt * args.steps / max(launches, 1)
-    achieved = flops_launch / avg_launch_s / 1e12
+    achieved = fpa * att_steps / max(launches, 1) / avg_launch_s / 1e12
+
+    # ---- error against a tight-tolerance solve of a stratified sample
+    idx = stratified(field, 48)
+    sub = field.subset(idx)
+    tight = lib.Solver(gm, Treact=TREACT, absTol=1e-14, relTol=1e-10, maxSteps=2000000).solve(sub, DT)
+    if tight["rc"] != 0:
+        log(f"tight reference: {(tight['status'] & 6 != 0).sum()} cells failed")
+    rho = tight["rho0"]
+    Yt = sub.Y + tight["RR"] * DT / rho[None, :]
+
+    def err(r):
+        Y = sub.Y + r["RR"][:, idx] * DT / rho[None, :]
+        dY = np.abs(Y - Yt)
+        return dict(max_rel_Y=float((dY / np.maximum(np.abs(Yt), 1e-6)).max()),
+                    max_abs_Y=float(dY.max()),
+                    max_rel_T=float((np.abs(r["Tend"][idx] - tight["Tend"]) / tight["Tend"]).max()))
+
+    # ---- plain Rodas34 on the 65,536-cell mixture-fraction field
+    secondary = None
+    if not args.no_secondary:
+        f1 = config_field(1, mech)
+        s1 = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4)
+        b1 = lib.Batch(s1, f1)
+        w1, _, _ = timed_runs(b1, 1, 1, profile=False)
+        secondary = {"workload": "plain OptRodas34, 65,536-cell 1-D mixture-fraction field, deltaT 1e-6 s",
+                     "value": f1.n_cells / w1, "ms_per_step": 1e3 * w1,
+                     "attempts_per_step": int(b1.download()["nAttempts"].sum())}
+
+    value = field.n_cells / (wall_l / args.steps)
+    plain_value = field.n_cells / (wall_p / args.steps)
     line = {
         "metric": METRIC,
         "value": value,
@@ -103,28 +260,43 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1e3 * wall_l / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"plain {args.solver}, 53-species/325-reaction synthetic mechanism, "
-                               f"{args.cells} cells (1-D mixture-fraction profile), deltaT 1e-6 s",
-                   "cells": args.cells, "species": mech.n_species, "reactions": mech.n_reactions,
-                   "solver": args.solver, "jacobian": "exact", "absTol": 1e-8, "relTol": 1e-4,
-                   "attempts_per_step": nAtt, "failed_cells": bad},
+        "config": {"workload": "learned-seeded OptRodas34 (verify mode), 512x512 synthetic counterflow field "
+                               "(262,144 cells), 53-species/325-reaction synthetic mechanism, deltaT 1e-6 s, "
+                               f"Treact 500 K, step launches replayed from {GRAPH_LAUNCHES}-launch hipGraphs",
+                   "cells": field.n_cells, "species": mech.n_species, "reactions": mech.n_reactions,
+                   "solver": "OptRodas34", "jacobian": "exact", "absTol": 1e-8, "relTol": 1e-4,
+                   "network": "56-128-128-128-55 SiLU, BF16 MFMA / FP32 accumulation",
+                   "training_samples": int(X.shape[0]), "training_epochs": args.epochs,
+                   "training_s": t_data + t_train},
+        "plain": {"value": plain_value, "ms_per_step": 1e3 * wall_p / args.steps,
+                  "attempts_per_step": int(rp["nAttempts"].sum()),
+                  "error_vs_tight": err(rp)},
+        "learned": {"speedup_vs_plain": value / plain_value, "proposals": npred, "verified": nver,
+                    "accepted": nacc, "acceptance_rate": nacc / max(npred, 1),
+                    "attempts_per_step": int(rl["nAttempts"].sum()),
+                    "fallback_cells_bit_identical": bool(all(
+                        np.array_equal(rl[k][..., (rl["status"] & 8) == 0], rp[k][..., (rl["status"] & 8) == 0])
+                        for k in ("RR", "Tend", "nAttempts"))),
+                    "failed_cells": int(((rl["status"] & 6) != 0).sum()),
+                    "error_vs_tight": err(rl)},
+        "error_sample": "48 product + 48 flame-front cells, relTol 1e-10 / absTol 1e-14 GPU solve as reference "
+                        "(max_rel_Y: |dY| / max(|Y|, 1e-6))",
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_step<56,Rodas34> (FP64 VALU; one Rosenbrock attempt per active cell)",
-                     "flops_per_attempt": fpa,
-                     "launches": launches,
-                     "avg_launch_ms": 1e3 * avg_launch_s,
-                     "step_kernel_ms_per_step": step_ms / args.steps,
-                     "hbm_GBs_algorithmic": hbm_bytes_per_cell(mech) * args.cells * args.steps / (1e-3 * step_ms) / 1e9},
+                     "flops_per_attempt": fpa, "launches": launches,
+                     "avg_launch_ms": 1e3 * avg_launch_s},
     }
+    if secondary:
+        line["plain_mixture_fraction_65536"] = secondary
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(mech, field, dt)
+        line["cpu_baseline"] = cpu_baseline(mech, field)
     print(json.dumps(line), flush=True)
 
 
