@@ -123,6 +123,33 @@ class Mechanism:
                 a["plog_Ta"][r * pmax:r * pmax + k] = T
         return a
 
+    @staticmethod
+    def from_arrays(a: dict, names=None) -> "Mechanism":
+        """Inverse of arrays(): a Mechanism from the flat fcs_mech_desc arrays."""
+        ns, nr = int(a["n_species"]), int(a["n_reactions"])
+        pmax = int(a["plog_max_points"])
+        rx = []
+        for r in range(nr):
+            t = int(a["rxn_type"][r])
+            x = Reaction(t, bool(a["rxn_reversible"][r]),
+                         [int(v) for v in a["rxn_lhs"][r * 3:r * 3 + int(a["rxn_n_lhs"][r])]],
+                         [int(v) for v in a["rxn_rhs"][r * 3:r * 3 + int(a["rxn_n_rhs"][r])]],
+                         float(a["A"][r]), float(a["beta"][r]), float(a["Ta"][r]),
+                         float(a["A0"][r]), float(a["beta0"][r]), float(a["Ta0"][r]))
+            if t in (RXN_THIRD_BODY, RXN_LINDEMANN, RXN_TROE):
+                x.eff = np.array(a["efficiencies"][r * ns:(r + 1) * ns], np.float64)
+            x.troe = tuple(float(v) for v in a["troe"][r * 4:r * 4 + 4])
+            if t == RXN_PLOG:
+                k = int(a["plog_n"][r])
+                x.plog = tuple([float(v) for v in a[f"plog_{q}"][r * pmax:r * pmax + k]]
+                               for q in ("p", "A", "beta", "Ta"))
+            rx.append(x)
+        return Mechanism(list(names) if names is not None else [f"S{i}" for i in range(ns)],
+                         np.array(a["W"], np.float64), np.array(a["T_low"], np.float64),
+                         np.array(a["T_high"], np.float64), np.array(a["T_common"], np.float64),
+                         np.array(a["coeffs_low"], np.float64).reshape(ns, 7),
+                         np.array(a["coeffs_high"], np.float64).reshape(ns, 7), rx)
+
     def stoich_matrix(self) -> np.ndarray:
         """Net stoichiometric coefficients nu[r, i] (products minus reactants)."""
         nu = np.zeros((self.n_reactions, self.n_species))

@@ -23,3 +23,17 @@ def mech0():
 def mech1():
     from fcs.mechanism import config_mechanism
     return config_mechanism(1)
+
+
+@pytest.fixture(scope="session")
+def gri30():
+    """GRI-3.0 (53 species, 325 reactions) from the reference's counterflow
+    tutorial and 1,000 cells of its converged flame snapshot
+    (tests/golden/gri30_counterflow.npz, written by tests/golden/make_gri30.py)."""
+    import numpy as np
+    from fcs.mechanism import Field, Mechanism
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "gri30_counterflow.npz"))
+    a = {k[5:]: g[k] for k in g.files if k.startswith("mech_")}
+    mech = Mechanism.from_arrays(a, names=[str(s) for s in g["names"]])
+    field = Field(g["T"].copy(), g["p"].copy(), g["Y"].copy(), np.full(g["T"].size, 1e-7))
+    return mech, field
