@@ -38,7 +38,7 @@ constexpr int WAVE = 64;
 constexpr int JS = 65;
 
 enum { RXN_ARR = 0, RXN_TB = 1, RXN_LIND = 2, RXN_TROE = 3, RXN_PLOG = 4 };
-enum { SOLVER_RODAS34 = 0, SOLVER_ROS34 = 1, SOLVER_SEULEX = 2 };
+enum { SOLVER_RODAS34 = 0, SOLVER_ROS34 = 1, SOLVER_SEULEX = 2, SOLVER_RODAS23 = 3, SOLVER_ROS23 = 4 };
 enum { CELL_SKIPPED = 1, CELL_MAXSTEPS = 2, CELL_NONFINITE = 4 };
 
 // Device-resident mechanism tables (built on the host by fcs_mech_create).

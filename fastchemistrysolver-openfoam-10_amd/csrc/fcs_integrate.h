@@ -1,5 +1,5 @@
 // fcs_integrate.h -- gfx950 device code of the ODE steppers: Rodas3(4),
-// ROS3(4) and linearly-implicit Euler extrapolation (Seulex), each with the
+// ROS3(4), Rodas3(2), ROS3(2) and linearly-implicit Euler extrapolation (Seulex), each with the
 // reference's step-size controller, run by one wavefront per cell.
 #pragma once
 #include "fcs_device.h"
@@ -35,6 +35,28 @@ constexpr double e1 = 34.0 / 108.0, e2 = 7.0 / 36.0, e4 = 125.0 / 108.0;
 constexpr double d1 = 1.0 / 2.0, d2 = -3.0 / 2.0, d3 = 605.0 / 250.0, d4 = 29.0 / 250.0;
 constexpr double invGamma = 2.0;
 }  // namespace ros
+
+// Rodas3(2) (Sandu et al. 1997): src/ODE/OptRodas23/OptRodas23.H
+namespace rodas23 {
+constexpr double d1 = 1.0 / 2.0, d2 = 3.0 / 2.0;
+constexpr double a31 = 2.0;
+constexpr double c21 = 4.0, c31 = 1.0, c32 = -1.0, c41 = 1.0, c42 = -1.0, c43 = -8.0 / 3.0;
+constexpr double invGamma = 1.0 / (1.0 / 2.0);
+}  // namespace rodas23
+
+// ROS3 with embedded order 2 (Sandu et al. 1997):
+// src/ODE/OptRosenbrock23/OptRosenbrock23.H
+namespace ros23 {
+constexpr double a21 = 1.0;
+constexpr double c21 = -1.0156171083877702091975600115545;
+constexpr double c31 = 4.0759956452537699824805835358067, c32 = 9.2076794298330791242156818474003;
+constexpr double b1 = 1.0, b2 = 6.1697947043828245592553615689730, b3 = -0.4277225654321857332623837380651;
+constexpr double e1 = 0.5, e2 = -2.9079558716805469821718236208017, e3 = 0.2235406989781156962736090927619;
+constexpr double gamma = 0.43586652150845899941601945119356;
+constexpr double d1 = 0.43586652150845899941601945119356, d2 = 0.24291996454816804366592249683314,
+                 d3 = 2.1851380027664058511513169485832;
+constexpr double invGamma = 1.0 / gamma;
+}  // namespace ros23
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
@@ -124,6 +146,55 @@ __constant__ const Tableau kRos34 = {
     {ros::b1, ros::b2, ros::b3, ros::b4, 0, 0},
     {ros::e1, ros::e2, 0.0, ros::e4, 0, 0}};
 
+// Rodas3(2): OptRodas23.C:245-311.  Stage 2 takes f(y0) (newF = 0 with the
+// carried f initialised to f(y0)); stage 3 evaluates f at y0 + a31 k1 and
+// stage 4 at y0 + a31 k1 + k3; the new state is y0 + a31 k1 + k3 + k4 and the
+// error estimate is k4.
+__constant__ const Tableau kRodas23 = {
+    4, rodas23::invGamma, {0, 0, 1, 1, 0, 0},
+    {{0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {rodas23::a31, 0, 0, 0, 0, 0},
+     {rodas23::a31, 0, 1.0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0}},
+    {{0, 0, 0, 0, 0, 0},
+     {rodas23::c21, 0, 0, 0, 0, 0},
+     {rodas23::c31, rodas23::c32, 0, 0, 0, 0},
+     {rodas23::c41, rodas23::c42, rodas23::c43, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0}},
+    {rodas23::d1, rodas23::d2, 0, 0, 0, 0},
+    {rodas23::a31, 0, 1.0, 1.0, 0, 0},
+    {0, 0, 0, 1.0, 0, 0}};
+
+// ROS3(2): OptRosenbrock23.C:253-290 (stage 3 reuses the stage-2 derivative).
+__constant__ const Tableau kRos23 = {
+    3, ros23::invGamma, {0, 1, 0, 0, 0, 0},
+    {{0, 0, 0, 0, 0, 0},
+     {ros23::a21, 0, 0, 0, 0, 0},
+     {ros23::a21, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0}},
+    {{0, 0, 0, 0, 0, 0},
+     {ros23::c21, 0, 0, 0, 0, 0},
+     {ros23::c31, ros23::c32, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0}},
+    {ros23::d1, ros23::d2, ros23::d3, 0, 0, 0},
+    {ros23::b1, ros23::b2, ros23::b3, 0, 0, 0},
+    {ros23::e1, ros23::e2, ros23::e3, 0, 0, 0}};
+
+// Tableau of a Rosenbrock solver id (SOLVER_*; Seulex has none).
+template <int SOLVER>
+__device__ __forceinline__ const Tableau& tableau_of()
+{
+    static_assert(SOLVER != SOLVER_SEULEX, "Seulex is not a Rosenbrock method");
+    return SOLVER == SOLVER_ROS34 ? kRos34 : SOLVER == SOLVER_RODAS23 ? kRodas23 : SOLVER == SOLVER_ROS23 ? kRos23 : kRodas34;
+}
+
 // One Rosenbrock attempt from y0 with step dx.  y0 is clipped in place by
 // the Jacobian (src/ChemistryModel/FastChemistryModel.C:498-501); yT
 // receives the candidate state.  Returns the error norm.
@@ -136,7 +207,7 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
     form_matrix<NP>(L, 1.0 * invdx * tb.invGamma);
     const LuState st = lu_factor<NP>(L);
     double k[6] = {0, 0, 0, 0, 0, 0};
-    double dydx = 0.0;
+    double dydx = dfdx;   // a stage without its own f before any f evaluation takes f(y0)
     for (int s = 0; s < tb.stages; ++s) {
         double rhs;
         if (s == 0) {

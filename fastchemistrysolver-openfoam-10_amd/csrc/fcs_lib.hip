@@ -144,7 +144,7 @@ __global__ __launch_bounds__(64) void k_step(DevMech M, DevOpts o, CellArrays a,
         const double xEnd = a.dtCell ? a.dtCell[c] : a.dt;
         const int nStep0 = nStep;
         double dxAcc = 0.0;
-        const int st = rosenbrock_step<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, a.p0[c], xEnd, y,
+        const int st = rosenbrock_step<NP>(M, L, o, tableau_of<SOLVER>(), a.p0[c], xEnd, y,
                                            x, dxTry, dx, dxTry0, flags, nStep, nAtt, dxAcc);
         if (lane == 0 && nStep0 == 0 && dxAcc > 0.0 && a.dxFirst[c] == 0.0) a.dxFirst[c] = dxAcc;
         if (st == STEP_CONTINUE) {
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(64) void k_attempt(DevMech M, DevOpts o, int nS, co
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
     double yT = 0.0;
-    const double e = rosenbrock_attempt<NP>(M, L, o, SOLVER == SOLVER_ROS34 ? kRos34 : kRodas34, p[s], log(p[s]),
+    const double e = rosenbrock_attempt<NP>(M, L, o, tableau_of<SOLVER>(), p[s], log(p[s]),
                                             dx[s], y, yT);
     if (ext >= 0) {
         YT[(size_t)ext * nS + s] = y;
@@ -386,21 +386,23 @@ namespace {
 
 struct KernelSet {
     int NP;
-    const void* step[3];        // Rodas34, Rosenbrock34, Seulex
+    const void* step[5];        // indexed by FCS_SOLVER_*
     const void* verify;
     const void* deriv;
     const void* jac;
     const void* lu;
-    const void* attempt[2];     // Rodas34, Rosenbrock34
+    const void* attempt[5];     // indexed by FCS_SOLVER_* (none for Seulex)
 };
 
 #define FCS_KSET(N)                                                                                        \
     {N,                                                                                                    \
      {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>,                        \
-      (const void*)k_step_seulex<N>},                                                                      \
+      (const void*)k_step_seulex<N>, (const void*)k_step<N, SOLVER_RODAS23>,                               \
+      (const void*)k_step<N, SOLVER_ROS23>},                                                               \
      (const void*)k_verify<N>,                                                                             \
      (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>,                \
-     {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>}},
+     {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>, nullptr,         \
+      (const void*)k_attempt<N, SOLVER_RODAS23>, (const void*)k_attempt<N, SOLVER_ROS23>}},
 const KernelSet kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET)};
 #undef FCS_KSET
 
@@ -881,8 +883,8 @@ static int solver_grid(fcs_solver* s)
 int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_solver** out)
 {
     if (!mech || !opts || !out) return FCS_E_ARG;
-    if (opts->solver < 0 || opts->solver > 2 || opts->jacobian < 0 || opts->jacobian > 1 || opts->maxSteps < 1 ||
-        opts->maxSteps > 10000000 || !(opts->absTol > 0) || !(opts->relTol > 0))
+    if (opts->solver < 0 || opts->solver > FCS_SOLVER_ROSENBROCK23 || opts->jacobian < 0 || opts->jacobian > 1 ||
+        opts->maxSteps < 1 || opts->maxSteps > 10000000 || !(opts->absTol > 0) || !(opts->relTol > 0))
         return FCS_E_ARG;
     const KernelSet* ks = pick_kernels(mech->ns + 1);
     if (!ks) return FCS_E_UNSUPPORTED;
@@ -897,8 +899,8 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
         return FCS_E_HIP;
     }
     if (s->ldsBytes > 65536)
-        for (const void* f : {ks->step[0], ks->step[1], ks->step[2], ks->deriv, ks->jac, ks->lu, ks->attempt[0],
-                              ks->attempt[1]})
+        for (const void* f : {ks->step[0], ks->step[1], ks->step[2], ks->step[3], ks->step[4], ks->deriv, ks->jac,
+                              ks->lu, ks->attempt[0], ks->attempt[1], ks->attempt[3], ks->attempt[4]})
             (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->ldsBytes);
     if (solver_grid(s) != FCS_OK) {
         (void)hipStreamDestroy(s->stream);

@@ -64,13 +64,18 @@ def trsv_flops(n: int) -> int:
     return 2 * n * n - n
 
 
+# (triangular solves, RHS evaluations, vector updates per row) of one attempt
+_STAGES = {"OptRodas34": (6, 5, 12), "OptRosenbrock34": (4, 2, 10),
+           "OptRodas23": (4, 2, 10), "OptRosenbrock23": (3, 1, 8)}
+
+
 def attempt_flops(mech, solver: str = "OptRodas34") -> int:
-    """FP64 operations of one attempt (Rodas34: 1 Jacobian, 1 LU, 6 solves,
-    5 RHS; Rosenbrock34: 1 Jacobian, 1 LU, 4 solves, 2 RHS)."""
+    """FP64 operations of one attempt: one Jacobian and one LU, then
+    Rodas34 6 solves / 5 RHS, Rosenbrock34 4 / 2, Rodas23 4 / 2,
+    Rosenbrock23 3 / 1."""
     n = mech.n_species + 1
-    if solver == "OptRosenbrock34":
-        return jacobian_flops(mech) + lu_flops(n) + 4 * trsv_flops(n) + 2 * rhs_flops(mech) + 10 * n
-    return jacobian_flops(mech) + lu_flops(n) + 6 * trsv_flops(n) + 5 * rhs_flops(mech) + 12 * n
+    solves, rhs, vec = _STAGES[solver]
+    return jacobian_flops(mech) + lu_flops(n) + solves * trsv_flops(n) + rhs * rhs_flops(mech) + vec * n
 
 
 def hbm_bytes_per_cell(mech) -> int:

@@ -7,7 +7,7 @@
  *
  * Reference interfaces replaced (paths relative to the reference tree):
  *   - solver selection `chemistryType { solver OptRodas34|OptRosenbrock34|
- *     OptSeulex; method FastChemistryModel; }`
+ *     OptRodas23|OptRosenbrock23|OptSeulex; method FastChemistryModel; }`
  *       src/odeChemistrySolvers.C:66-72, src/ChemistryModel/FastChemistryModel.H:230
  *   - Coeffs keys absTol/relTol/maxSteps        src/ODE/OptRodas34/OptRodas34.C:40-43,
  *                                                src/ODE/OptSeulex/OptSeulex.C:40-46
@@ -57,6 +57,8 @@ extern "C" {
 #define FCS_SOLVER_RODAS34        0   /* "OptRodas34"       */
 #define FCS_SOLVER_ROSENBROCK34   1   /* "OptRosenbrock34"  */
 #define FCS_SOLVER_SEULEX         2   /* "OptSeulex"        */
+#define FCS_SOLVER_RODAS23        3   /* "OptRodas23"       */
+#define FCS_SOLVER_ROSENBROCK23   4   /* "OptRosenbrock23"  */
 
 #define FCS_JACOBIAN_EXACT  0   /* jacobian exact; (default)                     */
 #define FCS_JACOBIAN_FAST   1   /* jacobian fast;  diagonal dC/dY only           */
@@ -262,9 +264,12 @@ int fcs_eval_derivatives(fcs_solver* solver, int64_t nStates, const double* p,
 int fcs_eval_jacobian(fcs_solver* solver, int64_t nStates, const double* p,
                       double* YT, double* dYTdt, double* Jy);
 /* One Rosenbrock attempt per state with step dx[s]: OptRodas34::Rodas34Solve
- * (src/ODE/OptRodas34/OptRodas34.C:291-879) or
+ * (src/ODE/OptRodas34/OptRodas34.C:291-879),
  * OptRosenbrock34::Rosenbrock34Solve (src/ODE/OptRosenbrock34/
- * OptRosenbrock34.C:281-677), by the solver's option.  YT [n*nStates] start
+ * OptRosenbrock34.C:281-677), OptRodas23::Rodas23Solve
+ * (src/ODE/OptRodas23/OptRodas23.C:198-314) or
+ * OptRosenbrock23::Rosenbrock23Solve (src/ODE/OptRosenbrock23/
+ * OptRosenbrock23.C:204-294), by the solver's option.  YT [n*nStates] start
  * states (clipped >= 0 on output, as the Jacobian evaluation clips them),
  * YTnew [n*nStates] candidate states, err [nStates] error norms.
  * FCS_E_UNSUPPORTED for OptSeulex. */

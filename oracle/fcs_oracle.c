@@ -775,6 +775,113 @@ static double ros34_attempt(const orc_mech* m, const fcs_solver_opts* o, double 
     return err_norm(n, y0, yT, err, o->absTol, o->relTol);
 }
 
+/* Rodas3(2) coefficients: src/ODE/OptRodas23/OptRodas23.H (c3 = 1; stage 2
+ * reuses f(y0), stages 3 and 4 evaluate f at y0 + a31 k1 and y0 + a31 k1 + k3) */
+static const double Q_d1 = 1.0 / 2.0, Q_d2 = 3.0 / 2.0;
+static const double Q_a31 = 2.0;
+static const double Q_c21 = 4.0, Q_c31 = 1.0, Q_c32 = -1.0;
+static const double Q_c41 = 1.0, Q_c42 = -1.0, Q_c43 = -8.0 / 3.0;
+static const double Q_gamma = 0.5;
+
+/* src/ODE/OptRodas23/OptRodas23.C:198-314 */
+static double rodas23_attempt(const orc_mech* m, const fcs_solver_opts* o, double p,
+                              double dx, work* w)
+{
+    const int n = w->n;
+    const double invdx = 1.0 / dx;
+    double *y0 = w->y0, *yT = w->yT, *k1 = w->k1, *k2 = w->k2, *k3 = w->k3, *dy = w->dy,
+           *err = w->err, *dydx = w->dydx, *dfdx = w->dfdx;
+    orc_jacobian(m, p, y0, dfdx, w->J, o->jacobian, NULL);
+    form_matrix(n, w->a, w->J, invdx * (1.0 / Q_gamma));
+    orc_lu_factor(n, w->a, w->piv, w->invD);
+    const double tq = g_dfdt_zero ? 0.0 : 1.0;
+    {
+        const double e = tq * dx * Q_d1;
+        for (int i = 0; i < n; ++i) k1[i] = dfdx[i] + e * dfdx[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k1);
+    {
+        /* stage 2 is evaluated at y0: f(y0) is the function-value slot */
+        const double e = tq * dx * Q_d2;
+        for (int i = 0; i < n; ++i) k2[i] = dfdx[i] + e * dfdx[i] + Q_c21 * k1[i] / dx;
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k2);
+    for (int i = 0; i < n; ++i) { dy[i] = Q_a31 * k1[i]; yT[i] = y0[i] + dy[i]; }
+    orc_derivatives(m, p, yT, dydx);
+    for (int i = 0; i < n; ++i) k3[i] = dydx[i] + (Q_c31 * k1[i] + Q_c32 * k2[i]) / dx;
+    orc_lu_solve(n, w->a, w->piv, w->invD, k3);
+    for (int i = 0; i < n; ++i) { dy[i] += k3[i]; yT[i] = y0[i] + dy[i]; }
+    orc_derivatives(m, p, yT, dydx);
+    for (int i = 0; i < n; ++i)
+        err[i] = dydx[i] + (Q_c41 * k1[i] + Q_c42 * k2[i] + Q_c43 * k3[i]) / dx;
+    orc_lu_solve(n, w->a, w->piv, w->invD, err);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i] + dy[i] + err[i];
+    return err_norm(n, y0, yT, err, o->absTol, o->relTol);
+}
+
+/* ROS3 coefficients (order 3, embedded order 2, L-stable):
+ * src/ODE/OptRosenbrock23/OptRosenbrock23.H */
+static const double T_a21 = 1.0;
+static const double T_c21 = -1.0156171083877702091975600115545;
+static const double T_c31 = 4.0759956452537699824805835358067;
+static const double T_c32 = 9.2076794298330791242156818474003;
+static const double T_b1 = 1.0, T_b2 = 6.1697947043828245592553615689730,
+                    T_b3 = -0.4277225654321857332623837380651;
+static const double T_e1 = 0.5, T_e2 = -2.9079558716805469821718236208017,
+                    T_e3 = 0.2235406989781156962736090927619;
+static const double T_gamma = 0.43586652150845899941601945119356;
+static const double T_d1 = 0.43586652150845899941601945119356,
+                    T_d2 = 0.24291996454816804366592249683314,
+                    T_d3 = 2.1851380027664058511513169485832;
+
+/* src/ODE/OptRosenbrock23/OptRosenbrock23.C:204-294.  a31 = 1, a32 = 0:
+ * stage 3 reuses stage 2's f(y0 + k1). */
+static double ros23_attempt(const orc_mech* m, const fcs_solver_opts* o, double p,
+                            double dx, work* w)
+{
+    const int n = w->n;
+    const double invdx = 1.0 / dx;
+    double *y0 = w->y0, *yT = w->yT, *k1 = w->k1, *k2 = w->k2, *k3 = w->k3, *err = w->err,
+           *dydx = w->dydx, *dfdx = w->dfdx;
+    orc_jacobian(m, p, y0, dfdx, w->J, o->jacobian, NULL);
+    form_matrix(n, w->a, w->J, invdx * (1.0 / T_gamma));
+    orc_lu_factor(n, w->a, w->piv, w->invD);
+    const double tq = g_dfdt_zero ? 0.0 : 1.0;
+    {
+        const double e = tq * dx * T_d1;
+        for (int i = 0; i < n; ++i) k1[i] = dfdx[i] + e * dfdx[i];
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k1);
+    for (int i = 0; i < n; ++i) yT[i] = y0[i] + T_a21 * k1[i];
+    orc_derivatives(m, p, yT, dydx);
+    {
+        const double e = tq * dx * T_d2;
+        for (int i = 0; i < n; ++i) k2[i] = dydx[i] + e * dfdx[i] + T_c21 * k1[i] / dx;
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k2);
+    {
+        const double e = tq * dx * T_d3;
+        for (int i = 0; i < n; ++i) k3[i] = dydx[i] + e * dfdx[i] + (T_c31 * k1[i] + T_c32 * k2[i]) / dx;
+    }
+    orc_lu_solve(n, w->a, w->piv, w->invD, k3);
+    for (int i = 0; i < n; ++i) {
+        yT[i] = y0[i] + T_b1 * k1[i] + T_b2 * k2[i] + T_b3 * k3[i];
+        err[i] = T_e1 * k1[i] + T_e2 * k2[i] + T_e3 * k3[i];
+    }
+    return err_norm(n, y0, yT, err, o->absTol, o->relTol);
+}
+
+static double rosenbrock_attempt(const orc_mech* m, const fcs_solver_opts* o, double p,
+                                 double dx, work* w)
+{
+    switch (o->solver) {
+    case FCS_SOLVER_RODAS34: return rodas34_attempt(m, o, p, dx, w);
+    case FCS_SOLVER_ROSENBROCK34: return ros34_attempt(m, o, p, dx, w);
+    case FCS_SOLVER_RODAS23: return rodas23_attempt(m, o, p, dx, w);
+    default: return ros23_attempt(m, o, p, dx, w);
+    }
+}
+
 static void logrec(orc_step_rec* log, int32_t cap, int32_t* len, double x, double dx,
                    double err, int acc, int order)
 {
@@ -798,8 +905,7 @@ static int rosenbrock_integrate(const orc_mech* m, const fcs_solver_opts* o, dou
         const double dxTry0 = dxTry;
         if ((x + dxTry - xEnd) * (x + dxTry - xStart) > 0) { last = 1; dxTry = xEnd - x; }
         double dx = dxTry;
-        double Err = (o->solver == FCS_SOLVER_RODAS34) ? rodas34_attempt(m, o, p, dx, w)
-                                                       : ros34_attempt(m, o, p, dx, w);
+        double Err = rosenbrock_attempt(m, o, p, dx, w);
         ++*nAtt;
         if (!(Err == Err)) return FCS_CELL_NONFINITE;
         while (Err > 1) {
@@ -809,8 +915,7 @@ static int rosenbrock_integrate(const orc_mech* m, const fcs_solver_opts* o, dou
             logrec(log, cap, len, x, dx, Err, 0, 0);
             const double sc = SAFE * pow(Err, -ALPHA_DEC);
             dx *= sc > MIN_SCALE ? sc : MIN_SCALE;
-            Err = (o->solver == FCS_SOLVER_RODAS34) ? rodas34_attempt(m, o, p, dx, w)
-                                                    : ros34_attempt(m, o, p, dx, w);
+            Err = rosenbrock_attempt(m, o, p, dx, w);
             ++*nAtt;
             if (!(Err == Err)) return FCS_CELL_NONFINITE;
         }
@@ -1104,8 +1209,7 @@ double orc_rosenbrock_attempt(const orc_mech* m, const fcs_solver_opts* o, doubl
     work w;
     work_init(&w, m->n);
     for (int i = 0; i < m->n; ++i) w.y0[i] = y[i];
-    const double err = (o->solver == FCS_SOLVER_RODAS34) ? rodas34_attempt(m, o, p, dx, &w)
-                                                         : ros34_attempt(m, o, p, dx, &w);
+    const double err = rosenbrock_attempt(m, o, p, dx, &w);
     for (int i = 0; i < m->n; ++i) { y[i] = w.y0[i]; yNew[i] = w.yT[i]; }
     work_free(&w);
     return err;

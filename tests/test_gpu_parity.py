@@ -86,9 +86,9 @@ def test_jacobian(fcslib, cfg, jac):
 
 
 @pytest.mark.parametrize("cfg", [0, 1])
-@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34"])
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34", "OptRodas23", "OptRosenbrock23"])
 def test_single_attempt(fcslib, cfg, solver):
-    """One Rodas34Solve / Rosenbrock34Solve per state, from accepted-size
+    """One Rodas34Solve / Rosenbrock34Solve / Rodas23Solve / Rosenbrock23Solve per state, from accepted-size
     steps (1e-14 s) to heavily rejected ones (1e-9 s): error norm and
     candidate state against the oracle.  Rejected steps amplify round-off,
     so the state bar is relative to the step's own increment."""
@@ -146,7 +146,7 @@ def test_lu_pivot_quirk_on_device(fcslib, mech0):
 
 
 # ------------------------------------------------------------ solve(deltaT)
-@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34", "OptSeulex"])
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34", "OptSeulex", "OptRodas23", "OptRosenbrock23"])
 def test_solve_config0_all_cells(fcslib, mech0, solver):
     field = M.config_field(0, mech0)
     dt = M.CONFIG_DT[0]
@@ -164,7 +164,7 @@ def test_solve_config0_fast_jacobian(fcslib, mech0):
     assert_solve_parity(g, o, field, dt)
 
 
-@pytest.mark.parametrize("solver", ["OptRodas34", "OptSeulex"])
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptSeulex", "OptRodas23", "OptRosenbrock23"])
 def test_solve_config1_sample(fcslib, mech1, solver):
     field = M.config_field(1, mech1).subset(np.random.default_rng(1).choice(65536, 192, replace=False))
     dt = M.CONFIG_DT[1]

@@ -35,10 +35,11 @@ def one_step(o, p, y0, h, solver):
     return r["y"], r["log"][0][2]
 
 
-@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34"])
-def test_rosenbrock_tableau_order(solver):
+@pytest.mark.parametrize("solver,order", [("OptRodas34", 4), ("OptRosenbrock34", 4),
+                                          ("OptRodas23", 3), ("OptRosenbrock23", 3)])
+def test_rosenbrock_tableau_order(solver, order):
     """Textbook form (df/dt = 0): local error of the propagated solution is
-    O(h^5) (order 4) and the embedded error estimate is O(h^4)."""
+    O(h^(order+1)) and the embedded error estimate is O(h^order)."""
     m = smooth_mech()
     o = Oracle(m)
     y0 = np.array([0.3, 0.3, 0.2, 0.2, 1200.0])
@@ -53,17 +54,21 @@ def test_rosenbrock_tableau_order(solver):
             errs.append(np.abs(y1[:-1] - ref[:-1]).max())
             ests.append(est * 1e10)  # norm = max|err| / (absTol + relTol max|y|)
         r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
-        assert 2 ** 4.5 < r1 < 2 ** 5.6 and 2 ** 4.5 < r2 < 2 ** 5.6, (errs, r1, r2)
+        lo, hi = 2 ** (order + 0.5), 2 ** (order + 1.6)
+        assert lo < r1 < hi and lo < r2 < hi, (errs, r1, r2)
         e1, e2 = ests[0] / ests[1], ests[1] / ests[2]
-        assert 2 ** 3.5 < e1 < 2 ** 4.6 and 2 ** 3.5 < e2 < 2 ** 4.6, (ests, e1, e2)
+        assert lo / 2 < e1 < hi / 2 and lo / 2 < e2 < hi / 2, (ests, e1, e2)
     finally:
         orc_lib().orc_set_textbook_rosenbrock(0)
 
 
-# Seulex keeps its order near 2-3 on this mechanism and needs more than
-# maxSteps = 10000 steps at relTol 1e-10, so it is checked at 1e-8.
+# Seulex keeps its order near 2-3 on this mechanism, and the third-order
+# Rosenbrock methods are third order: at relTol 1e-10 they need more than
+# maxSteps = 10000 steps, so they are checked at 1e-8.
 @pytest.mark.parametrize("solver,atol,rtol,tolY", [("OptRodas34", 1e-14, 1e-10, 1e-7),
                                                   ("OptRosenbrock34", 1e-14, 1e-10, 1e-7),
+                                                  ("OptRodas23", 1e-12, 1e-8, 1e-6),
+                                                  ("OptRosenbrock23", 1e-12, 1e-8, 1e-6),
                                                   ("OptSeulex", 1e-12, 1e-8, 1e-6)])
 def test_tight_tolerance_matches_radau(solver, atol, rtol, tolY):
     """Tight-tolerance integrations agree with an independent Radau solution."""
