@@ -37,7 +37,7 @@ constexpr int WAVE = 64;
 // instead of hitting one bank with every lane.
 constexpr int JS = 65;
 
-enum { RXN_ARR = 0, RXN_TB = 1, RXN_LIND = 2, RXN_TROE = 3, RXN_PLOG = 4 };
+enum { RXN_ARR = 0, RXN_TB = 1, RXN_LIND = 2, RXN_TROE = 3, RXN_PLOG = 4, RXN_SRI = 5 };
 enum { SOLVER_RODAS34 = 0, SOLVER_ROS34 = 1, SOLVER_SEULEX = 2, SOLVER_RODAS23 = 3, SOLVER_ROS23 = 4 };
 enum { CELL_SKIPPED = 1, CELL_MAXSTEPS = 2, CELL_NONFINITE = 4 };
 
@@ -57,7 +57,7 @@ struct DevMech {
     const double* clo;        // [7*64] coefficient-major: c[k*64 + s]
     const double* chi;        // [7*64]
     const int* rtype;         // [nr]
-    const int* rrev;          // [nr]
+    const int* rrev;          // [nr]  0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius in rA0/rbeta0/rTa0
     const int* rnl;           // [nr]
     const int* rnr;           // [nr]
     const int* rsp;           // [6*nr] slot-major: rsp[q*nr + r], q<3 lhs, q>=3 rhs
@@ -66,11 +66,12 @@ struct DevMech {
     const double* rA;         // [nr]  forward (k_inf)
     const double* rbeta;
     const double* rTa;
-    const double* rA0;        // [nr]  k_0 (falloff)
+    const double* rA0;        // [nr]  k_0 (falloff) or the reverse Arrhenius (nonEquilibrium)
     const double* rbeta0;
     const double* rTa0;
     const double* troe;       // [4*nTroe] (alpha, Tsss, Ts, Tss) per row
-    const double* rtroe;      // [4*nr] coefficient-major rtroe[q*nr + r], zero for non-Troe rows
+    const double* rtroe;      // [5*nr] coefficient-major rtroe[q*nr + r]: Troe (alpha, Tsss, Ts, Tss, 0)
+                              // or SRI (a, b, c, d, e); zero for other rows
     const double* eff;        // [nTB*64] row-major, zero beyond ns
     const double* effT;       // [64*nTBpad] species-major for M sums (zero padding)
     const int* plog_n;        // [nPlog]
@@ -361,6 +362,33 @@ __device__ inline RxnK rate_constant(const DevMech& M, const RxnData& D, int r, 
         k.kf = Mc * tmp * k0;
         return k;
     }
+    if (t == RXN_SRI) {
+        // SRI_F / SRI_F_dFdT_dFdPr (src/Reaction/pressureReactionFunctionI.H:
+        // 2-70), blended as ReactionRate.C:506-518 / Jacobian.C:681-711: log10
+        // of max(Pr, small) (OpenFOAM small), and a dF/dT without the
+        // dF/dPr dPr/dT chain term, as the reference has it.
+        const double a = D.alpha, b = D.Tsss, c = D.Ts, d = D.Tss;
+        const double e = M.rtroe[4 * M.nr + r];   // SRI rows only: loaded here
+        const double Pr = k0 * Mc / kinf;
+        const double logPr = log10(Pr > OF_SMALL ? Pr : OF_SMALL);
+        const double X = 1.0 / (1.0 + logPr * logPr);
+        const double eb = exp(-b * iT), ec = exp(-T / c);
+        const double psi = a * eb + ec;
+        const double F = d * pow(psi, X) * pow(T, e);
+        const double i1 = 1.0 / (1.0 + Pr);
+        if (JAC) {
+            const double dpsidT = a * b / (T * T) * eb - 1.0 / c * ec;
+            const double dFdT = F * (X / psi * dpsidT + e / T);
+            const double dlogPrdPr = Pr >= OF_SMALL ? 1.0 / (2.302585092994045684 * Pr) : 0.0;
+            const double dXdPr = -(X * X) * 2.0 * logPr * dlogPrdPr;
+            const double dFdPr = F * log(psi) * dXdPr;
+            const double dPrdT = (Mc * dk0 - Pr * dkinf) / kinf;
+            k.dkfdT = F * i1 * (Pr * dkinf) + F * i1 * i1 * dPrdT * kinf + k0 * i1 * dFdT * Mc;
+            k.dkfdM = k0 * i1 * (F * i1 + Pr * dFdPr);
+        }
+        k.kf = Mc * (i1 * F * k0);
+        return k;
+    }
     // Troe
     const double alpha = D.alpha, Tsss = D.Tsss, Ts = D.Ts, Tss = D.Tss;
     const double e1 = exp(-T / Tsss), e2 = exp(-Tss * iT), e3 = exp(-T / Ts);
@@ -461,8 +489,19 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
     const unsigned long long pk = D.pk;
     const int nl = pk_nl(pk), nrh = pk_nr(pk);
     const RxnK k = rate_constant<JAC>(M, D, r, T, lnT, iT, lnp);
-    double kr = 0.0, dkrdT = 0.0, invKc = 0.0;
-    if (D.rev) {
+    double kr = 0.0, dkrdT = 0.0, dkrdM = 0.0;
+    if (D.rev == 2) {
+        // nonEquilibrium: the reaction's own reverse Arrhenius (in the k_0
+        // slot), times M for third-body rows (ReactionFunctionI.H:20-23,
+        // ReactionRate.C:362-367; reverse efficiencies = forward ones,
+        // OptReaction.C:678)
+        const double kra = D.A0 * exp(D.b0 * lnT - D.Ta0 * iT);
+        const double dkra = JAC ? kra * (D.b0 + D.Ta0 * iT) * iT : 0.0;
+        const bool tb = D.t == RXN_TB;
+        kr = tb ? kra * D.Mc : kra;
+        dkrdT = tb ? dkra * D.Mc : dkra;
+        dkrdM = tb ? kra : 0.0;
+    } else if (D.rev) {
         double num = 1.0, den = 1.0, sdb = 0.0;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -477,9 +516,10 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
         double Kc = (num / den) * pw;
         const bool floored = !(Kc > KC_FLOOR);
         Kc = floored ? KC_FLOOR : Kc;
-        invKc = 1.0 / Kc;
+        const double invKc = 1.0 / Kc;
         kr = k.kf * invKc;
         if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * (sdb - dn * iT));
+        dkrdM = k.dkfdM * invKc;
     }
     double CF = 1.0, CR = 1.0;
 #pragma unroll
@@ -493,8 +533,8 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
         L.rq[r] = k.kf;
         L.rk[r] = kr;
         L.rdT[r] = k.dkfdT * CF - dkrdT * CR;
-        if (D.t == RXN_TB || D.t == RXN_LIND || D.t == RXN_TROE)
-            L.rw[D.tb] = k.dkfdM * CF - (k.dkfdM * invKc) * CR;
+        if (D.t == RXN_TB || D.t == RXN_LIND || D.t == RXN_TROE || D.t == RXN_SRI)
+            L.rw[D.tb] = k.dkfdM * CF - dkrdM * CR;
     }
 }
 

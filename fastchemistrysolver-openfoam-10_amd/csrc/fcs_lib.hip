@@ -608,7 +608,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     const int ns = d->n_species, nr = d->n_reactions;
     // ---- order reactions by rate form (file order within a form)
     std::vector<int> perm;
-    for (int t = 0; t <= FCS_RXN_PLOG; ++t)
+    for (int t = 0; t <= FCS_RXN_SRI; ++t)
         for (int r = 0; r < nr; ++r)
             if (d->rxn_type[r] == t) perm.push_back(r);
     if ((int)perm.size() != nr) return FCS_E_ARG;
@@ -639,14 +639,14 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     }
     std::vector<int> rtype(nr), rrev(nr), rnl(nr), rnr(nr), rsp(6 * std::max(nr, 1), 0), rtb(nr, -1), raux(nr, -1);
     std::vector<double> rA(nr), rbeta(nr), rTa(nr), rA0(nr), rbeta0(nr), rTa0(nr);
-    std::vector<double> troe, eff, plogLnp, plogLnA, plogBeta, plogTa, rtroe(4 * (size_t)std::max(nr, 1), 0.0);
+    std::vector<double> troe, eff, plogLnp, plogLnA, plogBeta, plogTa, rtroe(5 * (size_t)std::max(nr, 1), 0.0);
     std::vector<int> plogN;
     int nTB = 0;
     for (int r = 0; r < nr; ++r) {
         const int f = perm[r];
         const int t = d->rxn_type[f];
         rtype[r] = t;
-        rrev[r] = d->rxn_reversible[f] ? 1 : 0;
+        rrev[r] = d->rxn_reversible[f] == 2 ? 2 : (d->rxn_reversible[f] ? 1 : 0);
         rnl[r] = d->rxn_n_lhs[f];
         rnr[r] = d->rxn_n_rhs[f];
         for (int q = 0; q < 3; ++q) {
@@ -659,7 +659,16 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         rA0[r] = d->A0 ? d->A0[f] : 0.0;
         rbeta0[r] = d->beta0 ? d->beta0[f] : 0.0;
         rTa0[r] = d->Ta0 ? d->Ta0[f] : 0.0;
-        if (t == FCS_RXN_THIRD_BODY || t == FCS_RXN_LINDEMANN || t == FCS_RXN_TROE) {
+        if (rrev[r] == 2) {
+            // nonEquilibrium (Arrhenius and third-body forms only): the k_0
+            // slot, unused by these forms, carries the reverse Arrhenius
+            if ((t != FCS_RXN_ARRHENIUS && t != FCS_RXN_THIRD_BODY) || !d->A_rev || !d->beta_rev || !d->Ta_rev)
+                return FCS_E_ARG;
+            rA0[r] = d->A_rev[f];
+            rbeta0[r] = d->beta_rev[f];
+            rTa0[r] = d->Ta_rev[f];
+        }
+        if (t == FCS_RXN_THIRD_BODY || t == FCS_RXN_LINDEMANN || t == FCS_RXN_TROE || t == FCS_RXN_SRI) {
             if (!d->efficiencies) return FCS_E_ARG;
             rtb[r] = nTB++;
             for (int s = 0; s < 64; ++s) eff.push_back(s < ns ? d->efficiencies[(size_t)f * ns + s] : 0.0);
@@ -669,6 +678,10 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             raux[r] = (int)troe.size() / 4;
             for (int q = 0; q < 4; ++q) troe.push_back(d->troe[4 * (size_t)f + q]);
             for (int q = 0; q < 4; ++q) rtroe[(size_t)q * nr + r] = d->troe[4 * (size_t)f + q];
+        }
+        if (t == FCS_RXN_SRI) {
+            if (!d->sri) return FCS_E_ARG;
+            for (int q = 0; q < 5; ++q) rtroe[(size_t)q * nr + r] = d->sri[5 * (size_t)f + q];
         }
         if (t == FCS_RXN_PLOG) {
             const int np = d->plog_n[f];

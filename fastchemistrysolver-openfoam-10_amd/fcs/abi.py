@@ -31,6 +31,7 @@ class MechDesc(C.Structure):
         ("efficiencies", _dp), ("troe", _dp),
         ("plog_max_points", C.c_int32), ("plog_n", _ip), ("plog_p", _dp),
         ("plog_A", _dp), ("plog_beta", _dp), ("plog_Ta", _dp),
+        ("sri", _dp), ("A_rev", _dp), ("beta_rev", _dp), ("Ta_rev", _dp),
     ]
 
 

@@ -4,7 +4,8 @@ Reads the `reactions` and `speciesThermo` dictionaries the plug-in takes
 (the reaction forms OptReaction::readInfo accepts, src/Reaction/OptReaction.C:
 348-1175: (ir)reversibleArrhenius, (ir)reversibleThirdBodyArrhenius,
 (ir)reversibleArrheniusLindemannFallOff, (ir)reversibleArrheniusTroeFallOff,
-(ir)reversiblePlog) into a fcs.mechanism.Mechanism, and volScalarField files
+(ir)reversibleArrheniusSRIFallOff, (ir)reversiblePlog,
+nonEquilibriumReversible(ThirdBody)Arrhenius) into a fcs.mechanism.Mechanism, and volScalarField files
 (`internalField uniform v` or `nonuniform List<scalar> n (...)`) into arrays.
 
 The dictionary grammar handled is the subset these files use: `key value;`
@@ -18,8 +19,8 @@ import re
 
 import numpy as np
 
-from .mechanism import (RXN_ARRHENIUS, RXN_LINDEMANN, RXN_PLOG, RXN_THIRD_BODY, RXN_TROE, MAX_STOICH,
-                        Mechanism, Reaction)
+from .mechanism import (RXN_ARRHENIUS, RXN_LINDEMANN, RXN_PLOG, RXN_SRI, RXN_THIRD_BODY, RXN_TROE,
+                        MAX_STOICH, Mechanism, Reaction)
 
 _PUNCT = set("{};")
 
@@ -159,8 +160,10 @@ _FORMS = {
     "ThirdBodyArrhenius": RXN_THIRD_BODY,
     "ArrheniusLindemannFallOff": RXN_LINDEMANN,
     "ArrheniusTroeFallOff": RXN_TROE,
+    "ArrheniusSRIFallOff": RXN_SRI,
     "Plog": RXN_PLOG,
 }
+_NONEQ = "nonEquilibriumReversible"
 
 
 def _side(text, index):
@@ -215,25 +218,41 @@ def mechanism(reactions: dict, thermo: dict) -> Mechanism:
     rxns = []
     for key, r in reactions["reactions"].items():
         typ = r["type"]
+        noneq = typ.startswith(_NONEQ)
         rev = not typ.startswith("irreversible")
-        form = typ[len("irreversible"):] if typ.startswith("irreversible") else typ[len("reversible"):]
+        if noneq:
+            form = typ[len(_NONEQ):]
+            if form not in ("Arrhenius", "ThirdBodyArrhenius"):
+                raise ValueError(f"{key}: reaction type {typ!r} is not supported")
+        else:
+            form = typ[len("irreversible"):] if typ.startswith("irreversible") else typ[len("reversible"):]
         if form not in _FORMS:
             raise ValueError(f"{key}: reaction type {typ!r} is not supported")
         t = _FORMS[form]
         lhs_s, rhs_s = r["reaction"].split("=", 1)
         lhs, rhs = _side(lhs_s, index), _side(rhs_s, index)
         rx = Reaction(t, rev, lhs, rhs, 0.0, 0.0, 0.0)
-        if t in (RXN_ARRHENIUS, RXN_THIRD_BODY):
+        if noneq:
+            # forward / reverse sub-dictionaries; third-body efficiencies of
+            # both directions come from the forward one (OptReaction.C:617-700)
+            rx.A, rx.beta, rx.Ta = _arrhenius(r["forward"])
+            rx.rev_arr = _arrhenius(r["reverse"])
+            if t == RXN_THIRD_BODY:
+                rx.eff = _efficiencies(r["forward"]["coeffs"], index, ns)
+        elif t in (RXN_ARRHENIUS, RXN_THIRD_BODY):
             rx.A, rx.beta, rx.Ta = _arrhenius(r)
-        if t == RXN_THIRD_BODY:
+        if t == RXN_THIRD_BODY and not noneq:
             rx.eff = _efficiencies(r["coeffs"], index, ns)
-        if t in (RXN_LINDEMANN, RXN_TROE):
+        if t in (RXN_LINDEMANN, RXN_TROE, RXN_SRI):
             rx.A, rx.beta, rx.Ta = _arrhenius(r["kInf"])
             rx.A0, rx.beta0, rx.Ta0 = _arrhenius(r["k0"])
             rx.eff = _efficiencies(r["thirdBodyEfficiencies"]["coeffs"], index, ns)
         if t == RXN_TROE:
             F = r["F"]
             rx.troe = (float(F["alpha"]), float(F["Tsss"]), float(F["Ts"]), float(F["Tss"]))
+        if t == RXN_SRI:
+            F = r["F"]
+            rx.sri = tuple(float(F[k]) for k in "abcde")
         if t == RXN_PLOG:
             rows = r["ArrheniusData"] if "ArrheniusData" in r else r["coeffs"]
             p = [float(row[0]) for row in rows]

@@ -24,7 +24,9 @@ import numpy as np
 RU = 8314.470066505449722171761095523834228515625
 PSTD = 1.0e5
 
-RXN_ARRHENIUS, RXN_THIRD_BODY, RXN_LINDEMANN, RXN_TROE, RXN_PLOG = 0, 1, 2, 3, 4
+RXN_ARRHENIUS, RXN_THIRD_BODY, RXN_LINDEMANN, RXN_TROE, RXN_PLOG, RXN_SRI = 0, 1, 2, 3, 4, 5
+FALLOFF = (RXN_LINDEMANN, RXN_TROE, RXN_SRI)
+USES_M = (RXN_THIRD_BODY,) + FALLOFF
 MAX_STOICH = 3
 ELEM_MASS = np.array([1.00794, 12.0107, 15.9994, 14.0067])  # H C O N [kg/kmol]
 
@@ -58,6 +60,8 @@ class Reaction:
     eff: np.ndarray | None = None
     troe: tuple = (0.0, 0.0, 0.0, 0.0)
     plog: tuple = ()      # (p[], A[], beta[], Ta[])
+    sri: tuple = ()       # (a, b, c, d, e) of an SRI falloff
+    rev_arr: tuple = ()   # (A, beta, Ta) of the reverse rate of a nonEquilibrium reaction
 
 
 @dataclass
@@ -100,10 +104,15 @@ class Mechanism:
             efficiencies=np.ones(nr * ns), troe=np.zeros(nr * 4),
             plog_n=np.zeros(nr, np.int32), plog_p=np.zeros(nr * pmax),
             plog_A=np.zeros(nr * pmax), plog_beta=np.zeros(nr * pmax), plog_Ta=np.zeros(nr * pmax),
+            sri=np.zeros(nr * 5), A_rev=np.zeros(nr), beta_rev=np.zeros(nr), Ta_rev=np.zeros(nr),
         )
         for r, rx in enumerate(self.reactions):
             a["rxn_type"][r] = rx.rtype
-            a["rxn_reversible"][r] = int(rx.reversible)
+            a["rxn_reversible"][r] = 2 if rx.rev_arr else int(rx.reversible)
+            if rx.rev_arr:
+                a["A_rev"][r], a["beta_rev"][r], a["Ta_rev"][r] = rx.rev_arr
+            if rx.rtype == RXN_SRI:
+                a["sri"][r * 5:r * 5 + 5] = rx.sri
             a["rxn_n_lhs"][r] = len(rx.lhs)
             a["rxn_n_rhs"][r] = len(rx.rhs)
             a["rxn_lhs"][r * 3:r * 3 + len(rx.lhs)] = rx.lhs
@@ -136,8 +145,12 @@ class Mechanism:
                          [int(v) for v in a["rxn_rhs"][r * 3:r * 3 + int(a["rxn_n_rhs"][r])]],
                          float(a["A"][r]), float(a["beta"][r]), float(a["Ta"][r]),
                          float(a["A0"][r]), float(a["beta0"][r]), float(a["Ta0"][r]))
-            if t in (RXN_THIRD_BODY, RXN_LINDEMANN, RXN_TROE):
+            if t in USES_M:
                 x.eff = np.array(a["efficiencies"][r * ns:(r + 1) * ns], np.float64)
+            if "A_rev" in a and int(a["rxn_reversible"][r]) == 2:
+                x.rev_arr = (float(a["A_rev"][r]), float(a["beta_rev"][r]), float(a["Ta_rev"][r]))
+            if t == RXN_SRI:
+                x.sri = tuple(float(v) for v in a["sri"][r * 5:r * 5 + 5])
             x.troe = tuple(float(v) for v in a["troe"][r * 4:r * 4 + 4])
             if t == RXN_PLOG:
                 k = int(a["plog_n"][r])
@@ -285,13 +298,18 @@ def _arrhenius(rng, order):
 
 
 def synthetic_mechanism(n_species, n_reactions, n_third_body=0, n_troe=0, n_plog=0,
-                        seed=1000, n_lindemann=0) -> Mechanism:
+                        seed=1000, n_lindemann=0, n_sri=0, n_noneq=0) -> Mechanism:
     """Seeded synthetic mechanism as specified for the benchmark configurations:
     1-2 reactants and 1-2 products, reversible through Kc from the NASA-7 fits,
     A log-uniform in [1e6, 1e14], beta U[-1.5, 1.5], Ta U[0, 25000] K,
     third-body efficiencies U[0.5, 3], Troe alpha U[0.3, 0.9], T*** U[50, 300],
     T* U[500, 3000], T** U[3000, 8000] with k0 = 1e3 k_inf, PLOG with 5
-    pressures log-spaced over 1e4..1e8 Pa."""
+    pressures log-spaced over 1e4..1e8 Pa.  Optionally n_sri SRI falloff
+    reactions (a U[0.3, 1.5], b U[100, 1000], c U[500, 3000], d U[0.8, 1.2],
+    e U[-0.3, 0.3]) and n_noneq Arrhenius / third-body reactions turned
+    nonEquilibrium (reverse Arrhenius A 10^U[-1, 1] times the forward A, the
+    forward beta, Ta U[0.5, 1.5] times the forward Ta); with both zero the
+    mechanism is the same as without these options."""
     rng = np.random.default_rng(seed)
     for attempt in range(200):
         comps = _species_set(rng, n_species)
@@ -334,9 +352,9 @@ def synthetic_mechanism(n_species, n_reactions, n_third_body=0, n_troe=0, n_plog
             chosen.append(by_class[k].pop())
         elif not any(by_class.values()):
             break
-    n_fall = n_troe + n_lindemann
+    n_fall = n_troe + n_lindemann + n_sri
     types = ([RXN_THIRD_BODY] * n_third_body + [RXN_TROE] * n_troe
-             + [RXN_LINDEMANN] * n_lindemann + [RXN_PLOG] * n_plog)
+             + [RXN_LINDEMANN] * n_lindemann + [RXN_PLOG] * n_plog + [RXN_SRI] * n_sri)
     types += [RXN_ARRHENIUS] * (n_reactions - len(types))
     types = list(rng.permutation(types))
     assert n_fall + n_third_body + n_plog <= n_reactions
@@ -345,14 +363,17 @@ def synthetic_mechanism(n_species, n_reactions, n_third_body=0, n_troe=0, n_plog
         order = len(lhs) + (1 if t == RXN_THIRD_BODY else 0)
         A, b, Ta = _arrhenius(rng, order)
         rx = Reaction(rtype=t, reversible=True, lhs=list(lhs), rhs=list(rhs), A=A, beta=b, Ta=Ta)
-        if t in (RXN_THIRD_BODY, RXN_TROE, RXN_LINDEMANN):
+        if t in USES_M:
             rx.eff = rng.uniform(0.5, 3.0, n_species)
-        if t in (RXN_TROE, RXN_LINDEMANN):
+        if t in FALLOFF:
             # k0 = 1e3 k_inf in CHEMKIN units; one order higher (the M factor)
             rx.A0, rx.beta0, rx.Ta0 = A * 1e3 * 1e-3, b, Ta
         if t == RXN_TROE:
             rx.troe = (rng.uniform(0.3, 0.9), rng.uniform(50, 300), rng.uniform(500, 3000),
                        rng.uniform(3000, 8000))
+        if t == RXN_SRI:
+            rx.sri = (rng.uniform(0.3, 1.5), rng.uniform(100, 1000), rng.uniform(500, 3000),
+                      rng.uniform(0.8, 1.2), rng.uniform(-0.3, 0.3))
         if t == RXN_PLOG:
             p = np.logspace(4, 8, 5)
             pA = A * 10 ** rng.uniform(-1.0, 1.0, 5)
@@ -360,6 +381,12 @@ def synthetic_mechanism(n_species, n_reactions, n_third_body=0, n_troe=0, n_plog
             pT = np.clip(Ta + rng.uniform(-1000, 1000, 5), 0.0, None)
             rx.plog = (p, pA, pb, pT)
         mech.reactions.append(rx)
+    if n_noneq:
+        rng2 = np.random.default_rng(seed + 7)
+        pool = [r for r in mech.reactions if r.rtype in (RXN_ARRHENIUS, RXN_THIRD_BODY)]
+        for k in rng2.choice(len(pool), n_noneq, replace=False):
+            r = pool[int(k)]
+            r.rev_arr = (r.A * 10 ** rng2.uniform(-1.0, 1.0), r.beta, r.Ta * rng2.uniform(0.5, 1.5))
     return mech
 
 

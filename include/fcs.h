@@ -52,6 +52,7 @@ extern "C" {
 #define FCS_RXN_LINDEMANN   2   /* (ir)reversibleArrheniusLindemannFallOff      */
 #define FCS_RXN_TROE        3   /* (ir)reversibleArrheniusTroeFallOff           */
 #define FCS_RXN_PLOG        4   /* (ir)reversiblePlog (log-p interpolation)     */
+#define FCS_RXN_SRI         5   /* (ir)reversibleArrheniusSRIFallOff            */
 
 /* ---- ODE solvers (chemistryType.solver) -------------------------------- */
 #define FCS_SOLVER_RODAS34        0   /* "OptRodas34"       */
@@ -76,6 +77,12 @@ extern "C" {
  * third-body and falloff reactions only (may be NULL when there are none).
  * troe is [n_reactions x 4] = (alpha, Tsss, Ts, Tss) read for TROE rows.
  * PLOG rows use plog_n[r] points stored at [r*plog_max_points + k].
+ * rxn_reversible: 0 irreversible, 1 reverse rate kf/Kc, 2 nonEquilibrium:
+ * the reverse rate is its own Arrhenius (A_rev, beta_rev, Ta_rev), times the
+ * same third-body concentration for third-body rows
+ * (nonEquilibriumReversible[ThirdBody]Arrhenius, OptReaction.C:617-700).
+ * sri is [n_reactions x 5] = (a, b, c, d, e) read for SRI rows; sri, A_rev,
+ * beta_rev and Ta_rev may be NULL when no row needs them.
  */
 typedef struct fcs_mech_desc {
     int32_t n_species;
@@ -106,6 +113,10 @@ typedef struct fcs_mech_desc {
     const double* plog_A;
     const double* plog_beta;
     const double* plog_Ta;
+    const double* sri;          /* [nr*5]                                 */
+    const double* A_rev;        /* [nr] reverse Arrhenius (rxn_reversible == 2) */
+    const double* beta_rev;
+    const double* Ta_rev;
 } fcs_mech_desc;
 
 /* Solver options: the chemistryProperties keys of the reference. */

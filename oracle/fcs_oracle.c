@@ -57,7 +57,7 @@ struct orc_mech {
     double *W, *invW, *Tc, *clo, *chi;
     double TlowMin, ThighMax, TcMin, TcMax;
     int32_t *type, *rev, *nl, *nrh, *lhs, *rhs;
-    double *A, *beta, *Ta, *A0, *beta0, *Ta0, *eff, *troe;
+    double *A, *beta, *Ta, *A0, *beta0, *Ta0, *eff, *troe, *sri, *Arev, *brev, *Tarev;
     int pmax;
     int32_t* plog_n;
     double *plog_p, *plog_lnA, *plog_beta, *plog_Ta;
@@ -106,6 +106,8 @@ orc_mech* orc_mech_create(const fcs_mech_desc* d)
     m->A0 = dup_d(d->A0, nr); m->beta0 = dup_d(d->beta0, nr); m->Ta0 = dup_d(d->Ta0, nr);
     m->eff = dup_d(d->efficiencies, (size_t)nr * ns);
     m->troe = dup_d(d->troe, 4 * (size_t)nr);
+    m->sri = dup_d(d->sri, 5 * (size_t)nr);
+    m->Arev = dup_d(d->A_rev, nr); m->brev = dup_d(d->beta_rev, nr); m->Tarev = dup_d(d->Ta_rev, nr);
     m->pmax = d->plog_max_points > 0 ? d->plog_max_points : 1;
     m->plog_n = dup_i(d->plog_n, nr);
     size_t np = (size_t)nr * m->pmax;
@@ -124,7 +126,8 @@ void orc_mech_destroy(orc_mech* m)
     free(m->W); free(m->invW); free(m->Tc); free(m->clo); free(m->chi);
     free(m->type); free(m->rev); free(m->nl); free(m->nrh); free(m->lhs); free(m->rhs);
     free(m->A); free(m->beta); free(m->Ta); free(m->A0); free(m->beta0); free(m->Ta0);
-    free(m->eff); free(m->troe); free(m->plog_n); free(m->plog_p); free(m->plog_lnA);
+    free(m->eff); free(m->troe); free(m->sri); free(m->Arev); free(m->brev); free(m->Tarev);
+    free(m->plog_n); free(m->plog_p); free(m->plog_lnA);
     free(m->plog_beta); free(m->plog_Ta);
     free(m);
 }
@@ -176,11 +179,11 @@ void orc_thermo(const orc_mech* m, double T, double* cp, double* ha,
  *   PLOG:                    ReactionRate.C:577-672, Jacobian.C:798-910
  * ---------------------------------------------------------------------- */
 static double rate_k(const orc_mech* m, int r, double T, double lnT, double p,
-                     const double* C, int jac, double* dkfdT, double* dkfdM)
+                     const double* C, int jac, double* dkfdT, double* dkfdM, double* Mout)
 {
     const double iT = 1.0 / T;
     const int t = m->type[r];
-    *dkfdT = 0.0; *dkfdM = 0.0;
+    *dkfdT = 0.0; *dkfdM = 0.0; *Mout = 0.0;
     if (t == FCS_RXN_PLOG) {
         const int np = m->plog_n[r];
         const double* P = m->plog_p + (size_t)r * m->pmax;
@@ -213,6 +216,7 @@ static double rate_k(const orc_mech* m, int r, double T, double lnT, double p,
     double M = 0.0;
     const double* e = m->eff + (size_t)r * m->ns;
     for (int j = 0; j < m->ns; ++j) M += e[j] * C[j];
+    *Mout = M;
     if (t == FCS_RXN_THIRD_BODY) {
         *dkfdM = k;
         *dkfdT = dk * M;
@@ -231,6 +235,33 @@ static double rate_k(const orc_mech* m, int r, double T, double lnT, double p,
             *dkfdM = k0 * tmp * tmp;
         }
         return M * tmp * k0;
+    }
+    if (t == FCS_RXN_SRI) {
+        /* SRI_F / SRI_F_dFdT_dFdPr (src/Reaction/pressureReactionFunctionI.H:2-70)
+         * with the falloff blend of ReactionRate.C:506-518 and Jacobian.C:681-711.
+         * log10 Pr is taken of max(Pr, small) with OpenFOAM's small; the
+         * Jacobian's dF/dT is the explicit derivative only (no dF/dPr dPr/dT
+         * chain term, as in Jacobian.C:699-708). */
+        const double* q = m->sri + 5 * (size_t)r;
+        const double a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+        const double Pr = k0 * M / kinf;
+        const double logPr = log10(Pr > OF_SMALL ? Pr : OF_SMALL);
+        const double X = 1.0 / (1.0 + logPr * logPr);
+        const double eb = exp(-b * iT), ec = exp(-T / c);
+        const double psi = a * eb + ec;
+        const double F = d * pow(psi, X) * pow(T, e);
+        const double i1 = 1.0 / (1.0 + Pr);
+        if (jac) {
+            const double dpsidT = a * b / (T * T) * eb - 1.0 / c * ec;
+            const double dFdT = F * (X / psi * dpsidT + e / T);
+            const double dlogPrdPr = Pr >= OF_SMALL ? 1.0 / (log(10.0) * Pr) : 0.0;
+            const double dXdPr = -(X * X) * 2.0 * logPr * dlogPrdPr;
+            const double dFdPr = F * log(psi) * dXdPr;
+            const double dPrdT = (M * dk0 - Pr * dkinf) / kinf;
+            *dkfdT = F * i1 * (Pr * dkinf) + F * i1 * i1 * dPrdT * kinf + k0 * i1 * dFdT * M;
+            *dkfdM = k0 * i1 * (F * i1 + Pr * dFdPr);
+        }
+        return M * (i1 * F * k0);
     }
     /* Troe */
     const double* tr = m->troe + 4 * (size_t)r;
@@ -300,6 +331,41 @@ static double kc_of(const orc_mech* m, int r, const double* expG, const double* 
     return (num / den) * pw;
 }
 
+/* Reverse rate of reaction r with forward kf (dkfdT, dkfdM) and third-body
+ * concentration M: kf/Kc (floored Kc), or for nonEquilibrium reactions the
+ * reaction's own reverse Arrhenius, times M for third-body rows
+ * (ReactionFunctionI.H:20-23 reads Kf_[Ikf[9] + ...]; ReactionRate.C:362-367
+ * scales it by the reverse third-body sum, whose efficiencies OptReaction.C:678
+ * reads from the forward dictionary). */
+static double reverse_k(const orc_mech* m, int r, double kf, double dkfdT, double dkfdM, double M,
+                        const double* expG, const double* dBdT, double T, double lnT,
+                        double* dkrdT, double* dkrdM)
+{
+    *dkrdT = 0.0; *dkrdM = 0.0;
+    if (m->rev[r] == 2) {
+        const double iT = 1.0 / T;
+        const double k = m->Arev[r] * exp(m->brev[r] * lnT - m->Tarev[r] * iT);
+        const double dk = k * (m->brev[r] + m->Tarev[r] * iT) * iT;
+        if (m->type[r] == FCS_RXN_THIRD_BODY) {
+            *dkrdT = dk * M;
+            *dkrdM = k;
+            return k * M;
+        }
+        *dkrdT = dk;
+        return k;
+    }
+    if (!m->rev[r]) return 0.0;
+    double dlnKc;
+    double Kc = kc_of(m, r, expG, dBdT, T, &dlnKc);
+    const int floored = !(Kc > KC_FLOOR);
+    Kc = floored ? KC_FLOOR : Kc;
+    const double invKc = 1.0 / Kc;
+    const double kr = kf * invKc;
+    if (dBdT) *dkrdT = dkfdT * invKc - (floored ? 0.0 : kr * dlnKc);
+    *dkrdM = dkfdM * invKc;
+    return kr;
+}
+
 void orc_rates(const orc_mech* m, double p, double Tin, const double* C, double* omega)
 {
     const int ns = m->ns;
@@ -310,14 +376,9 @@ void orc_rates(const orc_mech* m, double p, double Tin, const double* C, double*
     for (int i = 0; i < ns; ++i) expG[i] = exp(negG[i]);
     for (int i = 0; i < ns; ++i) omega[i] = 0.0;
     for (int r = 0; r < m->nr; ++r) {
-        double dk, dm;
-        const double kf = rate_k(m, r, T, lnT, p, C, 0, &dk, &dm);
-        double kr = 0.0;
-        if (m->rev[r]) {
-            double Kc = kc_of(m, r, expG, NULL, T, NULL);
-            Kc = Kc > KC_FLOOR ? Kc : KC_FLOOR;
-            kr = kf / Kc;
-        }
+        double dk, dm, M, dkr, dmr;
+        const double kf = rate_k(m, r, T, lnT, p, C, 0, &dk, &dm, &M);
+        const double kr = reverse_k(m, r, kf, dk, dm, M, expG, NULL, T, lnT, &dkr, &dmr);
         double CF = 1.0, CR = 1.0;
         for (int s = 0; s < m->nl[r]; ++s) CF *= C[m->lhs[3 * r + s]];
         for (int s = 0; s < m->nrh[r]; ++s) CR *= C[m->rhs[3 * r + s]];
@@ -392,18 +453,9 @@ void orc_jacobian(const orc_mech* m, double p, double* YT, double* dYTdt,
     }
     memset(Jc, 0, (size_t)n * n * sizeof(double));
     for (int r = 0; r < m->nr; ++r) {
-        double dkfdT, dkfdM;
-        const double kf = rate_k(m, r, T, lnT, p, C, 1, &dkfdT, &dkfdM);
-        double kr = 0.0, dkrdT = 0.0, invKc = 0.0;
-        if (m->rev[r]) {
-            double dlnKc;
-            double Kc = kc_of(m, r, expG, dBdT, T, &dlnKc);
-            const int floored = !(Kc > KC_FLOOR);
-            Kc = floored ? KC_FLOOR : Kc;
-            invKc = 1.0 / Kc;
-            kr = kf * invKc;
-            dkrdT = dkfdT * invKc - (floored ? 0.0 : kr * dlnKc);
-        }
+        double dkfdT, dkfdM, M, dkrdT, dkrdM;
+        const double kf = rate_k(m, r, T, lnT, p, C, 1, &dkfdT, &dkfdM, &M);
+        const double kr = reverse_k(m, r, kf, dkfdT, dkfdM, M, expG, dBdT, T, lnT, &dkrdT, &dkrdM);
         const int nl = m->nl[r], nrr = m->nrh[r];
         const int32_t* L = m->lhs + 3 * r;
         const int32_t* R = m->rhs + 3 * r;
@@ -432,8 +484,7 @@ void orc_jacobian(const orc_mech* m, double p, double* YT, double* dYTdt,
             }
         }
         const int t = m->type[r];
-        if (t == FCS_RXN_THIRD_BODY || t == FCS_RXN_LINDEMANN || t == FCS_RXN_TROE) {
-            const double dkrdM = dkfdM * invKc;
+        if (t == FCS_RXN_THIRD_BODY || t == FCS_RXN_LINDEMANN || t == FCS_RXN_TROE || t == FCS_RXN_SRI) {
             const double w = dkfdM * CF - dkrdM * CR;
             const double* e = m->eff + (size_t)r * ns;
             for (int j = 0; j < ns; ++j) {

@@ -314,3 +314,36 @@ def test_autotuned_launch_shape_keeps_results(fcslib, mech0):
     r1 = b.download()
     for k in ("RR", "deltaTChem", "Tend", "status", "nAttempts"):
         np.testing.assert_array_equal(r1[k], r0[k])
+
+
+# ------------------------------------------------- SRI and nonEquilibrium
+def sri_noneq_mech():
+    """12 species, 48 reactions: 4 third-body, 2 Troe, 4 SRI falloff, 6
+    nonEquilibrium (explicit reverse Arrhenius) reactions."""
+    return M.synthetic_mechanism(12, 48, n_third_body=4, n_troe=2, n_sri=4, n_noneq=6, seed=4242)
+
+
+def test_sri_noneq_kernels_and_solve(fcslib):
+    mech = sri_noneq_mech()
+    assert sum(r.rtype == M.RXN_SRI for r in mech.reactions) == 4
+    assert sum(bool(r.rev_arr) for r in mech.reactions) == 6
+    o = Oracle(mech)
+    fld = M.uniform_field(mech, 128, seed=99)
+    YT = np.vstack([fld.Y, fld.T[None, :]])
+    s = fcslib.Solver(fcslib.Mech(mech))
+    d, _ = s.derivatives(fld.p, YT)
+    f, J, _ = s.jacobian(fld.p, YT)
+    for c in range(0, 128, 4):
+        dref, _ = o.derivatives(fld.p[c], YT[:, c])
+        scale = np.abs(dref[:-1]).max()
+        assert np.abs(d[:-1, c] - dref[:-1]).max() <= 1e-11 * scale
+        assert d[-1, c] == pytest.approx(dref[-1], rel=1e-10, abs=1e-11 * scale)
+        fr, Jr, _ = o.jacobian(fld.p[c], YT[:, c], "exact")
+        np.testing.assert_allclose(f[:, c], fr, rtol=1e-10, atol=1e-11 * np.abs(fr[:-1]).max())
+        rs = np.abs(Jr).max(axis=1, keepdims=True) + 1e-300
+        assert (np.abs(J[c] - Jr) / rs).max() <= 1e-9
+    dt = 1e-5
+    g = s.solve(fld, dt)
+    r = Oracle(mech).solve(fld, dt, threads=1)
+    assert g["rc"] == 0 and r["rc"] == 0
+    assert_solve_parity(g, r, fld, dt)
