@@ -25,6 +25,7 @@ namespace fcs {
 constexpr double RU = 8314.470066505449722171761095523834228515625;  // OptReaction.H:404
 constexpr double PSTD = 1.0e5;                                         // OptReaction.H:409
 constexpr double KC_FLOOR = 1.4901171103413047e-8;                     // ReactionFunctionI.H:17
+constexpr double TC_KC_FLOOR = 1.49011611938476e-08;                   // Tc.C:599
 constexpr double TROE_SMALL = 2.2e-16;                                 // ReactionRate.C:475
 constexpr double PIVOT_EPS = 2.2204460492503131e-16;                   // LUsolver.C:155
 constexpr double OF_SMALL = 1.0e-15;    // OpenFOAM-10 doubleScalarSmall
@@ -89,6 +90,7 @@ struct DevMech {
     const unsigned long long* sp_pack;  // [maxDeg*64] rpack of each per-species list entry
     int maxTerms;             // per-species Jacobian term list length, multiple of 16
     const unsigned long long* sp_term;  // [maxTerms*64] lane-interleaved terms, see jterm_*
+    const double* Hf;         // [64] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
 };
 
 // Jacobian term of species row i: nu_ir * d q_r / d C_j for one reactant
@@ -573,6 +575,68 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
     }
 }
 
+// Molar production rate of this lane's species from the rates q_r in
+// L.rq: sum over its reaction list of nu_ir q_r, in list order.
+__device__ __forceinline__ double production_rate(const DevMech& M, const Lds& L)
+{
+    const int lane = lane_id();
+    double om = 0.0;
+    for (int e0 = 0; e0 < M.maxDeg; e0 += 16) {   // unconditional batches of 16 (maxDeg % 16 == 0)
+        int w[16];
+        double q[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) w[u] = M.sp_rn[(e0 + u) * WAVE + lane];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) q[u] = L.rq[rn_r(w[u])];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int nu = rn_nu(w[u]);
+            if (nu != 0) om += (double)nu * q[u];
+        }
+    }
+    return om;
+}
+
+// Chemical time scale sums of OptReaction::Tc (src/Reaction/Tc.C:566-636,
+// 752-832): per reaction the forward and reverse molar rates, each times the
+// number of species slots it produces, enter sumW and (squared)
+// sumWRateByCTot.  The reverse rate is kf/Kc with the floor Tc.C uses
+// (1.49011611938476e-8) for Kc-reversible reactions and zero otherwise
+// (Tc.C:583 tests isIrreversible == 0, so nonEquilibrium reactions count
+// as irreversible there).  L.C, L.expG and L.Mtb must be complete.
+// Returns (sumW, sumWRateByCTot) in every lane.
+__device__ inline double2 time_scale_sums(const DevMech& M, const Lds& L, double T, double lnT, double iT,
+                                          double lnp)
+{
+    const double PbyRT = PSTD / (RU * T);
+    double sw = 0.0, sw2 = 0.0;
+    for (int r = lane_id(); r < M.nr; r += WAVE) {
+        const RxnData D = load_rxn(M, L, r);
+        const RxnK k = rate_constant<false>(M, D, r, T, lnT, iT, lnp);
+        const int nl = pk_nl(D.pk), nrh = pk_nr(D.pk);
+        double CF = 1.0, CR = 1.0, num = 1.0, den = 1.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a < nl) { CF *= L.C[pk_species(D.pk, a)]; den *= L.expG[pk_species(D.pk, a)]; }
+            if (a < nrh) { CR *= L.C[pk_species(D.pk, 3 + a)]; num *= L.expG[pk_species(D.pk, 3 + a)]; }
+        }
+        double kr = 0.0;
+        if (D.rev == 1) {
+            const int dn = nrh - nl;
+            double pw = 1.0;
+            if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
+            if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
+            double Kc = (num / den) * pw;
+            Kc = Kc > TC_KC_FLOOR ? Kc : TC_KC_FLOOR;
+            kr = k.kf / Kc;
+        }
+        const double wf = (double)nrh * (k.kf * CF), wr = (double)nl * (kr * CR);
+        sw += wf + wr;
+        sw2 += wf * wf + wr * wr;
+    }
+    return make_double2(wave_sum(sw), wave_sum(sw2));
+}
+
 // ---------------------------------------------------------------- RHS
 // FastChemistryModel::derivatives (src/ChemistryModel/FastChemistryModel.C:
 // 194-273, 340-417): y (this lane's element) is clipped in place for species
@@ -599,22 +663,7 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
     wave_sync();
     reaction_round<false>(M, L, Tc, lnT, iT, lnp);
     wave_sync();
-    double om = 0.0;
-    if (sp) {
-        for (int e0 = 0; e0 < M.maxDeg; e0 += 16) {   // unconditional batches of 16 (maxDeg % 16 == 0)
-            int w[16];
-            double q[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) w[u] = M.sp_rn[(e0 + u) * WAVE + lane];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) q[u] = L.rq[rn_r(w[u])];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int nu = rn_nu(w[u]);
-                if (nu != 0) om += (double)nu * q[u];
-            }
-        }
-    }
+    const double om = sp ? production_rate(M, L) : 0.0;
     const double Yd = sp ? om * (S.W * invrho) : 0.0;
     const double cpM = wave_sum(sp ? y * th.cp : 0.0);
     const double hY = wave_sum(sp ? th.ha * Yd : 0.0);

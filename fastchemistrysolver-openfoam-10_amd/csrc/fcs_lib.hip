@@ -381,6 +381,76 @@ __global__ __launch_bounds__(64) void k_lu_solve(DevMech M, int nSys, const doub
     if (ext >= 0) b[(size_t)s * n + ext] = x;
 }
 
+// Reaction-rate state of one cell from (rho, T, p, Y) as the reference's
+// calculate() and tc() form it: c_i = rho Y_i / W_i without clipping,
+// thermo and rate constants at T clamped to the fit range.  Leaves C, expG
+// and Mtb in LDS; returns this lane's concentration (0 beyond ns).
+template <int NP>
+__device__ inline double cell_rate_state(const DevMech& M, const Lds& L, int nc, int c, const double* rho,
+                                         const double* T, const double* Y, double& Tcl, double& lnT, double& iT)
+{
+    const int lane = lane_id();
+    const bool sp = lane < M.ns;
+    const SpeciesData S = load_species(M, lane);
+    const double Cl = sp ? rho[c] * Y[(size_t)lane * nc + c] / S.W : 0.0;
+    Tcl = clampT(M, T[c]);
+    lnT = log(Tcl);
+    iT = 1.0 / Tcl;
+    Thermo th{0, 0, 0, 0, 0};
+    if (sp) th = species_thermo(M, S, Tcl, lnT, iT);
+    L.C[lane] = Cl;
+    L.expG[lane] = sp ? exp(th.negG) : 0.0;
+    wave_sync();
+    third_body_sums(M, L);
+    wave_sync();
+    return Cl;
+}
+
+// FastChemistryModel::calculate (src/ChemistryModel/FastChemistryModel.C:
+// 700-740): RR_i = W_i dNdtByV_i(p, T, c), one wavefront per cell, fields
+// [ns][nc].
+template <int NP>
+__global__ __launch_bounds__(64) void k_calculate(DevMech M, int nc, const double* rho, const double* T,
+                                                  const double* p, const double* Y, double* RR)
+{
+    extern __shared__ double lds_raw[];
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const int c = blockIdx.x, lane = lane_id();
+    double Tcl, lnT, iT;
+    (void)cell_rate_state<NP>(M, L, nc, c, rho, T, Y, Tcl, lnT, iT);
+    reaction_round<false>(M, L, Tcl, lnT, iT, log(p[c]));
+    wave_sync();
+    if (lane < M.ns) RR[(size_t)lane * nc + c] = production_rate(M, L) * M.W[lane];
+}
+
+// FastChemistryModel::tc (src/ChemistryModel/FastChemistryModel.C:588-643)
+// with OptReaction::Tc (src/Reaction/Tc.C:4-834): tc = sumW /
+// sumWRateByCTot * sum(c), or vGreat when no reaction proceeds.
+template <int NP>
+__global__ __launch_bounds__(64) void k_tc(DevMech M, int nc, const double* rho, const double* T, const double* p,
+                                           const double* Y, double* tc)
+{
+    extern __shared__ double lds_raw[];
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const int c = blockIdx.x;
+    double Tcl, lnT, iT;
+    const double Cl = cell_rate_state<NP>(M, L, nc, c, rho, T, Y, Tcl, lnT, iT);
+    const double sumc = wave_sum(Cl);
+    const double2 sw = time_scale_sums(M, L, Tcl, lnT, iT, log(p[c]));
+    if (lane_id() == 0) tc[c] = sw.y == 0.0 ? OF_VGREAT : sw.x / sw.y * sumc;
+}
+
+// FastChemistryModel::Qdot (src/ChemistryModel/FastChemistryModel.C:
+// 646-675): Qdot = -sum_i Hf_i RR_i, species in order; one thread per cell.
+__global__ __launch_bounds__(256) void k_qdot(int nc, int ns, const double* Hf, const double* RR, double* Qdot)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double q = 0.0;
+    for (int i = 0; i < ns; ++i) q -= Hf[i] * RR[(size_t)i * nc + c];
+    Qdot[c] = q;
+}
+
 // ============================================================== host side
 namespace {
 
@@ -392,6 +462,8 @@ struct KernelSet {
     const void* jac;
     const void* lu;
     const void* attempt[5];     // indexed by FCS_SOLVER_* (none for Seulex)
+    const void* calculate;
+    const void* tc;
 };
 
 #define FCS_KSET(N)                                                                                        \
@@ -402,7 +474,8 @@ struct KernelSet {
      (const void*)k_verify<N>,                                                                             \
      (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>,                \
      {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>, nullptr,         \
-      (const void*)k_attempt<N, SOLVER_RODAS23>, (const void*)k_attempt<N, SOLVER_ROS23>}},
+      (const void*)k_attempt<N, SOLVER_RODAS23>, (const void*)k_attempt<N, SOLVER_ROS23>},                 \
+     (const void*)k_calculate<N>, (const void*)k_tc<N>},
 const KernelSet kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET)};
 #undef FCS_KSET
 
@@ -622,7 +695,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     }
     const int pmax = d->plog_max_points > 0 ? d->plog_max_points : 1;
     // ---- host images of every table
-    std::vector<double> W(64, 0.0), invW(64, 0.0), Tc(64, 1000.0), clo(7 * 64, 0.0), chi(7 * 64, 0.0);
+    std::vector<double> W(64, 0.0), invW(64, 0.0), Tc(64, 1000.0), clo(7 * 64, 0.0), chi(7 * 64, 0.0), Hf(64, 0.0);
     double TlowMin = d->T_low[0], ThighMax = d->T_high[0], TcMin = d->T_common[0], TcMax = d->T_common[0];
     for (int s = 0; s < ns; ++s) {
         W[s] = d->W[s];
@@ -636,6 +709,11 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         ThighMax = std::max(ThighMax, d->T_high[s]);
         TcMin = std::min(TcMin, d->T_common[s]);
         TcMax = std::max(TcMax, d->T_common[s]);
+        // janafThermo::Hf of OpenFOAM-10 (not in the reference tree): the
+        // low-range fit's enthalpy at Tstd = 298.15 K, per unit mass
+        const double* a = d->coeffs_low + 7 * s;
+        const double Ts = 298.15;
+        Hf[s] = (RU / d->W[s]) * (((((a[4] / 5 * Ts + a[3] / 4) * Ts + a[2] / 3) * Ts + a[1] / 2) * Ts + a[0]) * Ts + a[5]);
     }
     std::vector<int> rtype(nr), rrev(nr), rnl(nr), rnr(nr), rsp(6 * std::max(nr, 1), 0), rtb(nr, -1), raux(nr, -1);
     std::vector<double> rA(nr), rbeta(nr), rTa(nr), rA0(nr), rbeta0(nr), rTa0(nr);
@@ -754,7 +832,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     };
     auto vd = [&](const std::vector<double>& v) { return add(v.data(), v.size() * sizeof(double)); };
     auto vi = [&](const std::vector<int>& v) { return add(v.data(), v.size() * sizeof(int)); };
-    const size_t iW = vd(W), iInvW = vd(invW), iTc = vd(Tc), iClo = vd(clo), iChi = vd(chi);
+    const size_t iW = vd(W), iInvW = vd(invW), iTc = vd(Tc), iClo = vd(clo), iChi = vd(chi), iHf = vd(Hf);
     const size_t iType = vi(rtype), iRev = vi(rrev), iNl = vi(rnl), iNr = vi(rnr), iSp = vi(rsp), iTb = vi(rtb),
                  iAux = vi(raux);
     const size_t iA = vd(rA), iB = vd(rbeta), iTa = vd(rTa), iA0 = vd(rA0), iB0 = vd(rbeta0), iTa0 = vd(rTa0);
@@ -837,6 +915,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.Tc = (const double*)P(iTc);
     dm.clo = (const double*)P(iClo);
     dm.chi = (const double*)P(iChi);
+    dm.Hf = (const double*)P(iHf);
     dm.rtype = (const int*)P(iType);
     dm.rrev = (const int*)P(iRev);
     dm.rnl = (const int*)P(iNl);
@@ -913,7 +992,8 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
     }
     if (s->ldsBytes > 65536)
         for (const void* f : {ks->step[0], ks->step[1], ks->step[2], ks->step[3], ks->step[4], ks->deriv, ks->jac,
-                              ks->lu, ks->attempt[0], ks->attempt[1], ks->attempt[3], ks->attempt[4]})
+                              ks->lu, ks->attempt[0], ks->attempt[1], ks->attempt[3], ks->attempt[4], ks->calculate,
+                              ks->tc})
             (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->ldsBytes);
     if (solver_grid(s) != FCS_OK) {
         (void)hipStreamDestroy(s->stream);
@@ -1472,6 +1552,81 @@ int fcs_eval_attempt(fcs_solver* s, int64_t nS, const double* p, const double* d
         HIP_OK(hipMemcpy(YT, dy, nv * 8, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(YTnew, dn, nv * 8, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(err, de, nS * 8, hipMemcpyDeviceToHost));
+        return FCS_OK;
+    });
+}
+
+// Per-species host fields -> device [ns][nc] (rho, T, p first), one
+// wavefront per cell; out receives nOut rows of nc doubles.
+static int field_kernel(fcs_solver* s, const void* kern, int64_t nCells, const double* rho, const double* T,
+                        const double* p, const double* const* Y, int nOut, double* const* out)
+{
+    if (!s || nCells < 0 || nCells > 0x7fffffff || (nCells > 0 && (!rho || !T || !p || !Y || !out))) return FCS_E_ARG;
+    const int ns = s->mech->ns;
+    const size_t nc = (size_t)nCells;
+    if (nc == 0) return FCS_OK;
+    for (int i = 0; i < ns; ++i)
+        if (!Y[i]) return FCS_E_ARG;
+    for (int i = 0; i < nOut; ++i)
+        if (!out[i]) return FCS_E_ARG;
+    return with_device((3 + ns + nOut) * nc * 8, [&](char* base) -> int {
+        double* dRho = (double*)base;
+        double* dT = dRho + nc;
+        double* dp = dT + nc;
+        double* dY = dp + nc;
+        double* dOut = dY + (size_t)ns * nc;
+        hipStream_t st = s->stream;
+        HIP_OK(hipMemcpyAsync(dRho, rho, nc * 8, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(dT, T, nc * 8, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(dp, p, nc * 8, hipMemcpyHostToDevice, st));
+        for (int i = 0; i < ns; ++i)
+            HIP_OK(hipMemcpyAsync(dY + (size_t)i * nc, Y[i], nc * 8, hipMemcpyHostToDevice, st));
+        DevMech M = s->mech->dm;
+        int nc32 = (int)nc;
+        void* args[] = {&M, &nc32, &dRho, &dT, &dp, &dY, &dOut};
+        HIP_OK(hipLaunchKernel(kern, dim3((unsigned)nc), dim3(64), args, s->ldsBytes, st));
+        for (int i = 0; i < nOut; ++i)
+            HIP_OK(hipMemcpyAsync(out[i], dOut + (size_t)i * nc, nc * 8, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        return FCS_OK;
+    });
+}
+
+int fcs_calculate(fcs_solver* s, int64_t nCells, const double* rho, const double* T, const double* p,
+                  const double* const* Y, double* const* RR)
+{
+    if (!s) return FCS_E_ARG;
+    return field_kernel(s, s->ks->calculate, nCells, rho, T, p, Y, s->mech->ns, RR);
+}
+
+int fcs_tc(fcs_solver* s, int64_t nCells, const double* rho, const double* T, const double* p,
+           const double* const* Y, double* tc)
+{
+    if (!s) return FCS_E_ARG;
+    double* out[1] = {tc};
+    return field_kernel(s, s->ks->tc, nCells, rho, T, p, Y, 1, out);
+}
+
+int fcs_qdot(fcs_solver* s, int64_t nCells, const double* const* RR, double* Qdot)
+{
+    if (!s || nCells < 0 || nCells > 0x7fffffff || (nCells > 0 && (!RR || !Qdot))) return FCS_E_ARG;
+    const int ns = s->mech->ns;
+    const size_t nc = (size_t)nCells;
+    if (nc == 0) return FCS_OK;
+    for (int i = 0; i < ns; ++i)
+        if (!RR[i]) return FCS_E_ARG;
+    return with_device((ns + 1) * nc * 8, [&](char* base) -> int {
+        double* dRR = (double*)base;
+        double* dQ = dRR + (size_t)ns * nc;
+        hipStream_t st = s->stream;
+        for (int i = 0; i < ns; ++i)
+            HIP_OK(hipMemcpyAsync(dRR + (size_t)i * nc, RR[i], nc * 8, hipMemcpyHostToDevice, st));
+        const double* hf = s->mech->dm.Hf;
+        int nc32 = (int)nc, ns32 = ns;
+        void* args[] = {&nc32, &ns32, &hf, &dRR, &dQ};
+        HIP_OK(hipLaunchKernel((const void*)k_qdot, dim3((unsigned)((nc + 255) / 256)), dim3(256), args, 0, st));
+        HIP_OK(hipMemcpyAsync(Qdot, dQ, nc * 8, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
         return FCS_OK;
     });
 }

@@ -24,7 +24,7 @@ EXPORTS = [
     "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_download",
     "fcs_eval_derivatives", "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
     "fcs_batch_set_learned", "fcs_batch_learned_stats", "fcs_batch_first_steps", "fcs_batch_set_launch",
-    "fcs_batch_autotune",
+    "fcs_batch_autotune", "fcs_calculate", "fcs_tc", "fcs_qdot",
     "fcs_mlp_create", "fcs_mlp_destroy", "fcs_mlp_param_count", "fcs_mlp_get_params", "fcs_mlp_set_params",
     "fcs_mlp_set_norm", "fcs_mlp_forward", "fcs_mlp_loss_grad", "fcs_mlp_train",
 ]
@@ -170,6 +170,35 @@ class Solver:
         check(rc, "fcs_solve", allow=(FCS_E_CELLS,))
         return dict(rc=rc, RR=RR, deltaTChem=dtc, deltaTMin=float(dmin[0]), status=st, Tend=Tend,
                     nAttempts=na, rho0=rho0)
+
+    # ---- field evaluations at the current state (calculate, tc, Qdot)
+    def _state(self, rho, T, p, Y):
+        Y = np.ascontiguousarray(Y, np.float64).reshape(self.ns, -1)
+        nc = Y.shape[1]
+        arr = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (nc,))) for v in (rho, T, p)]
+        return nc, arr, Y
+
+    def calculate(self, rho, T, p, Y):
+        """RR [ns, nCells] in kg/m^3/s at the given state (fcs_calculate)."""
+        nc, (r, t, q), Y = self._state(rho, T, p, Y)
+        RR = np.zeros((self.ns, nc))
+        check(load().fcs_calculate(self._h, nc, dptr(r), dptr(t), dptr(q), row_pointers(Y), row_pointers(RR)),
+              "fcs_calculate")
+        return RR
+
+    def tc(self, rho, T, p, Y):
+        """Chemical time scale [nCells] in s (fcs_tc)."""
+        nc, (r, t, q), Y = self._state(rho, T, p, Y)
+        out = np.zeros(nc)
+        check(load().fcs_tc(self._h, nc, dptr(r), dptr(t), dptr(q), row_pointers(Y), dptr(out)), "fcs_tc")
+        return out
+
+    def qdot(self, RR):
+        """Heat release rate [nCells] in W/m^3 from RR [ns, nCells] (fcs_qdot)."""
+        RR = np.ascontiguousarray(RR, np.float64).reshape(self.ns, -1)
+        out = np.zeros(RR.shape[1])
+        check(load().fcs_qdot(self._h, RR.shape[1], row_pointers(RR), dptr(out)), "fcs_qdot")
+        return out
 
     # ---- kernel-level entries
     def derivatives(self, p, YT):

@@ -191,6 +191,28 @@ int fcs_solve_lts(fcs_solver* solver, int64_t nCells, const double* deltaT,
                   double* Tend, int32_t* nAttempts);
 
 /*
+ * Field evaluations at the current state (per-species host pointers as in
+ * fcs_solve; rho, T, p [nCells]; c_i = rho Y_i / W_i, not clipped).
+ *
+ * fcs_calculate: FastChemistryModel::calculate (src/ChemistryModel/
+ *   FastChemistryModel.C:700-740): RR[i] = W_i * dNdtByV_i(p, T, c).
+ * fcs_tc: FastChemistryModel::tc (FastChemistryModel.C:588-643) with
+ *   OptReaction::Tc (src/Reaction/Tc.C:4-834): chemical time scale
+ *   sumW / sumWRateByCTot * sum(c), 1e300 (vGreat) where nothing reacts.
+ * fcs_qdot: FastChemistryModel::Qdot (FastChemistryModel.C:646-675):
+ *   Qdot = -sum_i Hf_i RR[i], Hf_i the mass-specific enthalpy of formation
+ *   (low-range NASA-7 fit at 298.15 K, as OpenFOAM-10's janafThermo::Hf).
+ */
+int fcs_calculate(fcs_solver* solver, int64_t nCells, const double* rho,
+                  const double* T, const double* p, const double* const* Y,
+                  double* const* RR);
+int fcs_tc(fcs_solver* solver, int64_t nCells, const double* rho,
+           const double* T, const double* p, const double* const* Y,
+           double* tc);
+int fcs_qdot(fcs_solver* solver, int64_t nCells, const double* const* RR,
+             double* Qdot);
+
+/*
  * Device-resident batch: inputs uploaded once, integrated in place in HBM.
  * Layout in HBM is structure-of-arrays, species-major: Y[i*nCells + c].
  */

@@ -388,6 +388,80 @@ void orc_rates(const orc_mech* m, double p, double Tin, const double* C, double*
     }
 }
 
+/* FastChemistryModel::calculate (src/ChemistryModel/FastChemistryModel.C:
+ * 700-740): RR_i = W_i dNdtByV_i(p, T, c) with c_i = rho Y_i / W_i (no
+ * clipping).  Y and RR are [ns]. */
+void orc_calculate(const orc_mech* m, double rho, double T, double p, const double* Y, double* RR)
+{
+    const int ns = m->ns;
+    double* C = (double*)alloca(ns * sizeof(double));
+    for (int i = 0; i < ns; ++i) C[i] = rho * Y[i] / m->W[i];
+    orc_rates(m, p, T, C, RR);
+    for (int i = 0; i < ns; ++i) RR[i] *= m->W[i];
+}
+
+/* FastChemistryModel::tc (FastChemistryModel.C:588-643) with OptReaction::Tc
+ * (src/Reaction/Tc.C:4-834).  Per reaction: forward and reverse molar rates
+ * kf CF and kr CR, each summed once per product (forward) or reactant
+ * (reverse) slot, enter sumW and, squared, sumWRateByCTot.  kr = kf/Kc with
+ * the Kc floor of Tc.C:599 for Kc-reversible reactions, 0 otherwise (Tc.C:583
+ * tests isIrreversible == 0, so nonEquilibrium reactions count as
+ * irreversible).  tc = sumW / sumWRateByCTot * sum(c), vGreat if the
+ * denominator is 0. */
+#define TC_KC_FLOOR 1.49011611938476e-08
+double orc_tc(const orc_mech* m, double rho, double Tin, double p, const double* Y)
+{
+    const int ns = m->ns;
+    double* C = (double*)alloca(ns * sizeof(double));
+    double* negG = (double*)alloca(ns * sizeof(double));
+    double* expG = (double*)alloca(ns * sizeof(double));
+    double sumc = 0.0;
+    for (int i = 0; i < ns; ++i) { C[i] = rho * Y[i] / m->W[i]; sumc += C[i]; }
+    const double T = clampT(m, Tin), lnT = log(T);
+    orc_thermo(m, T, NULL, NULL, negG, NULL, NULL);
+    for (int i = 0; i < ns; ++i) expG[i] = exp(negG[i]);
+    double sumW = 0.0, sumW2 = 0.0;
+    for (int r = 0; r < m->nr; ++r) {
+        double dk, dm, M;
+        const double kf = rate_k(m, r, T, lnT, p, C, 0, &dk, &dm, &M);
+        double kr = 0.0;
+        if (m->rev[r] == 1) {
+            double Kc = kc_of(m, r, expG, NULL, T, NULL);
+            Kc = Kc > TC_KC_FLOOR ? Kc : TC_KC_FLOOR;
+            kr = kf / Kc;
+        }
+        double CF = 1.0, CR = 1.0;
+        for (int a = 0; a < m->nl[r]; ++a) CF *= C[m->lhs[3 * r + a]];
+        for (int a = 0; a < m->nrh[r]; ++a) CR *= C[m->rhs[3 * r + a]];
+        const double of = kf * CF, orv = kr * CR;
+        double wf = 0.0, wr = 0.0;
+        for (int a = 0; a < m->nrh[r]; ++a) wf += of;
+        for (int a = 0; a < m->nl[r]; ++a) wr += orv;
+        sumW += wf;
+        sumW2 += wf * wf;
+        sumW += wr;
+        sumW2 += wr * wr;
+    }
+    return sumW2 == 0.0 ? OF_VGREAT : sumW / sumW2 * sumc;
+}
+
+/* FastChemistryModel::Qdot (FastChemistryModel.C:646-675): -sum_i Hf_i RR_i
+ * with Hf_i OpenFOAM-10 janafThermo::Hf (low-range fit at Tstd = 298.15 K,
+ * mass-specific; not in the reference tree). */
+double orc_hf(const orc_mech* m, int i)
+{
+    const double* a = m->clo + 7 * (size_t)i;
+    const double Ts = 298.15;
+    return (RU / m->W[i]) * (((((a[4] / 5 * Ts + a[3] / 4) * Ts + a[2] / 3) * Ts + a[1] / 2) * Ts + a[0]) * Ts + a[5]);
+}
+
+double orc_qdot(const orc_mech* m, const double* RR)
+{
+    double q = 0.0;
+    for (int i = 0; i < m->ns; ++i) q -= orc_hf(m, i) * RR[i];
+    return q;
+}
+
 /* src/ChemistryModel/FastChemistryModel.C:340-417 (6-argument derivatives) */
 void orc_derivatives(const orc_mech* m, double p, double* YT, double* dYTdt)
 {

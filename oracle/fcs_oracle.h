@@ -29,6 +29,10 @@ void orc_thermo(const orc_mech* m, double T, double* cp, double* ha,
 
 /* Molar production rates omega [kmol/m^3/s] from C [kmol/m^3] (T clamped
  * to [min Tlow, max Thigh] inside, as the reference does). */
+void orc_calculate(const orc_mech* m, double rho, double T, double p, const double* Y, double* RR);
+double orc_tc(const orc_mech* m, double rho, double T, double p, const double* Y);
+double orc_hf(const orc_mech* m, int i);
+double orc_qdot(const orc_mech* m, const double* RR);
 void orc_rates(const orc_mech* m, double p, double T, const double* C,
                double* omega);
 

@@ -59,6 +59,13 @@ def lib():
         L.orc_rosenbrock_attempt.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_double, C.c_double,
                                              _dp, _dp]
         L.orc_rosenbrock_attempt.restype = C.c_double
+        L.orc_calculate.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.orc_tc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_tc.restype = C.c_double
+        L.orc_hf.argtypes = [C.c_void_p, C.c_int]
+        L.orc_hf.restype = C.c_double
+        L.orc_qdot.argtypes = [C.c_void_p, _dp]
+        L.orc_qdot.restype = C.c_double
         L.orc_solve.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, C.c_double,
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
         L.orc_solve_lts.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, _dp,
@@ -91,6 +98,24 @@ class Oracle:
         om = np.zeros(self.ns)
         lib().orc_rates(self._h, float(p), float(T), dptr(Cc), dptr(om))
         return om
+
+    def calculate(self, rho, T, p, Y):
+        """FastChemistryModel::calculate for one cell: RR [ns] (kg/m^3/s)."""
+        y = np.ascontiguousarray(Y, np.float64)
+        rr = np.zeros(self.ns)
+        lib().orc_calculate(self._h, float(rho), float(T), float(p), dptr(y), dptr(rr))
+        return rr
+
+    def tc(self, rho, T, p, Y):
+        """FastChemistryModel::tc for one cell (s)."""
+        return lib().orc_tc(self._h, float(rho), float(T), float(p), dptr(np.ascontiguousarray(Y, np.float64)))
+
+    def hf(self):
+        return np.array([lib().orc_hf(self._h, i) for i in range(self.ns)])
+
+    def qdot(self, RR):
+        """FastChemistryModel::Qdot for one cell (W/m^3)."""
+        return lib().orc_qdot(self._h, dptr(np.ascontiguousarray(RR, np.float64)))
 
     def derivatives(self, p, YT):
         y = np.array(YT, np.float64)

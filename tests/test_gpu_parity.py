@@ -347,3 +347,33 @@ def test_sri_noneq_kernels_and_solve(fcslib):
     r = Oracle(mech).solve(fld, dt, threads=1)
     assert g["rc"] == 0 and r["rc"] == 0
     assert_solve_parity(g, r, fld, dt)
+
+
+# ------------------------------------------------ calculate / tc / Qdot
+@pytest.mark.parametrize("which", ["cfg0", "cfg1", "sri_noneq"])
+def test_calculate_tc_qdot(fcslib, which):
+    """fcs_calculate, fcs_tc and fcs_qdot against the oracle; the states
+    include a negative mass fraction (no clipping in these paths) and a cold
+    cell below the fit range (clamped temperature)."""
+    if which == "sri_noneq":
+        mech = sri_noneq_mech()
+        fld = M.uniform_field(mech, 96, seed=5)
+    else:
+        cfg = int(which[-1])
+        mech = M.config_mechanism(cfg)
+        fld = M.config_field(cfg, mech, n_cells=96)
+    Y = fld.Y.copy()
+    Y[1, 3] = -1e-4
+    T = fld.T.copy()
+    T[5] = 150.0
+    rho = fld.p / (M.RU * T * (np.abs(Y) / mech.W[:, None]).sum(axis=0))
+    s = fcslib.Solver(fcslib.Mech(mech))
+    o = Oracle(mech)
+    RR = s.calculate(rho, T, fld.p, Y)
+    tc = s.tc(rho, T, fld.p, Y)
+    q = s.qdot(RR)
+    for c in range(96):
+        rr = o.calculate(rho[c], T[c], fld.p[c], Y[:, c])
+        np.testing.assert_allclose(RR[:, c], rr, rtol=1e-10, atol=1e-11 * np.abs(rr).max())
+        assert tc[c] == pytest.approx(o.tc(rho[c], T[c], fld.p[c], Y[:, c]), rel=1e-10)
+        assert q[c] == pytest.approx(o.qdot(RR[:, c]), rel=1e-12, abs=1e-12 * np.abs(o.hf() * RR[:, c]).sum())
