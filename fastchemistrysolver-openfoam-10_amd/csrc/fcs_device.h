@@ -1,8 +1,11 @@
 // fcs_device.h -- gfx950 device code of the per-cell stiff chemistry
-// integrator (one cell per 64-lane wavefront).
+// integrator: one cell per 64-lane wavefront for systems up to 64 rows
+// (matrix in LDS), one cell per workgroup of G = 64 * ceil(NP / 64) threads
+// beyond that (matrix in HBM, LU in fcs_lu_large.h).
 //
-// Layout of one cell inside a wavefront (NP = padded system size, a multiple
-// of 4, NP <= 64):
+// Layout of one cell (NP = padded system size, a multiple of 4; a multiple
+// of 16 for the workgroup form; thread = lane of the wavefront or thread of
+// the workgroup):
 //   lane s < ns        species s: state y[s] = Y_s, matrix row s in a[0..NP-1]
 //   ns <= lane < NP-1  padding: zero state, identity-like rows
 //   lane NP-1          temperature: y = T, matrix row of dT/dt
@@ -38,6 +41,18 @@ constexpr int WAVE = 64;
 // instead of hitting one bank with every lane.
 constexpr int JS = 65;
 
+// Geometry of one cell for a padded system size NP.
+template <int NP>
+struct Cell {
+    static constexpr bool kLarge = NP > 64;                       // workgroup per cell, matrix in HBM
+    static constexpr int G = kLarge ? (NP + 63) / 64 * 64 : 64;  // threads per cell
+    static constexpr int NW = G / 64;                             // wavefronts per cell
+    static constexpr int SW = G;                                  // species stride of per-species tables
+    static constexpr int ONE = kLarge ? 255 : 64;                 // concentration slot holding 1.0
+    static constexpr int CS = kLarge ? 256 : 65;                  // concentration slots
+    static constexpr int JSt = kLarge ? NP : JS;                  // matrix column stride
+};
+
 enum { RXN_ARR = 0, RXN_TB = 1, RXN_LIND = 2, RXN_TROE = 3, RXN_PLOG = 4, RXN_SRI = 5 };
 enum { SOLVER_RODAS34 = 0, SOLVER_ROS34 = 1, SOLVER_SEULEX = 2, SOLVER_RODAS23 = 3, SOLVER_ROS23 = 4 };
 enum { CELL_SKIPPED = 1, CELL_MAXSTEPS = 2, CELL_NONFINITE = 4 };
@@ -52,11 +67,13 @@ struct DevMech {
     int maxDeg;               // per-species list length, padded to a multiple of 16
     int nTBpad;               // row stride of effT: nTB padded to a multiple of 128
     double TlowMin, ThighMax, TcMin, TcMax;
-    const double* W;          // [64]  (0 beyond ns)
-    const double* invW;       // [64]  (0 beyond ns)
-    const double* Tc;         // [64]
-    const double* clo;        // [7*64] coefficient-major: c[k*64 + s]
-    const double* chi;        // [7*64]
+    // per-species tables are padded to SW = Cell<NP>::SW species (64, or the
+    // workgroup size): "[SW]" below
+    const double* W;          // [SW]  (0 beyond ns)
+    const double* invW;       // [SW]  (0 beyond ns)
+    const double* Tc;         // [SW]
+    const double* clo;        // [7*SW] coefficient-major: c[k*SW + s]
+    const double* chi;        // [7*SW]
     const int* rtype;         // [nr]
     const int* rrev;          // [nr]  0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius in rA0/rbeta0/rTa0
     const int* rnl;           // [nr]
@@ -73,24 +90,24 @@ struct DevMech {
     const double* troe;       // [4*nTroe] (alpha, Tsss, Ts, Tss) per row
     const double* rtroe;      // [5*nr] coefficient-major rtroe[q*nr + r]: Troe (alpha, Tsss, Ts, Tss, 0)
                               // or SRI (a, b, c, d, e); zero for other rows
-    const double* eff;        // [nTB*64] row-major, zero beyond ns
-    const double* effT;       // [64*nTBpad] species-major for M sums (zero padding)
+    const double* eff;        // [nTB*SW] row-major, zero beyond ns
+    const double* effT;       // [SW*nTBpad] species-major for M sums (zero padding)
     const int* plog_n;        // [nPlog]
     const double* plog_lnp;   // [nPlog*pmax]
     const double* plog_lnA;
     const double* plog_beta;
     const double* plog_Ta;
-    const int* sp_rn;         // [maxDeg*64] lane-interleaved: reaction | (nu + 8) << 16, nu = net
+    const int* sp_rn;         // [maxDeg*SW] thread-interleaved: reaction | (nu + 8) << 16, nu = net
                               // stoichiometric coefficient (padding: reaction 0, nu 0)
     int nTBz;                 // third-body/falloff (species, row) entries, padded to a multiple of 16
     const int* tbz_row;       // [nTBz] species, ascending (padding repeats the last)
     const int* tbz_tb;        // [nTBz] third-body row (padding: 0)
     const double* tbz_nu;     // [nTBz] net stoichiometric coefficient (padding: 0)
     const unsigned long long* rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
-    const unsigned long long* sp_pack;  // [maxDeg*64] rpack of each per-species list entry
+    const unsigned long long* sp_pack;  // [maxDeg*SW] rpack of each per-species list entry
     int maxTerms;             // per-species Jacobian term list length, multiple of 16
-    const unsigned long long* sp_term;  // [maxTerms*64] lane-interleaved terms, see jterm_*
-    const double* Hf;         // [64] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
+    const unsigned long long* sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*
+    const double* Hf;         // [SW] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
 };
 
 // Jacobian term of species row i: nu_ir * d q_r / d C_j for one reactant
@@ -174,34 +191,81 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 // of one row index across the lanes are conflict-free; the pivot rows of the
 // LU are read by all lanes at one address (broadcast).
 struct Lds {
-    double* C;     // [65] concentrations, C[64] = 1
-    double* expG;  // [64] exp(-G/RT)
-    double* dBdT;  // [64] d(-G/RT)/dT
+    double* C;     // [CS] concentrations, C[ONE] = 1
+    double* expG;  // [G] exp(-G/RT)
+    double* dBdT;  // [G] d(-G/RT)/dT
     double* rq;    // [nrP] q (rates) or kf (Jacobian)
     double* rk;    // [nrP] kr (Jacobian)
     double* rdT;   // [nrP] dq/dT (Jacobian)
-    double* H;     // [64] species enthalpy (T-row column sums), over rk once the
-    double* Q;     // [64] dY/dt * cp (T-row column sums)   sparse partials are in
+    double* H;     // [G] species enthalpy (T-row column sums), over rk once the
+    double* Q;     // [G] dY/dt * cp (T-row column sums)    sparse partials are in
     double* rw;    // [nTBP] third-body partial (Jacobian)
     double* Mtb;   // [nTBP] third-body concentration M of each third-body/falloff row
-    double* J;     // [NP*JS] Jacobian, then iteration matrix and its LU factors
-    int* ip;       // [64] lane holding each logical row after the LU
+    double* J;     // [NP*JSt] Jacobian, then iteration matrix and its LU factors
+                   // (LDS; for the workgroup form a per-workgroup slice in HBM
+                   // of NP*NP + NP doubles, the last NP holding 1/pivot)
+    int* ip;       // [G] thread holding each logical row after the LU
+    double* lu;    // workgroup form: LU / solve scratch over rq..Mtb (dead by then)
+    double* red;   // workgroup form: [16] cross-wavefront reduction / broadcast slots
 };
 
 __host__ __device__ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// LDS doubles per wavefront; four wavefronts of the 53-species mechanism fit
-// the 160 KiB of a CU (40,112 B each).
+// Panel width of the workgroup-form LU (columns per MFMA trailing update).
+constexpr int LU_PW = 16;
+// LU / solve scratch of the workgroup form (doubles): the 4 pivot rows of a
+// 4-block (double-buffered, 2 x 4 x LU_PW), the panel's unit-lower block
+// (LU_PW^2), -L21 and U12 tiles (2 x LU_PW x NP), 3 solve vectors of
+// LU_PW, the panel's logical-to-physical map (LU_PW ints).
+__host__ __device__ inline size_t lu_scratch_doubles(int NP)
+{
+    return 8 * LU_PW + LU_PW * LU_PW + 2 * (size_t)LU_PW * NP + 3 * LU_PW + LU_PW / 2 + 8;
+}
+
+__host__ __device__ inline int group_of(int NP) { return NP > 64 ? (NP + 63) / 64 * 64 : 64; }
+
+// LDS doubles per cell.  Wavefront form: four wavefronts of the 53-species
+// mechanism fit the 160 KiB of a CU (40,112 B each).  Workgroup form: the
+// matrix is in HBM; the LU scratch reuses the reaction arrays.
 __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
 {
     const size_t nrP = round_up(nr, 2);
-    const size_t kq = 2 * nrP > 128 ? 2 * nrP : 128;   // rk + rdT, later H + Q
-    return 3 * 64 + 2 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32;
+    if (NP <= 64) {
+        const size_t kq = 2 * nrP > 128 ? 2 * nrP : 128;   // rk + rdT, later H + Q
+        return 3 * 64 + 2 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32;
+    }
+    const size_t G = group_of(NP);
+    const size_t kq = 2 * nrP > 2 * G ? 2 * nrP : 2 * G;
+    const size_t react = nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2);
+    const size_t lu = lu_scratch_doubles(NP);
+    return 256 + 2 * G + (react > lu ? react : lu) + G / 2 + 16;
 }
 
-__device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
+__device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jglobal = nullptr)
 {
     Lds L;
+    if (NP > 64) {
+        const int G = group_of(NP);
+        const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
+        const int kq = 2 * nrP > 2 * G ? 2 * nrP : 2 * G;
+        const size_t react = nrP + kq + 2 * (size_t)nTBP;
+        const size_t lu = lu_scratch_doubles(NP);
+        L.C = base;
+        L.expG = base + 256;
+        L.dBdT = L.expG + G;
+        L.rq = L.dBdT + G;
+        L.rk = L.rq + nrP;
+        L.rdT = L.rk + nrP;
+        L.H = L.rk;
+        L.Q = L.rk + G;
+        L.rw = L.rk + kq;
+        L.Mtb = L.rw + nTBP;
+        L.lu = L.rq;
+        L.ip = (int*)(L.rq + (react > lu ? react : lu));
+        L.red = L.rq + (react > lu ? react : lu) + G / 2;
+        L.J = Jglobal;
+        return L;
+    }
     const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
     const int kq = 2 * nrP > 128 ? 2 * nrP : 128;
     L.C = base;              // C[64] = 1.0: the "no other species" slot of the Jacobian terms
@@ -216,7 +280,76 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB)
     L.Mtb = L.rw + nTBP;
     L.J = L.Mtb + nTBP;
     L.ip = (int*)(L.J + (size_t)NP * JS);
+    L.lu = nullptr;
+    L.red = nullptr;
     return L;
+}
+
+// ---------------------------------------------------------------- group ops
+// Thread index within the cell's group, and reductions / broadcasts over
+// the group: the wavefront primitives for the wavefront form, LDS slots and
+// barriers across the wavefronts of the workgroup form.  Every thread of the
+// group must call them (uniform control flow).
+template <int NP>
+__device__ __forceinline__ int tid()
+{
+    if constexpr (Cell<NP>::kLarge) return (int)threadIdx.x;
+    else return lane_id();
+}
+
+template <int NP>
+__device__ __forceinline__ double gsum(double v, const Lds& L)
+{
+    if constexpr (!Cell<NP>::kLarge) {
+        return wave_sum(v);
+    } else {
+        const double w = wave_sum(v);
+        __syncthreads();
+        if (lane_id() == 0) L.red[threadIdx.x >> 6] = w;
+        __syncthreads();
+        double r = L.red[0];
+#pragma unroll
+        for (int k = 1; k < Cell<NP>::NW; ++k) r += L.red[k];
+        return r;
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ double gmax(double v, const Lds& L)
+{
+    if constexpr (!Cell<NP>::kLarge) {
+        return wave_max(v);
+    } else {
+        const double w = wave_max(v);
+        __syncthreads();
+        if (lane_id() == 0) L.red[threadIdx.x >> 6] = w;
+        __syncthreads();
+        double r = L.red[0];
+#pragma unroll
+        for (int k = 1; k < Cell<NP>::NW; ++k) r = fmax(r, L.red[k]);
+        return r;
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ bool gany(bool p, const Lds& L)
+{
+    if constexpr (!Cell<NP>::kLarge) return __ballot(p) != 0ull;
+    else return gmax<NP>(p ? 1.0 : 0.0, L) > 0.0;
+}
+
+// Value of v held by thread src (uniform).
+template <int NP>
+__device__ __forceinline__ double gbcast(double v, int src, const Lds& L)
+{
+    if constexpr (!Cell<NP>::kLarge) {
+        return readlane_d(v, src);
+    } else {
+        __syncthreads();
+        if ((int)threadIdx.x == src) L.red[8] = v;
+        __syncthreads();
+        return L.red[8];
+    }
 }
 
 // ---------------------------------------------------------------- thermo
@@ -228,13 +361,15 @@ struct Thermo { double cp, ha, negG, dBdT, dcpdT; };
 // the temperature is known so that the loads overlap the density sum.
 struct SpeciesData { double lo[7], hi[7], Tc, invW, W; };
 
+template <int NP>
 __device__ __forceinline__ SpeciesData load_species(const DevMech& M, int s)
 {
+    constexpr int SW = Cell<NP>::SW;
     SpeciesData d;
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-        d.lo[k] = M.clo[k * 64 + s];
-        d.hi[k] = M.chi[k * 64 + s];
+        d.lo[k] = M.clo[k * SW + s];
+        d.hi[k] = M.chi[k * SW + s];
     }
     d.Tc = M.Tc[s];
     d.invW = M.invW[s];
@@ -444,15 +579,17 @@ __device__ inline RxnK rate_constant(const DevMech& M, const RxnData& D, int r, 
 // Third-body concentrations M_t = sum_j eff_tj C_j of every third-body and
 // falloff row (src/Reaction/ReactionRate.C:178-399 forms the same sums per
 // row), one row per lane, two rows per lane pass; L.C must be complete.
+template <int NP>
 __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 {
-    const int lane = lane_id();
-    for (int t0 = 0; t0 < M.nTB; t0 += 2 * WAVE) {
-        const int ta = t0 + lane, tb = t0 + WAVE + lane;
+    constexpr int G = Cell<NP>::G;
+    const int lane = tid<NP>();
+    for (int t0 = 0; t0 < M.nTB; t0 += 2 * G) {
+        const int ta = t0 + lane, tb = t0 + G + lane;
         const bool ina = ta < M.nTB, inb = tb < M.nTB;
         double ma = 0.0, mb = 0.0;
         for (int j0 = 0; j0 < M.ns; j0 += 16) {   // batches of 16 species
-            // unconditional loads (the table is zero-padded to 64 species and
+            // unconditional loads (the table is zero-padded to SW species and
             // nTBpad rows, C is zero beyond ns), so all 48 are in flight at once
             double ea[16], eb[16], c[16];
 #pragma unroll
@@ -549,13 +686,14 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
 // src/Reaction/JacobianFunctionI.H:1150-1158.  The species of all six slots
 // are gathered unconditionally (unused slots name species 0) and the
 // products take the used slots in slot order.
-template <bool JAC>
+template <bool JAC, int NP>
 __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, double lnT, double iT,
                                       double lnp)
 {
+    constexpr int G = Cell<NP>::G;
     const double PbyRT = PSTD / (RU * T);
-    for (int r0 = lane_id(); r0 < M.nr; r0 += 2 * WAVE) {
-        const int r1 = r0 + WAVE;
+    for (int r0 = tid<NP>(); r0 < M.nr; r0 += 2 * G) {
+        const int r1 = r0 + G;
         const bool has1 = r1 < M.nr;
         const RxnData D0 = load_rxn(M, L, r0);
         const RxnData D1 = load_rxn(M, L, has1 ? r1 : r0);
@@ -577,15 +715,17 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
 
 // Molar production rate of this lane's species from the rates q_r in
 // L.rq: sum over its reaction list of nu_ir q_r, in list order.
+template <int NP>
 __device__ __forceinline__ double production_rate(const DevMech& M, const Lds& L)
 {
-    const int lane = lane_id();
+    constexpr int SW = Cell<NP>::SW;
+    const int lane = tid<NP>();
     double om = 0.0;
     for (int e0 = 0; e0 < M.maxDeg; e0 += 16) {   // unconditional batches of 16 (maxDeg % 16 == 0)
         int w[16];
         double q[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) w[u] = M.sp_rn[(e0 + u) * WAVE + lane];
+        for (int u = 0; u < 16; ++u) w[u] = M.sp_rn[(e0 + u) * SW + lane];
 #pragma unroll
         for (int u = 0; u < 16; ++u) q[u] = L.rq[rn_r(w[u])];
 #pragma unroll
@@ -605,12 +745,13 @@ __device__ __forceinline__ double production_rate(const DevMech& M, const Lds& L
 // (Tc.C:583 tests isIrreversible == 0, so nonEquilibrium reactions count
 // as irreversible there).  L.C, L.expG and L.Mtb must be complete.
 // Returns (sumW, sumWRateByCTot) in every lane.
+template <int NP>
 __device__ inline double2 time_scale_sums(const DevMech& M, const Lds& L, double T, double lnT, double iT,
                                           double lnp)
 {
     const double PbyRT = PSTD / (RU * T);
     double sw = 0.0, sw2 = 0.0;
-    for (int r = lane_id(); r < M.nr; r += WAVE) {
+    for (int r = tid<NP>(); r < M.nr; r += Cell<NP>::G) {
         const RxnData D = load_rxn(M, L, r);
         const RxnK k = rate_constant<false>(M, D, r, T, lnT, iT, lnp);
         const int nl = pk_nl(D.pk), nrh = pk_nr(D.pk);
@@ -634,7 +775,8 @@ __device__ inline double2 time_scale_sums(const DevMech& M, const Lds& L, double
         sw += wf + wr;
         sw2 += wf * wf + wr * wr;
     }
-    return make_double2(wave_sum(sw), wave_sum(sw2));
+    const double a = gsum<NP>(sw, L);
+    return make_double2(a, gsum<NP>(sw2, L));
 }
 
 // ---------------------------------------------------------------- RHS
@@ -644,14 +786,14 @@ __device__ inline double2 time_scale_sums(const DevMech& M, const Lds& L, double
 template <int NP>
 __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, double lnp, double& y)
 {
-    const int lane = lane_id();
+    const int lane = tid<NP>();
     const bool sp = lane < M.ns;
-    const SpeciesData S = load_species(M, lane);      // tables are padded to 64 lanes
+    const SpeciesData S = load_species<NP>(M, lane);  // tables are padded to the group size
     if (sp) y = y > 0.0 ? y : 0.0;
-    const double T = readlane_d(y, NP - 1);
+    const double T = gbcast<NP>(y, NP - 1, L);
     const double RuTByP = RU * T / p;                 // unclamped T for the density
     const double invW = sp ? S.invW : 0.0;
-    const double invrho = wave_sum(sp ? y * invW * RuTByP : 0.0);
+    const double invrho = gsum<NP>(sp ? y * invW * RuTByP : 0.0, L);
     const double rho = 1.0 / invrho;
     const double Tc = clampT(M, T), lnT = log(Tc), iT = 1.0 / Tc;
     Thermo th{0, 0, 0, 0, 0};
@@ -659,14 +801,14 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
     L.C[lane] = sp ? rho * invW * y : 0.0;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     wave_sync();
-    third_body_sums(M, L);
+    third_body_sums<NP>(M, L);
     wave_sync();
-    reaction_round<false>(M, L, Tc, lnT, iT, lnp);
+    reaction_round<false, NP>(M, L, Tc, lnT, iT, lnp);
     wave_sync();
-    const double om = sp ? production_rate(M, L) : 0.0;
+    const double om = sp ? production_rate<NP>(M, L) : 0.0;
     const double Yd = sp ? om * (S.W * invrho) : 0.0;
-    const double cpM = wave_sum(sp ? y * th.cp : 0.0);
-    const double hY = wave_sum(sp ? th.ha * Yd : 0.0);
+    const double cpM = gsum<NP>(sp ? y * th.cp : 0.0, L);
+    const double hY = gsum<NP>(sp ? th.ha * Yd : 0.0, L);
     wave_sync();
     return sp ? Yd : (lane == NP - 1 ? -hY / cpM : 0.0);
 }
@@ -685,33 +827,35 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
 template <int NP>
 __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, double lnp, int jacType, double& y)
 {
-    const int lane = lane_id();
+    constexpr int SW = Cell<NP>::SW, JSt = Cell<NP>::JSt, ONE = Cell<NP>::ONE, G = Cell<NP>::G;
+    const int lane = tid<NP>();
     const int ns = M.ns;
     const bool sp = lane < ns;
-    const SpeciesData S = load_species(M, lane);      // tables are padded to 64 lanes
+    const SpeciesData S = load_species<NP>(M, lane);  // tables are padded to the group size
     if (sp) y = y > 0.0 ? y : 0.0;
-    const double T = clampT(M, readlane_d(y, NP - 1));
+    const double T = clampT(M, gbcast<NP>(y, NP - 1, L));
     const double lnT = log(T), iT = 1.0 / T;
     const double invW = sp ? S.invW : 0.0;
     Thermo th{0, 0, 0, 0, 0};
     if (sp) th = species_thermo(M, S, T, lnT, iT);
     const double RuTByP = RU * T / p;
-    const double invrho = wave_sum(sp ? RuTByP * (y * invW) : 0.0);
-    const double MW = 1.0 / wave_sum(sp ? y * invW : 0.0);
+    const double invrho = gsum<NP>(sp ? RuTByP * (y * invW) : 0.0, L);
+    const double MW = 1.0 / gsum<NP>(sp ? y * invW : 0.0, L);
     const double rho = 1.0 / invrho;
-    const double cpM = wave_sum(sp ? y * th.cp : 0.0);
-    const double dcpMdT = wave_sum(sp ? y * th.dcpdT : 0.0);
+    const double cpM = gsum<NP>(sp ? y * th.cp : 0.0, L);
+    const double dcpMdT = gsum<NP>(sp ? y * th.dcpdT : 0.0, L);
     const double Cl = sp ? rho * y * invW : 0.0;
-    L.C[lane] = Cl;
-    if (lane == 0) L.C[64] = 1.0;
+    L.C[lane] = lane == ONE ? 1.0 : Cl;
+    if (ONE >= G && lane == 0) L.C[ONE] = 1.0;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     L.dBdT[lane] = th.dBdT;
-    double* Jrow = L.J + lane;   // element j of this lane's row at Jrow[j*JS]
-    for (int j = 0; j < NP; ++j) Jrow[j * JS] = 0.0;
+    double* Jrow = L.J + lane;   // element j of this thread's row at Jrow[j*JSt]
+    if (!Cell<NP>::kLarge || lane < NP)   // (wavefront form: every lane owns a slot of each column)
+        for (int j = 0; j < NP; ++j) Jrow[j * JSt] = 0.0;
     wave_sync();
-    third_body_sums(M, L);
+    third_body_sums<NP>(M, L);
     wave_sync();
-    reaction_round<true>(M, L, T, lnT, iT, lnp);
+    reaction_round<true, NP>(M, L, T, lnT, iT, lnp);
     wave_sync();
     // species rows, sparse partials: row i is a list of terms nu_ir dq_r/dC_j
     // sorted by column (host order within a column follows the species'
@@ -723,7 +867,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
         int cur = jterm_col(M.sp_term[lane]);
         unsigned long long wn[16];   // next batch, loaded one batch ahead
 #pragma unroll
-        for (int u = 0; u < 16; ++u) wn[u] = M.sp_term[u * WAVE + lane];
+        for (int u = 0; u < 16; ++u) wn[u] = M.sp_term[u * SW + lane];
         for (int t0 = 0; t0 < M.maxTerms; t0 += 16) {
             unsigned long long w[16];
             double k[16], c1[16], c2[16];
@@ -731,12 +875,12 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             for (int u = 0; u < 16; ++u) w[u] = wn[u];
             const int tn = t0 + 16 < M.maxTerms ? t0 + 16 : t0;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) wn[u] = M.sp_term[(tn + u) * WAVE + lane];
+            for (int u = 0; u < 16; ++u) wn[u] = M.sp_term[(tn + u) * SW + lane];
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 const int r = jterm_r(w[u]);
                 k[u] = jterm_rev(w[u]) ? L.rk[r] : L.rq[r];
-                c1[u] = L.C[jterm_o1(w[u])];   // 64 (= 1.0) when there is no other species
+                c1[u] = L.C[jterm_o1(w[u])];   // ONE (= 1.0) when there is no other species
                 c2[u] = L.C[jterm_o2(w[u])];
             }
 #pragma unroll
@@ -745,14 +889,14 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
                 if (nu == 0) continue;
                 const int col = jterm_col(w[u]);
                 if (col != cur) {
-                    Jrow[cur * JS] = acc;
+                    Jrow[cur * JSt] = acc;
                     acc = 0.0;
                     cur = col;
                 }
                 acc += (double)nu * ((k[u] * c1[u]) * c2[u]);
             }
         }
-        Jrow[cur * JS] = acc;
+        Jrow[cur * JSt] = acc;
     }
     // production rate and d(omega)/dT of this lane's species (its reaction
     // list, loads and gathers issued per block of 8 entries)
@@ -764,8 +908,8 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             int wv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                wv[u] = M.sp_rn[(e0 + u) * WAVE + lane];
-                pk[u] = M.sp_pack[(e0 + u) * WAVE + lane];
+                wv[u] = M.sp_rn[(e0 + u) * SW + lane];
+                pk[u] = M.sp_pack[(e0 + u) * SW + lane];
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -803,7 +947,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     if (M.nTBz > 0) {
         double acc = 0.0;
         int cur = M.tbz_row[0];
-        double old = sp ? L.J[lane * JS + cur] : 0.0;
+        double old = sp ? L.J[lane * JSt + cur] : 0.0;
         for (int k0 = 0; k0 < M.nTBz; k0 += 16) {
             int rw_[16];
             double cf[16], ef[16];
@@ -812,20 +956,20 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
                 const int t = M.tbz_tb[k0 + u];
                 rw_[u] = M.tbz_row[k0 + u];
                 cf[u] = M.tbz_nu[k0 + u] * L.rw[t];
-                ef[u] = M.eff[t * WAVE + lane];
+                ef[u] = M.eff[t * SW + lane];
             }
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 if (rw_[u] != cur) {
-                    if (sp) L.J[lane * JS + cur] = old + acc;
+                    if (sp) L.J[lane * JSt + cur] = old + acc;
                     acc = 0.0;
                     cur = rw_[u];
-                    old = sp ? L.J[lane * JS + cur] : 0.0;
+                    old = sp ? L.J[lane * JSt + cur] : 0.0;
                 }
                 acc += cf[u] * ef[u];
             }
         }
-        if (sp) L.J[lane * JS + cur] = old + acc;
+        if (sp) L.J[lane * JSt + cur] = old + acc;
     }
     wave_sync();
     // dC/dY conversion of the species rows, T column
@@ -838,7 +982,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (j0 + u < ns) {
-                    v[u] = Jrow[(j0 + u) * JS];
+                    v[u] = Jrow[(j0 + u) * JSt];
                     c[u] = L.C[j0 + u];
                 }
 #pragma unroll
@@ -851,23 +995,23 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (j0 + u < ns) {
-                    v[u] = Jrow[(j0 + u) * JS];
+                    v[u] = Jrow[(j0 + u) * JSt];
                     iw[u] = M.invW[j0 + u];
                 }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
                 if (j0 + u < ns) {
                     const double iWj = iw[u];
-                    Jrow[(j0 + u) * JS] = jacType == 0
+                    Jrow[(j0 + u) * JSt] = jacType == 0
                         ? WiByRho * (v[u] * (rho * iWj) - (MW * iWj) * sCx) + (MW * iWj) * Yd
                         : WiByRho * (v[u] * (rho * iWj)) + (MW * iWj) * Yd;
                 }
         }
-        Jrow[(NP - 1) * JS] = WiByRho * (jT - sC * iT) + iT * Yd;
+        Jrow[(NP - 1) * JSt] = WiByRho * (jT - sC * iT) + iT * Yd;
     }
     // temperature row (src/Reaction/Jacobian.C:3743-3791): lane j < NP forms
     // column j's sum over the species rows and writes row NP-1, column j
-    const double dTdt = -wave_sum(sp ? Yd * th.ha : 0.0) / cpM;
+    const double dTdt = -gsum<NP>(sp ? Yd * th.ha : 0.0, L) / cpM;
     L.H[lane] = sp ? th.ha : 0.0;
     L.Q[lane] = sp ? Yd * th.cp : 0.0;
     wave_sync();
@@ -881,7 +1025,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             for (int u = 0; u < 8; ++u)
                 if (q0 + u < ns) {
                     const int i = q0 + u;
-                    v[u] = L.J[lane * JS + i];
+                    v[u] = L.J[lane * JSt + i];
                     h[u] = L.H[i];
                     qq[u] = tcol ? L.Q[i] : 0.0;
                 }
@@ -891,7 +1035,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
         }
         const double rowT = !tcol ? (sp ? (-colsum - th.cp * dTdt) / cpM : 0.0)
                                   : (-colsum - dTdt * dcpMdT) / cpM;
-        L.J[lane * JS + (NP - 1)] = rowT;
+        L.J[lane * JSt + (NP - 1)] = rowT;
     }
     wave_sync();
     return sp ? Yd : (lane == NP - 1 ? dTdt : 0.0);
@@ -915,7 +1059,7 @@ __device__ __forceinline__ double cand(double ak, int pos, int r)
 template <int NP>
 __device__ inline LuState lu_factor(const Lds& L)
 {
-    static_assert(NP % 4 == 0, "padded system size must be a multiple of 4");
+    static_assert(NP % 4 == 0 && NP <= 64, "wavefront-form LU: padded size a multiple of 4, at most 64");
     const int lane = lane_id();
     const bool row = lane < NP;
     double* Jrow = L.J + lane;

@@ -3,6 +3,7 @@
 // reference's step-size controller, run by one wavefront per cell.
 #pragma once
 #include "fcs_device.h"
+#include "fcs_lu_large.h"
 
 namespace fcs {
 
@@ -63,27 +64,43 @@ __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 // Max-norm error (src/ODE/OptRodas34/OptRodas34.C:822-877); NaN in any
 // lane is returned as NaN.
 template <int NP>
-__device__ inline double error_norm(const DevOpts& o, double y0, double yT, double err)
+__device__ inline double error_norm(const DevOpts& o, const Lds& L, double y0, double yT, double err)
 {
-    const int lane = lane_id();
+    const int lane = tid<NP>();
     double r = 0.0;
     if (lane < NP) {
         const double a0 = fabs(y0), a1 = fabs(yT);
         r = fabs(err) / (o.absTol + o.relTol * (a1 > a0 ? a1 : a0));
     }
-    const bool bad = wave_any(!(r == r));
-    const double mx = wave_max(r);
+    const bool bad = gany<NP>(!(r == r), L);
+    const double mx = gmax<NP>(r, L);
     return bad ? __builtin_nan("") : mx;
 }
 
-// Iteration matrix diag*I - J in place in the LDS matrix.
+// Iteration matrix diag*I - J in place.
 template <int NP>
 __device__ inline void form_matrix(const Lds& L, double diag)
 {
-    const int lane = lane_id();
+    constexpr int JSt = Cell<NP>::JSt;
+    const int lane = tid<NP>();
     double* Jrow = L.J + lane;
-    for (int j = 0; j < NP; ++j) Jrow[j * JS] = (lane == j ? diag : 0.0) - Jrow[j * JS];
+    if (!Cell<NP>::kLarge || lane < NP)
+        for (int j = 0; j < NP; ++j) Jrow[j * JSt] = (lane == j ? diag : 0.0) - Jrow[j * JSt];
     wave_sync();
+}
+
+// LU factor / solve of either form.
+template <int NP>
+__device__ inline auto lu_factor_any(const Lds& L)
+{
+    if constexpr (Cell<NP>::kLarge) return lu_factor_large<NP>(L);
+    else return lu_factor<NP>(L);
+}
+template <int NP, class St>
+__device__ inline double lu_solve_any(const Lds& L, const St& st, double b)
+{
+    if constexpr (Cell<NP>::kLarge) return lu_solve_large<NP>(L, st, b);
+    else return lu_solve<NP>(L, st, b);
 }
 
 // Rosenbrock tableau in the transformed (Hairer-Wanner) form the reference
@@ -205,7 +222,7 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
     const double invdx = 1.0 / dx;
     const double dfdx = jacobian<NP>(M, L, p, lnp, o.jac, y0);
     form_matrix<NP>(L, 1.0 * invdx * tb.invGamma);
-    const LuState st = lu_factor<NP>(L);
+    const auto st = lu_factor_any<NP>(L);
     double k[6] = {0, 0, 0, 0, 0, 0};
     double dydx = dfdx;   // a stage without its own f before any f evaluation takes f(y0)
     for (int s = 0; s < tb.stages; ++s) {
@@ -223,7 +240,7 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
 #pragma unroll
             for (int j = 0; j < 5; ++j) rhs += (tb.C[s][j] * invdx) * k[j];
         }
-        const double ks = lu_solve<NP>(L, st, rhs);
+        const double ks = lu_solve_any<NP>(L, st, rhs);
 #pragma unroll
         for (int j = 0; j < 6; ++j)
             if (j == s) k[j] = ks;
@@ -235,7 +252,7 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
         yT += tb.B[j] * k[j];
         err += tb.E[j] * k[j];
     }
-    return error_norm<NP>(o, y0, yT, err);
+    return error_norm<NP>(o, L, y0, yT, err);
 }
 
 // Step-size proposal after an accepted step of size dx with error norm Err

@@ -17,6 +17,8 @@ using namespace fcs;
 
 // Padded system sizes compiled into the library (T sits at NP-1).
 #define FCS_FOR_EACH_NP(X) X(12) X(56)
+// workgroup form (matrix in HBM): padded sizes, multiples of 16
+#define FCS_FOR_EACH_NP_LARGE(X) X(176)
 
 // ============================================================== kernels
 // Device arrays of one batch of cells: the flow-step inputs and outputs of
@@ -40,7 +42,16 @@ struct CellArrays {
     double *Y0s, *Scl, *theta, *Jy;
     int* kTarg;
     double* dxFirst;                 // first accepted sub-step of the last run (0: none yet)
+    double* Jg;                      // workgroup form: matrix slices [grid][NP*NP + NP]
 };
+
+// Matrix slice of this workgroup (workgroup form), else null.
+template <int NP>
+__device__ __forceinline__ double* jslice(double* Jg)
+{
+    if constexpr (Cell<NP>::kLarge) return Jg + (size_t)blockIdx.x * ((size_t)NP * NP + NP);
+    else return nullptr;
+}
 
 // Clears the active-list counters and the deltaTMin accumulator (a kernel
 // rather than memset nodes, so the sequence can be captured in a graph).
@@ -95,12 +106,12 @@ __global__ __launch_bounds__(256) void k_setup(DevOpts o, int ns, CellArrays a, 
 // solve), the new deltaTChem (the input one when the integration failed)
 // capped by deltaTChemMax, and the deltaTMin reduction.
 template <int NP>
-__device__ void finish_cell(const DevMech& M, const DevOpts& o, const CellArrays& a, int c, int st, double y,
-                            double dxTry, int nAtt)
+__device__ void finish_cell(const DevMech& M, const DevOpts& o, const Lds& L, const CellArrays& a, int c, int st,
+                            double y, double dxTry, int nAtt)
 {
-    const int lane = lane_id();
+    const int lane = tid<NP>();
     const size_t nc = (size_t)a.nc;
-    if (wave_any(!(y == y) || isinf(y))) st |= CELL_NONFINITE;
+    if (gany<NP>(!(y == y) || isinf(y), L)) st |= CELL_NONFINITE;
     if (lane < M.ns) {
         const double Yn = y > 0.0 ? y : 0.0;
         const double r0 = a.rho0[c];
@@ -124,13 +135,13 @@ __device__ void finish_cell(const DevMech& M, const DevOpts& o, const CellArrays
 // counter *countZero (the list after next) is cleared for the next launch.
 // Every loop here is bounded by the list length read at entry.
 template <int NP, int SOLVER>
-__global__ __launch_bounds__(64) void k_step(DevMech M, DevOpts o, CellArrays a, const int* listIn,
-                                             const unsigned int* countIn, int* listOut, unsigned int* countOut,
-                                             unsigned int* countZero)
+__global__ __launch_bounds__(Cell<NP>::G) void k_step(DevMech M, DevOpts o, CellArrays a, const int* listIn,
+                                                      const unsigned int* countIn, int* listOut,
+                                                      unsigned int* countOut, unsigned int* countZero)
 {
     extern __shared__ double lds_raw[];
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int lane = lane_id();
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
+    const int lane = tid<NP>();
     const int ns = M.ns;
     const size_t nc = (size_t)a.nc;
     const int row = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
@@ -160,7 +171,7 @@ __global__ __launch_bounds__(64) void k_step(DevMech M, DevOpts o, CellArrays a,
                 listOut[atomicAdd(countOut, 1u)] = c;
             }
         } else {
-            finish_cell<NP>(M, o, a, c, st, y, dxTry, nAtt);
+            finish_cell<NP>(M, o, L, a, c, st, y, dxTry, nAtt);
         }
     }
 }
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(64) void k_step_seulex(DevMech M, DevOpts o, CellAr
                 listOut[atomicAdd(countOut, 1u)] = c;
             }
         } else {
-            finish_cell<NP>(M, o, a, c, st, y, sc.dxTry, sc.nAtt);
+            finish_cell<NP>(M, o, L, a, c, st, y, sc.dxTry, sc.nAtt);
         }
     }
 }
@@ -283,11 +294,11 @@ __global__ __launch_bounds__(64) void k_verify(DevMech M, DevOpts o, CellArrays 
                 const double m = fabs(yT) > fabs(yP) ? fabs(yT) : fabs(yP);
                 r = fabs(yT - yP) / (o.absTol + o.relTol * m);
             }
-            const bool bad = wave_any(!(r == r));
-            const double agree = wave_max(r);
+            const bool bad = gany<NP>(!(r == r), L);
+            const double agree = gmax<NP>(r, L);
             accept = (err <= 1.0) && !bad && (agree <= 1.0);
             if (accept) {
-                finish_cell<NP>(M, o, a, c, FCS_CELL_LEARNED, yT, accepted_dxTry(err, dt), 1);
+                finish_cell<NP>(M, o, L, a, c, FCS_CELL_LEARNED, yT, accepted_dxTry(err, dt), 1);
                 if (lane == 0) atomicAdd(nAccepted, 1ull);
             }
         }
@@ -304,11 +315,12 @@ __global__ __launch_bounds__(64) void k_verify(DevMech M, DevOpts o, CellArrays 
 // One wavefront per state: derivatives.  External layout [n][nS] with T at
 // row ns.
 template <int NP>
-__global__ __launch_bounds__(64) void k_derivatives(DevMech M, int nS, const double* p, double* YT, double* out)
+__global__ __launch_bounds__(Cell<NP>::G) void k_derivatives(DevMech M, int nS, const double* p, double* YT,
+                                                             double* out)
 {
     extern __shared__ double lds_raw[];
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int s = blockIdx.x, lane = lane_id(), ns = M.ns;
+    const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
     const double d = derivatives<NP>(M, L, p[s], log(p[s]), y);
@@ -319,12 +331,12 @@ __global__ __launch_bounds__(64) void k_derivatives(DevMech M, int nS, const dou
 }
 
 template <int NP>
-__global__ __launch_bounds__(64) void k_jacobian(DevMech M, int jacType, int nS, const double* p, double* YT,
-                                                 double* f, double* J)
+__global__ __launch_bounds__(Cell<NP>::G) void k_jacobian(DevMech M, int jacType, int nS, const double* p,
+                                                          double* YT, double* f, double* J, double* Jg)
 {
     extern __shared__ double lds_raw[];
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int s = blockIdx.x, lane = lane_id(), ns = M.ns, n = ns + 1;
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
+    const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
     const double d = jacobian<NP>(M, L, p[s], log(p[s]), jacType, y);
@@ -334,7 +346,7 @@ __global__ __launch_bounds__(64) void k_jacobian(DevMech M, int jacType, int nS,
         double* row = J + ((size_t)s * n + ext) * n;
         for (int j = 0; j < NP; ++j) {
             const int ej = j < ns ? j : (j == NP - 1 ? ns : -1);
-            if (ej >= 0) row[ej] = L.J[j * JS + lane];
+            if (ej >= 0) row[ej] = L.J[j * Cell<NP>::JSt + lane];
         }
     }
 }
@@ -345,12 +357,13 @@ __global__ __launch_bounds__(64) void k_jacobian(DevMech M, int jacType, int nS,
 // OptRosenbrock34.C:281-677).  External layout [n][nS], T at row ns; the
 // start state is clipped in place as the Jacobian evaluation does.
 template <int NP, int SOLVER>
-__global__ __launch_bounds__(64) void k_attempt(DevMech M, DevOpts o, int nS, const double* p, const double* dx,
-                                                double* YT, double* YTnew, double* err)
+__global__ __launch_bounds__(Cell<NP>::G) void k_attempt(DevMech M, DevOpts o, int nS, const double* p,
+                                                         const double* dx, double* YT, double* YTnew, double* err,
+                                                         double* Jg)
 {
     extern __shared__ double lds_raw[];
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int s = blockIdx.x, lane = lane_id(), ns = M.ns;
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
+    const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
     double yT = 0.0;
@@ -364,20 +377,22 @@ __global__ __launch_bounds__(64) void k_attempt(DevMech M, DevOpts o, int nS, co
 }
 
 template <int NP>
-__global__ __launch_bounds__(64) void k_lu_solve(DevMech M, int nSys, const double* A, double* b)
+__global__ __launch_bounds__(Cell<NP>::G) void k_lu_solve(DevMech M, int nSys, const double* A, double* b,
+                                                          double* Jg)
 {
     extern __shared__ double lds_raw[];
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int s = blockIdx.x, lane = lane_id(), ns = M.ns, n = ns + 1;
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
+    const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
-    for (int j = 0; j < NP; ++j) {
-        const int ej = j < ns ? j : (j == NP - 1 ? ns : -1);
-        L.J[j * JS + lane] =
-            (ext >= 0 && ej >= 0) ? A[((size_t)s * n + ext) * n + ej] : ((lane == j && lane < NP) ? 1.0 : 0.0);
-    }
+    if (!Cell<NP>::kLarge || lane < NP)
+        for (int j = 0; j < NP; ++j) {
+            const int ej = j < ns ? j : (j == NP - 1 ? ns : -1);
+            L.J[j * Cell<NP>::JSt + lane] =
+                (ext >= 0 && ej >= 0) ? A[((size_t)s * n + ext) * n + ej] : ((lane == j && lane < NP) ? 1.0 : 0.0);
+        }
     wave_sync();
-    const LuState st = lu_factor<NP>(L);
-    const double x = lu_solve<NP>(L, st, ext >= 0 ? b[(size_t)s * n + ext] : 0.0);
+    const auto st = lu_factor_any<NP>(L);
+    const double x = lu_solve_any<NP>(L, st, ext >= 0 ? b[(size_t)s * n + ext] : 0.0);
     if (ext >= 0) b[(size_t)s * n + ext] = x;
 }
 
@@ -389,9 +404,9 @@ template <int NP>
 __device__ inline double cell_rate_state(const DevMech& M, const Lds& L, int nc, int c, const double* rho,
                                          const double* T, const double* Y, double& Tcl, double& lnT, double& iT)
 {
-    const int lane = lane_id();
+    const int lane = tid<NP>();
     const bool sp = lane < M.ns;
-    const SpeciesData S = load_species(M, lane);
+    const SpeciesData S = load_species<NP>(M, lane);
     const double Cl = sp ? rho[c] * Y[(size_t)lane * nc + c] / S.W : 0.0;
     Tcl = clampT(M, T[c]);
     lnT = log(Tcl);
@@ -401,7 +416,7 @@ __device__ inline double cell_rate_state(const DevMech& M, const Lds& L, int nc,
     L.C[lane] = Cl;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     wave_sync();
-    third_body_sums(M, L);
+    third_body_sums<NP>(M, L);
     wave_sync();
     return Cl;
 }
@@ -410,34 +425,34 @@ __device__ inline double cell_rate_state(const DevMech& M, const Lds& L, int nc,
 // 700-740): RR_i = W_i dNdtByV_i(p, T, c), one wavefront per cell, fields
 // [ns][nc].
 template <int NP>
-__global__ __launch_bounds__(64) void k_calculate(DevMech M, int nc, const double* rho, const double* T,
-                                                  const double* p, const double* Y, double* RR)
+__global__ __launch_bounds__(Cell<NP>::G) void k_calculate(DevMech M, int nc, const double* rho, const double* T,
+                                                           const double* p, const double* Y, double* RR)
 {
     extern __shared__ double lds_raw[];
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int c = blockIdx.x, lane = lane_id();
+    const int c = blockIdx.x, lane = tid<NP>();
     double Tcl, lnT, iT;
     (void)cell_rate_state<NP>(M, L, nc, c, rho, T, Y, Tcl, lnT, iT);
-    reaction_round<false>(M, L, Tcl, lnT, iT, log(p[c]));
+    reaction_round<false, NP>(M, L, Tcl, lnT, iT, log(p[c]));
     wave_sync();
-    if (lane < M.ns) RR[(size_t)lane * nc + c] = production_rate(M, L) * M.W[lane];
+    if (lane < M.ns) RR[(size_t)lane * nc + c] = production_rate<NP>(M, L) * M.W[lane];
 }
 
 // FastChemistryModel::tc (src/ChemistryModel/FastChemistryModel.C:588-643)
 // with OptReaction::Tc (src/Reaction/Tc.C:4-834): tc = sumW /
 // sumWRateByCTot * sum(c), or vGreat when no reaction proceeds.
 template <int NP>
-__global__ __launch_bounds__(64) void k_tc(DevMech M, int nc, const double* rho, const double* T, const double* p,
-                                           const double* Y, double* tc)
+__global__ __launch_bounds__(Cell<NP>::G) void k_tc(DevMech M, int nc, const double* rho, const double* T,
+                                                    const double* p, const double* Y, double* tc)
 {
     extern __shared__ double lds_raw[];
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
     const int c = blockIdx.x;
     double Tcl, lnT, iT;
     const double Cl = cell_rate_state<NP>(M, L, nc, c, rho, T, Y, Tcl, lnT, iT);
-    const double sumc = wave_sum(Cl);
-    const double2 sw = time_scale_sums(M, L, Tcl, lnT, iT, log(p[c]));
-    if (lane_id() == 0) tc[c] = sw.y == 0.0 ? OF_VGREAT : sw.x / sw.y * sumc;
+    const double sumc = gsum<NP>(Cl, L);
+    const double2 sw = time_scale_sums<NP>(M, L, Tcl, lnT, iT, log(p[c]));
+    if (tid<NP>() == 0) tc[c] = sw.y == 0.0 ? OF_VGREAT : sw.x / sw.y * sumc;
 }
 
 // FastChemistryModel::Qdot (src/ChemistryModel/FastChemistryModel.C:
@@ -456,6 +471,7 @@ namespace {
 
 struct KernelSet {
     int NP;
+    int G;                      // threads per cell
     const void* step[5];        // indexed by FCS_SOLVER_*
     const void* verify;
     const void* deriv;
@@ -467,7 +483,7 @@ struct KernelSet {
 };
 
 #define FCS_KSET(N)                                                                                        \
-    {N,                                                                                                    \
+    {N, group_of(N),                                                                                       \
      {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>,                        \
       (const void*)k_step_seulex<N>, (const void*)k_step<N, SOLVER_RODAS23>,                               \
       (const void*)k_step<N, SOLVER_ROS23>},                                                               \
@@ -476,8 +492,19 @@ struct KernelSet {
      {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>, nullptr,         \
       (const void*)k_attempt<N, SOLVER_RODAS23>, (const void*)k_attempt<N, SOLVER_ROS23>},                 \
      (const void*)k_calculate<N>, (const void*)k_tc<N>},
-const KernelSet kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET)};
+// workgroup form: Rosenbrock solvers only (no Seulex, no learned-seed verification)
+#define FCS_KSET_LARGE(N)                                                                                  \
+    {N, group_of(N),                                                                                       \
+     {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>, nullptr,               \
+      (const void*)k_step<N, SOLVER_RODAS23>, (const void*)k_step<N, SOLVER_ROS23>},                       \
+     nullptr,                                                                                              \
+     (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>,                \
+     {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>, nullptr,         \
+      (const void*)k_attempt<N, SOLVER_RODAS23>, (const void*)k_attempt<N, SOLVER_ROS23>},                 \
+     (const void*)k_calculate<N>, (const void*)k_tc<N>},
+const KernelSet kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET) FCS_FOR_EACH_NP_LARGE(FCS_KSET_LARGE)};
 #undef FCS_KSET
+#undef FCS_KSET_LARGE
 
 const KernelSet* pick_kernels(int n)
 {
@@ -534,6 +561,8 @@ struct DevCells {
     int *status = nullptr, *nAtt = nullptr, *flags = nullptr, *nStep = nullptr, *list0 = nullptr, *list1 = nullptr,
         *kTarg = nullptr;
     size_t jyPerCell = 0;                    // doubles of Jy per cell (Seulex), 0 otherwise
+    double* Jg = nullptr;                    // workgroup form: per-workgroup matrix slices
+    size_t jgDoubles = 0;
     unsigned int* counts = nullptr;          // [3] active-list lengths
     unsigned long long* dtMin = nullptr;
     unsigned long long* nAccepted = nullptr; // learned proposals accepted in the last run
@@ -607,6 +636,20 @@ struct DevCells {
         fcap = rows;
         return FCS_OK;
     }
+    // per-workgroup matrix slices of the workgroup form (kept; regrown on demand)
+    int ensure_jg(size_t doubles)
+    {
+        if (doubles <= jgDoubles) return FCS_OK;
+        drop_graph();   // a captured graph holds the old address
+        if (Jg) (void)hipFree(Jg), Jg = nullptr;
+        jgDoubles = 0;
+        if (hipMalloc(&Jg, doubles * 8) != hipSuccess) {
+            Jg = nullptr;
+            return FCS_E_NOMEM;
+        }
+        jgDoubles = doubles;
+        return FCS_OK;
+    }
     void drop_graph()
     {
         if (graphExec) (void)hipGraphExecDestroy(graphExec);
@@ -618,6 +661,8 @@ struct DevCells {
         if (Xf) (void)hipFree(Xf), Xf = nullptr;
         if (Pf) (void)hipFree(Pf), Pf = nullptr;
         fcap = 0;
+        if (Jg) (void)hipFree(Jg), Jg = nullptr;
+        jgDoubles = 0;
         if (base) (void)hipFree(base);
         if (hostCount) (void)hipHostFree(hostCount);
         for (auto& e : ev)
@@ -638,6 +683,7 @@ struct DevCells {
         a.dtcOut = dtcOut; a.RR = RR; a.Tend = Tend; a.status = status; a.nAtt = nAtt; a.dtMinBits = dtMin;
         a.Yw = Yw; a.x = x; a.dxTry = dxTry; a.dx = dx; a.dxTry0 = dxTry0; a.flags = flags; a.nStep = nStep;
         a.Y0s = Y0s; a.Scl = Scl; a.theta = theta; a.Jy = Jy; a.kTarg = kTarg; a.dxFirst = dxFirst;
+        a.Jg = Jg;
         return a;
     }
 };
@@ -677,8 +723,13 @@ int fcs_device_count(int* count)
 
 int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
 {
-    if (!d || !out || d->n_species < 1 || d->n_species > 63 || d->n_reactions < 0) return FCS_E_ARG;
+    if (!d || !out || d->n_species < 1 || d->n_reactions < 0) return FCS_E_ARG;
     const int ns = d->n_species, nr = d->n_reactions;
+    const KernelSet* kset = pick_kernels(ns + 1);
+    if (!kset) return FCS_E_UNSUPPORTED;
+    // per-species tables are padded to the threads of one cell; ONE is the
+    // concentration slot holding 1.0 (the "no other species" Jacobian factor)
+    const int SW = kset->G, ONE = kset->NP > 64 ? 255 : 64;
     // ---- order reactions by rate form (file order within a form)
     std::vector<int> perm;
     for (int t = 0; t <= FCS_RXN_SRI; ++t)
@@ -695,15 +746,15 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     }
     const int pmax = d->plog_max_points > 0 ? d->plog_max_points : 1;
     // ---- host images of every table
-    std::vector<double> W(64, 0.0), invW(64, 0.0), Tc(64, 1000.0), clo(7 * 64, 0.0), chi(7 * 64, 0.0), Hf(64, 0.0);
+    std::vector<double> W(SW, 0.0), invW(SW, 0.0), Tc(SW, 1000.0), clo(7 * SW, 0.0), chi(7 * SW, 0.0), Hf(SW, 0.0);
     double TlowMin = d->T_low[0], ThighMax = d->T_high[0], TcMin = d->T_common[0], TcMax = d->T_common[0];
     for (int s = 0; s < ns; ++s) {
         W[s] = d->W[s];
         invW[s] = 1.0 / d->W[s];
         Tc[s] = d->T_common[s];
         for (int k = 0; k < 7; ++k) {
-            clo[k * 64 + s] = d->coeffs_low[7 * s + k];
-            chi[k * 64 + s] = d->coeffs_high[7 * s + k];
+            clo[k * SW + s] = d->coeffs_low[7 * s + k];
+            chi[k * SW + s] = d->coeffs_high[7 * s + k];
         }
         TlowMin = std::min(TlowMin, d->T_low[s]);
         ThighMax = std::max(ThighMax, d->T_high[s]);
@@ -749,7 +800,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         if (t == FCS_RXN_THIRD_BODY || t == FCS_RXN_LINDEMANN || t == FCS_RXN_TROE || t == FCS_RXN_SRI) {
             if (!d->efficiencies) return FCS_E_ARG;
             rtb[r] = nTB++;
-            for (int s = 0; s < 64; ++s) eff.push_back(s < ns ? d->efficiencies[(size_t)f * ns + s] : 0.0);
+            for (int s = 0; s < SW; ++s) eff.push_back(s < ns ? d->efficiencies[(size_t)f * ns + s] : 0.0);
         }
         if (t == FCS_RXN_TROE) {
             if (!d->troe) return FCS_E_ARG;
@@ -776,10 +827,11 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             }
         }
     }
-    const int nTBpad = std::max(128, (nTB + 127) / 128 * 128);
-    std::vector<double> effT((size_t)64 * nTBpad, 0.0);
+    // third_body_sums reads rows t0 + [0, 2 SW) per pass unconditionally
+    const int nTBpad = std::max(2 * SW, (nTB + 2 * SW - 1) / (2 * SW) * (2 * SW));
+    std::vector<double> effT((size_t)SW * nTBpad, 0.0);
     for (int t = 0; t < nTB; ++t)
-        for (int s = 0; s < 64; ++s) effT[(size_t)s * nTBpad + t] = eff[(size_t)t * 64 + s];
+        for (int s = 0; s < SW; ++s) effT[(size_t)s * nTBpad + t] = eff[(size_t)t * SW + s];
     // ---- per-species lists (net stoichiometry), lane-interleaved
     std::vector<std::vector<std::pair<int, double>>> lst(ns), lstTB(ns);
     for (int r = 0; r < nr; ++r) {
@@ -804,16 +856,16 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         w |= (unsigned long long)rnr[r] << 56;
         rpack[r] = w;
     }
-    std::vector<unsigned long long> spPack((size_t)maxDeg * 64, 0ull);
-    std::vector<int> spRN((size_t)maxDeg * 64, 8 << 16);   // padding: reaction 0, nu 0
+    std::vector<unsigned long long> spPack((size_t)maxDeg * SW, 0ull);
+    std::vector<int> spRN((size_t)maxDeg * SW, 8 << 16);   // padding: reaction 0, nu 0
     std::vector<int> tbzRow, tbzTb;
     std::vector<double> tbzNu;
     for (int s = 0; s < ns; ++s) {
         for (size_t e = 0; e < lst[s].size(); ++e) {
             const int nuI = (int)lst[s][e].second;
             if (nuI != lst[s][e].second || nuI < -7 || nuI > 7 || lst[s][e].first > 0xffff) return FCS_E_ARG;
-            spRN[e * 64 + s] = lst[s][e].first | ((nuI + 8) << 16);
-            spPack[e * 64 + s] = rpack[lst[s][e].first];
+            spRN[e * SW + s] = lst[s][e].first | ((nuI + 8) << 16);
+            spPack[e * SW + s] = rpack[lst[s][e].first];
         }
         for (size_t e = 0; e < lstTB[s].size(); ++e) {
             tbzRow.push_back(s);
@@ -849,7 +901,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
                 const int cnt = side == 0 ? rnl[r] : rnr[r];
                 const int sgn = side == 0 ? nu : -nu;
                 for (int q = 0; q < cnt; ++q) {
-                    int oth[2] = {64, 64}, no = 0;   // 64: the LDS slot holding 1.0
+                    int oth[2] = {ONE, ONE}, no = 0;
                     for (int b = 0; b < cnt; ++b)
                         if (b != q) oth[no++] = rsp[(3 * side + b) * nr + r];
                     const unsigned long long col = (unsigned long long)rsp[(3 * side + q) * nr + r];
@@ -866,11 +918,11 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     int maxTerms = 16;
     for (int s = 0; s < ns; ++s) maxTerms = std::max(maxTerms, (int)terms[s].size());
     maxTerms = (maxTerms + 15) / 16 * 16;
-    std::vector<unsigned long long> spTerm((size_t)maxTerms * 64, 8ull << 48);   // padding: column 0, nu 0
+    std::vector<unsigned long long> spTerm((size_t)maxTerms * SW, 8ull << 48);   // padding: column 0, nu 0
     for (int s = 0; s < ns; ++s) {
         const unsigned long long lastCol = terms[s].empty() ? 0ull : (terms[s].back() >> 16) & 255u;
         for (int e = 0; e < maxTerms; ++e)
-            spTerm[(size_t)e * 64 + s] = e < (int)terms[s].size() ? terms[s][e] : (lastCol << 16 | 8ull << 48);
+            spTerm[(size_t)e * SW + s] = e < (int)terms[s].size() ? terms[s][e] : (lastCol << 16 | 8ull << 48);
     }
     // flat third-body entry list padded to a multiple of 16 with zero entries on the last row
     const int nTBz = (int)tbzRow.size();
@@ -901,7 +953,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     DevMech& dm = m->dm;
     dm.ns = ns;
     dm.nr = nr;
-    dm.NPm = 64;
+    dm.NPm = SW;
     dm.nTB = nTB;
     dm.pmax = pmax;
     dm.maxDeg = maxDeg;
@@ -967,7 +1019,7 @@ static int solver_grid(fcs_solver* s)
     int dev = 0, cus = 0, per = 0;
     HIP_OK(hipGetDevice(&dev));
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, s->ks->step[s->opts.solver], 64, s->ldsBytes));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, s->ks->step[s->opts.solver], s->ks->G, s->ldsBytes));
     s->grid = std::max(1, per) * std::max(1, cus);
     return FCS_OK;
 }
@@ -979,7 +1031,7 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
         opts->maxSteps < 1 || opts->maxSteps > 10000000 || !(opts->absTol > 0) || !(opts->relTol > 0))
         return FCS_E_ARG;
     const KernelSet* ks = pick_kernels(mech->ns + 1);
-    if (!ks) return FCS_E_UNSUPPORTED;
+    if (!ks || !ks->step[opts->solver]) return FCS_E_UNSUPPORTED;   // no Seulex in the workgroup form
     fcs_solver* s = new (std::nothrow) fcs_solver();
     if (!s) return FCS_E_NOMEM;
     s->mech = mech;
@@ -993,8 +1045,8 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
     if (s->ldsBytes > 65536)
         for (const void* f : {ks->step[0], ks->step[1], ks->step[2], ks->step[3], ks->step[4], ks->deriv, ks->jac,
                               ks->lu, ks->attempt[0], ks->attempt[1], ks->attempt[3], ks->attempt[4], ks->calculate,
-                              ks->tc})
-            (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->ldsBytes);
+                              ks->tc, ks->verify})
+            if (f) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->ldsBytes);
     if (solver_grid(s) != FCS_OK) {
         (void)hipStreamDestroy(s->stream);
         delete s;
@@ -1058,6 +1110,11 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
     }
     const void* f = s->ks->step[s->opts.solver];
     const int grid = std::max(1, std::min(d.gridOverride > 0 ? d.gridOverride : s->grid, nc));
+    const int blk = s->ks->G;
+    if (s->ks->NP > 64) {
+        if (d.ensure_jg((size_t)grid * ((size_t)s->ks->NP * s->ks->NP + s->ks->NP)) != FCS_OK) return FCS_E_NOMEM;
+        a.Jg = d.Jg;
+    }
     // learned seed: network proposals for the reacting cells, verified by one
     // Rodas34 step each; the rejected ones form list 1 and the step loop
     // starts there
@@ -1111,7 +1168,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
             unsigned int *cin, *cout, *czero;
             launch_args(k + it0, lin, cin, lout, cout, czero);
             void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
-            HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st));
+            HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(blk), args, s->ldsBytes, st));
         }
         const DevCells::GraphKey key{dt, nc, grid, lts ? 1 : 0, s->opts.solver};
         if (!d.graphExec || std::memcmp(&key, &d.gkey, sizeof key) != 0) {
@@ -1124,7 +1181,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
                 unsigned int *cin, *cout, *czero;
                 launch_args(j, lin, cin, lout, cout, czero);
                 void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
-                if (hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st) != hipSuccess) {
+                if (hipLaunchKernel(f, dim3(grid), dim3(blk), args, s->ldsBytes, st) != hipSuccess) {
                     (void)hipStreamEndCapture(st, &g);
                     if (g) (void)hipGraphDestroy(g);
                     return FCS_E_HIP;
@@ -1184,7 +1241,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
             }
             HIP_OK(hipEventRecord(d.pev[2 * k], st));
         }
-        HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(64), args, s->ldsBytes, st));
+        HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(blk), args, s->ldsBytes, st));
         if (d.profile) HIP_OK(hipEventRecord(d.pev[2 * k + 1], st));
         d.lastLaunches = k + 1;
         const int slot = (int)(k % DevCells::kPoll);
@@ -1405,7 +1462,7 @@ int fcs_batch_set_learned(fcs_batch* b, fcs_mlp* mlp, int32_t mode)
 {
     if (!b || mode < FCS_LEARNED_OFF || mode > FCS_LEARNED_SEED || (mode != FCS_LEARNED_OFF && !mlp)) return FCS_E_ARG;
     if (mode != FCS_LEARNED_OFF) {
-        if (b->s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
+        if (b->s->opts.solver == FCS_SOLVER_SEULEX || !b->s->ks->verify) return FCS_E_UNSUPPORTED;
         const int ns = b->s->mech->ns;
         // width check: a one-row forward pass on a scratch input
         float* x = nullptr;
@@ -1480,6 +1537,13 @@ int fcs_batch_download(fcs_batch* b, double* RR, double* deltaTChem, double* del
 
 // ------------------------------------------------------------ kernel-level entries
 
+// HBM matrix doubles per workgroup of the workgroup form (0 otherwise).
+static size_t jg_per_block(const fcs_solver* s)
+{
+    const size_t NP = (size_t)s->ks->NP;
+    return NP > 64 ? NP * NP + NP : 0;
+}
+
 int fcs_eval_derivatives(fcs_solver* s, int64_t nS, const double* p, double* YT, double* dYTdt)
 {
     if (!s || nS < 1 || nS > 0x7fffffff || !p || !YT || !dYTdt) return FCS_E_ARG;
@@ -1494,7 +1558,7 @@ int fcs_eval_derivatives(fcs_solver* s, int64_t nS, const double* p, double* YT,
         DevMech M = s->mech->dm;
         int ns32 = (int)nS;
         void* args[] = {&M, &ns32, &dp, &dy, &dd};
-        HIP_OK(hipLaunchKernel(s->ks->deriv, dim3((unsigned)nS), dim3(64), args, s->ldsBytes, s->stream));
+        HIP_OK(hipLaunchKernel(s->ks->deriv, dim3((unsigned)nS), dim3(s->ks->G), args, s->ldsBytes, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         HIP_OK(hipMemcpy(YT, dy, nv * 8, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(dYTdt, dd, nv * 8, hipMemcpyDeviceToHost));
@@ -1506,18 +1570,19 @@ int fcs_eval_jacobian(fcs_solver* s, int64_t nS, const double* p, double* YT, do
 {
     if (!s || nS < 1 || nS > 0x7fffffff || !p || !YT || !dYTdt || !Jy) return FCS_E_ARG;
     const int n = s->mech->ns + 1;
-    const size_t nv = (size_t)n * nS, nm = (size_t)n * n * nS;
-    return with_device((nS + 2 * nv + nm) * 8, [&](char* base) -> int {
+    const size_t nv = (size_t)n * nS, nm = (size_t)n * n * nS, ng = jg_per_block(s) * nS;
+    return with_device((nS + 2 * nv + nm + ng) * 8, [&](char* base) -> int {
         double* dp = (double*)base;
         double* dy = dp + nS;
         double* df = dy + nv;
         double* dJ = df + nv;
+        double* dG = ng ? dJ + nm : nullptr;
         HIP_OK(hipMemcpy(dp, p, nS * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(dy, YT, nv * 8, hipMemcpyHostToDevice));
         DevMech M = s->mech->dm;
         int jt = s->opts.jacobian, ns32 = (int)nS;
-        void* args[] = {&M, &jt, &ns32, &dp, &dy, &df, &dJ};
-        HIP_OK(hipLaunchKernel(s->ks->jac, dim3((unsigned)nS), dim3(64), args, s->ldsBytes, s->stream));
+        void* args[] = {&M, &jt, &ns32, &dp, &dy, &df, &dJ, &dG};
+        HIP_OK(hipLaunchKernel(s->ks->jac, dim3((unsigned)nS), dim3(s->ks->G), args, s->ldsBytes, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         HIP_OK(hipMemcpy(YT, dy, nv * 8, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(dYTdt, df, nv * 8, hipMemcpyDeviceToHost));
@@ -1532,22 +1597,23 @@ int fcs_eval_attempt(fcs_solver* s, int64_t nS, const double* p, const double* d
     if (!s || nS < 1 || nS > 0x7fffffff || !p || !dx || !YT || !YTnew || !err) return FCS_E_ARG;
     if (s->opts.solver == FCS_SOLVER_SEULEX) return FCS_E_UNSUPPORTED;
     const int n = s->mech->ns + 1;
-    const size_t nv = (size_t)n * nS;
-    return with_device((3 * nS + 2 * nv) * 8, [&](char* base) -> int {
+    const size_t nv = (size_t)n * nS, ng = jg_per_block(s) * nS;
+    return with_device((3 * nS + 2 * nv + ng) * 8, [&](char* base) -> int {
         double* dp = (double*)base;
         double* ddx = dp + nS;
         double* de = ddx + nS;
         double* dy = de + nS;
         double* dn = dy + nv;
+        double* dG = ng ? dn + nv : nullptr;
         HIP_OK(hipMemcpy(dp, p, nS * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(ddx, dx, nS * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(dy, YT, nv * 8, hipMemcpyHostToDevice));
         DevMech M = s->mech->dm;
         DevOpts o = dev_opts(s->opts);
         int ns32 = (int)nS;
-        void* args[] = {&M, &o, &ns32, &dp, &ddx, &dy, &dn, &de};
-        HIP_OK(hipLaunchKernel(s->ks->attempt[s->opts.solver], dim3((unsigned)nS), dim3(64), args, s->ldsBytes,
-                               s->stream));
+        void* args[] = {&M, &o, &ns32, &dp, &ddx, &dy, &dn, &de, &dG};
+        HIP_OK(hipLaunchKernel(s->ks->attempt[s->opts.solver], dim3((unsigned)nS), dim3(s->ks->G), args,
+                               s->ldsBytes, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         HIP_OK(hipMemcpy(YT, dy, nv * 8, hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(YTnew, dn, nv * 8, hipMemcpyDeviceToHost));
@@ -1584,7 +1650,7 @@ static int field_kernel(fcs_solver* s, const void* kern, int64_t nCells, const d
         DevMech M = s->mech->dm;
         int nc32 = (int)nc;
         void* args[] = {&M, &nc32, &dRho, &dT, &dp, &dY, &dOut};
-        HIP_OK(hipLaunchKernel(kern, dim3((unsigned)nc), dim3(64), args, s->ldsBytes, st));
+        HIP_OK(hipLaunchKernel(kern, dim3((unsigned)nc), dim3(s->ks->G), args, s->ldsBytes, st));
         for (int i = 0; i < nOut; ++i)
             HIP_OK(hipMemcpyAsync(out[i], dOut + (size_t)i * nc, nc * 8, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
@@ -1635,16 +1701,17 @@ int fcs_lu_solve(fcs_solver* s, int64_t nSys, const double* A, double* b)
 {
     if (!s || nSys < 1 || nSys > 0x7fffffff || !A || !b) return FCS_E_ARG;
     const int n = s->mech->ns + 1;
-    const size_t nm = (size_t)n * n * nSys, nv = (size_t)n * nSys;
-    return with_device((nm + nv) * 8, [&](char* base) -> int {
+    const size_t nm = (size_t)n * n * nSys, nv = (size_t)n * nSys, ng = jg_per_block(s) * nSys;
+    return with_device((nm + nv + ng) * 8, [&](char* base) -> int {
         double* dA = (double*)base;
         double* db = dA + nm;
+        double* dG = ng ? db + nv : nullptr;
         HIP_OK(hipMemcpy(dA, A, nm * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(db, b, nv * 8, hipMemcpyHostToDevice));
         DevMech M = s->mech->dm;
         int n32 = (int)nSys;
-        void* args[] = {&M, &n32, &dA, &db};
-        HIP_OK(hipLaunchKernel(s->ks->lu, dim3((unsigned)nSys), dim3(64), args, s->ldsBytes, s->stream));
+        void* args[] = {&M, &n32, &dA, &db, &dG};
+        HIP_OK(hipLaunchKernel(s->ks->lu, dim3((unsigned)nSys), dim3(s->ks->G), args, s->ldsBytes, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         HIP_OK(hipMemcpy(b, db, nv * 8, hipMemcpyDeviceToHost));
         return FCS_OK;

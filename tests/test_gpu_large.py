@@ -1,0 +1,121 @@
+"""Parity of the workgroup form (systems of more than 64 rows: one
+workgroup per cell, matrix in HBM, block-4-pivoted LU with FP64 MFMA
+trailing updates) with the CPU oracle, on the 160-species / 1,200-reaction
+synthetic mechanism and an 80-species one (many padding rows)."""
+import numpy as np
+import pytest
+
+import fcs.mechanism as M
+from oracle import Oracle, lu_solve as orc_lu_solve
+
+from test_gpu_parity import assert_solve_parity, fcslib  # noqa: F401  (fixture)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mech4():
+    return M.config_mechanism(4)
+
+
+@pytest.fixture(scope="module")
+def mech80():
+    return M.synthetic_mechanism(80, 320, n_third_body=30, n_troe=10, n_plog=4, n_sri=3, n_noneq=4, seed=8080)
+
+
+def yt_states(mech, n, seed):
+    fld = M.uniform_field(mech, n, seed=seed)
+    return fld, np.vstack([fld.Y, fld.T[None, :]])
+
+
+@pytest.mark.parametrize("which", ["mech4", "mech80"])
+def test_large_lu_matches_oracle(fcslib, which, request):
+    """Block-4-pivoted LU + solve of the workgroup form against the oracle's
+    restatement of the reference LU, on random matrices that need pivoting
+    (small diagonals), plus the signed-maximum pivot case of the reference."""
+    mech = request.getfixturevalue(which)
+    n = mech.n_species + 1
+    s = fcslib.Solver(fcslib.Mech(mech))
+    rng = np.random.default_rng(17)
+    A = rng.standard_normal((6, n, n))
+    A[:3] += 4.0 * np.eye(n)                         # well conditioned
+    A[3:] *= rng.uniform(0.1, 10.0, (3, n, 1))       # row scaling: pivots move inside 4-blocks
+    A[5, 1:4, 0] = 0.0
+    A[5, 1, 1], A[5, 2, 1], A[5, 3, 1] = 1.0, -5.0, 2.0   # signed running maximum (LUsolver.C:177)
+    b = rng.standard_normal((6, n))
+    x = s.lu_solve(A, b)
+    for k in range(6):
+        xo, _, _ = orc_lu_solve(A[k], b[k])
+        np.testing.assert_allclose(x[k], xo, rtol=1e-10, atol=1e-11 * np.abs(xo).max())
+        assert np.abs(A[k] @ x[k] - b[k]).max() <= 1e-10 * np.abs(A[k]).max() * np.abs(x[k]).max()
+
+
+@pytest.mark.parametrize("which", ["mech4", "mech80"])
+def test_large_derivatives_and_jacobian(fcslib, which, request):
+    mech = request.getfixturevalue(which)
+    o = Oracle(mech)
+    fld, YT = yt_states(mech, 16, 3)
+    YT[4, :3] = -1e-4
+    for jac in ("exact", "fast"):
+        s = fcslib.Solver(fcslib.Mech(mech), jacobian=jac)
+        d, yc = s.derivatives(fld.p, YT)
+        f, J, _ = s.jacobian(fld.p, YT)
+        for c in range(16):
+            dref, yref = o.derivatives(fld.p[c], YT[:, c])
+            np.testing.assert_array_equal(yc[:, c], yref)
+            scale = np.abs(dref[:-1]).max()
+            assert np.abs(d[:-1, c] - dref[:-1]).max() <= 1e-11 * scale
+            assert d[-1, c] == pytest.approx(dref[-1], rel=1e-10, abs=1e-11 * scale)
+            fr, Jr, _ = o.jacobian(fld.p[c], YT[:, c], jac)
+            np.testing.assert_allclose(f[:, c], fr, rtol=1e-10, atol=1e-11 * np.abs(fr[:-1]).max())
+            rs = np.abs(Jr).max(axis=1, keepdims=True) + 1e-300
+            assert (np.abs(J[c] - Jr) / rs).max() <= 1e-9
+
+
+@pytest.mark.parametrize("solver", ["OptRodas34", "OptRosenbrock34", "OptRodas23"])
+def test_large_single_attempt(fcslib, mech4, solver):
+    o = Oracle(mech4)
+    s = fcslib.Solver(fcslib.Mech(mech4), solver=solver)
+    fld, YT = yt_states(mech4, 12, 4)
+    dx = np.repeat([1e-13, 1e-11, 1e-9], 4)
+    err, YTn, yc = s.attempt(fld.p, YT, dx)
+    for c in range(12):
+        er, ynr, yr = o.attempt(fld.p[c], YT[:, c], dx[c], solver=solver)
+        np.testing.assert_array_equal(yc[:, c], yr)
+        # The MFMA trailing update sums in another order than the reference,
+        # and grossly rejected attempts amplify round-off.  Above
+        # (0.9 / 0.2)^4 = 410 the controller's step scale is clamped at
+        # minScale, so there only the magnitude of Err matters.
+        assert err[c] == pytest.approx(er, rel=1e-4 if er < 410 else 1e-2, abs=1e-6), (c, err[c], er)
+        inc = np.abs(ynr - yr).max()
+        assert np.abs(YTn[:, c] - ynr).max() <= (1e-9 if er < 1.0 else 1e-3) * inc + 1e-15, c
+
+
+@pytest.mark.parametrize("which,jac", [("mech4", "exact"), ("mech4", "fast"), ("mech80", "exact")])
+def test_large_solve_matches_oracle(fcslib, which, jac, request):
+    mech = request.getfixturevalue(which)
+    fld = M.uniform_field(mech, 8, seed=21)
+    dt = 1e-6
+    g = fcslib.Solver(fcslib.Mech(mech), jacobian=jac).solve(fld, dt)
+    r = Oracle(mech).solve(fld, dt, threads=1, jacobian=jac)
+    assert g["rc"] == 0 and r["rc"] == 0
+    assert_solve_parity(g, r, fld, dt)
+
+
+def test_large_calculate_and_tc(fcslib, mech4):
+    o = Oracle(mech4)
+    fld = M.uniform_field(mech4, 16, seed=6)
+    rho = fld.p / (M.RU * fld.T * (fld.Y / mech4.W[:, None]).sum(axis=0))
+    s = fcslib.Solver(fcslib.Mech(mech4))
+    RR = s.calculate(rho, fld.T, fld.p, fld.Y)
+    tc = s.tc(rho, fld.T, fld.p, fld.Y)
+    for c in range(16):
+        rr = o.calculate(rho[c], fld.T[c], fld.p[c], fld.Y[:, c])
+        np.testing.assert_allclose(RR[:, c], rr, rtol=1e-10, atol=1e-11 * np.abs(rr).max())
+        assert tc[c] == pytest.approx(o.tc(rho[c], fld.T[c], fld.p[c], fld.Y[:, c]), rel=1e-10)
+
+
+def test_large_seulex_is_unsupported(fcslib, mech4):
+    from fcs.lib import FcsError
+    with pytest.raises(FcsError):
+        fcslib.Solver(fcslib.Mech(mech4), solver="OptSeulex")
