@@ -147,6 +147,79 @@ def mixture(args, mech, gm):
     print(json.dumps(line), flush=True)
 
 
+def large_kernels():
+    """The workgroup form's components through the kernel-level entries, on
+    2,048 random states of the 160-species mechanism (two launches each):
+    derivatives, Jacobian, LU factor + one solve, one Rodas34 attempt --
+    for per-kernel timing under a profiler."""
+    from fcs import lib
+    from fcs.mechanism import config_mechanism, uniform_field
+    mech = config_mechanism(4)
+    s = lib.Solver(lib.Mech(mech), solver="OptRodas34")
+    n, ns1 = 2048, mech.n_species + 1
+    fld = uniform_field(mech, n, seed=3)
+    YT = np.vstack([fld.Y, fld.T[None, :]])
+    A = np.random.default_rng(1).standard_normal((n, ns1, ns1)) + 4 * np.eye(ns1)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        s.derivatives(fld.p, YT)
+        s.jacobian(fld.p, YT)
+        s.lu_solve(A, np.ones((n, ns1)))
+        s.attempt(fld.p, YT, np.full(n, 1e-9))
+    print(json.dumps({"workload": "160-species kernel-level entries, 2,048 states, 2 launches each",
+                      "seconds": time.perf_counter() - t0}), flush=True)
+
+
+def large(args):
+    """Plain Rodas34 on the 160-species / 1,200-reaction mechanism over a
+    512-column counterflow field (131,072 cells by default): one workgroup
+    per cell, matrix slices in HBM, LU trailing updates on FP64 MFMA.  Runs
+    the exact and the fast Jacobian; the step kernel's roofline comes from an
+    event pair around every step launch of the exact run."""
+    from fcs import lib
+    from fcs.flops import attempt_flops
+    from fcs.mechanism import config_field, config_mechanism
+    mech = config_mechanism(4)
+    gm = lib.Mech(mech)
+    eqs = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
+    t0 = time.perf_counter()
+    field = config_field(4, mech, n_cells=args.large_cells, equilibrate=lib.equilibrator(eqs))
+    log(f"field: {field.n_cells} cells in {time.perf_counter() - t0:.1f} s")
+    runs = {}
+    for jac in ("exact", "fast"):
+        s = lib.Solver(gm, solver="OptRodas34", jacobian=jac, absTol=1e-8, relTol=1e-4, Treact=TREACT)
+        b = lib.Batch(s, field)
+        wall, launches, step_ms = timed_runs(b, args.steps, args.warmup, profile=(jac == "exact"))
+        r = b.download()
+        runs[jac] = dict(wall=wall, launches=launches, step_ms=step_ms, att=int(r["nAttempts"].sum()),
+                         failed=int(((r["status"] & 6) != 0).sum()))
+        log(f"{jac}: {wall / args.steps * 1e3:.1f} ms per step, {runs[jac]['att']} attempts")
+    ex = runs["exact"]
+    fpa = attempt_flops(mech, "OptRodas34")
+    avg_launch_s = 1e-3 * ex["step_ms"] / max(ex["launches"], 1)
+    achieved = fpa * ex["att"] * args.steps / max(ex["launches"], 1) / avg_launch_s / 1e12
+    line = {"metric": "cells·dt/s at 160-species mech (plain GPU Rodas34, workgroup-per-cell LU on FP64 MFMA)",
+            "value": field.n_cells / (ex["wall"] / args.steps), "unit": "cells·dt/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * ex["wall"] / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"plain OptRodas34, jacobian exact, 512x{field.n_cells // 512} synthetic "
+                                   f"counterflow field ({field.n_cells} cells), 160-species/1,200-reaction "
+                                   "synthetic mechanism, deltaT 1e-6 s, Treact 500 K",
+                       "cells": field.n_cells, "attempts_per_step": ex["att"] // args.steps,
+                       "failed_cells": ex["failed"]},
+            "fast_jacobian": {"value": field.n_cells / (runs["fast"]["wall"] / args.steps),
+                              "attempts_per_step": runs["fast"]["att"] // args.steps,
+                              "failed_cells": runs["fast"]["failed"]},
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                         "kernel": "k_step<176,Rodas34> (one workgroup of 256 threads per active cell)",
+                         "flops_per_attempt": fpa, "launches": ex["launches"],
+                         "avg_launch_ms": 1e3 * avg_launch_s}}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(mech, field, seconds=args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,16 +229,25 @@ def main():
     ap.add_argument("--no-secondary", action="store_true", help="skip the 65,536-cell plain run")
     ap.add_argument("--train-side", type=int, default=256, help="side of each training field")
     ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt"), default="counterflow",
+    ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt", "large", "large-kernels"),
+                    default="counterflow",
                     help="counterflow: learned-seeded vs plain Rodas34 on the 262,144-cell field (default); "
                          "mixture: plain Rodas34 on the 65,536-cell mixture-fraction field only; "
-                         "attempt: two single-attempt launches on 1,024 cells (for counter collection)")
+                         "attempt: two single-attempt launches on 1,024 cells (for counter collection); "
+                         "large: plain Rodas34 on the 160-species mechanism's counterflow field; "
+                         "large-kernels: its components through the kernel-level entries (for profiling)")
+    ap.add_argument("--large-cells", type=int, default=131072, help="cells of the large workload (x 512)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); cells are independent, "
               "so the path shards by cell ranges with no collective on the data path.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "large":
+        return large(args)
+    if args.workload == "large-kernels":
+        return large_kernels()
     from fcs import learned, lib
     from fcs.flops import attempt_flops
     from fcs.mechanism import config_field, config_mechanism, counterflow_field
@@ -296,7 +378,7 @@ def main():
     if secondary:
         line["plain_mixture_fraction_65536"] = secondary
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(mech, field)
+        line["cpu_baseline"] = cpu_baseline(mech, field, seconds=args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

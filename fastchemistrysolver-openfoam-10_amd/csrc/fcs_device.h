@@ -1,7 +1,7 @@
 // fcs_device.h -- gfx950 device code of the per-cell stiff chemistry
 // integrator: one cell per 64-lane wavefront for systems up to 64 rows
-// (matrix in LDS), one cell per workgroup of G = 64 * ceil(NP / 64) threads
-// beyond that (matrix in HBM, LU in fcs_lu_large.h).
+// (matrix in LDS), one cell per workgroup of G = 256 threads (four
+// wavefronts) for up to 256 rows (matrix in HBM, LU in fcs_lu_large.h).
 //
 // Layout of one cell (NP = padded system size, a multiple of 4; a multiple
 // of 16 for the workgroup form; thread = lane of the wavefront or thread of
@@ -45,7 +45,7 @@ constexpr int JS = 65;
 template <int NP>
 struct Cell {
     static constexpr bool kLarge = NP > 64;                       // workgroup per cell, matrix in HBM
-    static constexpr int G = kLarge ? (NP + 63) / 64 * 64 : 64;  // threads per cell
+    static constexpr int G = kLarge ? 256 : 64;                   // threads per cell (all 4 SIMDs of a CU)
     static constexpr int NW = G / 64;                             // wavefronts per cell
     static constexpr int SW = G;                                  // species stride of per-species tables
     static constexpr int ONE = kLarge ? 255 : 64;                 // concentration slot holding 1.0
@@ -222,7 +222,7 @@ __host__ __device__ inline size_t lu_scratch_doubles(int NP)
     return 8 * LU_PW + LU_PW * LU_PW + 2 * (size_t)LU_PW * NP + 3 * LU_PW + LU_PW / 2 + 8;
 }
 
-__host__ __device__ inline int group_of(int NP) { return NP > 64 ? (NP + 63) / 64 * 64 : 64; }
+__host__ __device__ inline int group_of(int NP) { return NP > 64 ? 256 : 64; }
 
 // LDS doubles per cell.  Wavefront form: four wavefronts of the 53-species
 // mechanism fit the 160 KiB of a CU (40,112 B each).  Workgroup form: the

@@ -255,23 +255,41 @@ __device__ inline LuLarge lu_factor_large(const Lds& L)
         const int m0 = k0 + PW, nT = (NP - m0) / 16;
         const int lane = lane_id(), w = t >> 6;
         const int li = lane & 15, lk = lane >> 4;
-        for (int tile = w; tile < nT * nT; tile += Cell<NP>::NW) {
-            const int r0 = m0 + 16 * (tile % nT), c0 = m0 + 16 * (tile / nT);
-            // C / D layout of v_mfma_f64_16x16x4f64: lane l, element i -> row
-            // l / 16 + 4 i, column l % 16 (checked on the device)
-            double* cp = J + (size_t)(c0 + li) * NP + r0 + lk;
-            dbl4 acc = {cp[0], cp[4], cp[8], cp[12]};
+        // tiles of this wavefront in groups of 4: the C loads of a group are
+        // all in flight before its MFMAs
+        const int nTiles = nT * nT;
+        for (int base = w; base < nTiles; base += 4 * Cell<NP>::NW) {
+            dbl4 acc[4];
+            int r0[4], c0[4];
 #pragma unroll
-            for (int s = 0; s < PW / 4; ++s) {
-                // A (16 x 4): lane l -> row l % 16, k l / 16;  B (4 x 16): lane l -> k l / 16, column l % 16
-                const double a = S.negL[(4 * s + lk) * NP + r0 + li];
-                const double bb = S.U12[(4 * s + lk) * NP + c0 + li];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+            for (int u = 0; u < 4; ++u) {
+                const int tile = base + u * Cell<NP>::NW;   // uniform over the wavefront
+                r0[u] = m0 + 16 * (tile % nT);
+                c0[u] = m0 + 16 * (tile / nT);
+                if (tile < nTiles) {
+                    // C / D layout of v_mfma_f64_16x16x4f64: lane l, element i ->
+                    // row l / 16 + 4 i, column l % 16 (checked on the device)
+                    const double* cp = J + (size_t)(c0[u] + li) * NP + r0[u] + lk;
+                    acc[u] = dbl4{cp[0], cp[4], cp[8], cp[12]};
+                }
             }
-            cp[0] = acc[0];
-            cp[4] = acc[1];
-            cp[8] = acc[2];
-            cp[12] = acc[3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (base + u * Cell<NP>::NW < nTiles) {
+#pragma unroll
+                    for (int s4 = 0; s4 < PW / 4; ++s4) {
+                        // A (16 x 4): lane l -> row l % 16, k l / 16;  B (4 x 16): lane l -> k l / 16, column l % 16
+                        const double a = S.negL[(4 * s4 + lk) * NP + r0[u] + li];
+                        const double bb = S.U12[(4 * s4 + lk) * NP + c0[u] + li];
+                        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[u], 0, 0, 0);
+                    }
+                    double* cp = J + (size_t)(c0[u] + li) * NP + r0[u] + lk;
+                    cp[0] = acc[u][0];
+                    cp[4] = acc[u][1];
+                    cp[8] = acc[u][2];
+                    cp[12] = acc[u][3];
+                }
+            }
         }
         __syncthreads();
     }
@@ -282,7 +300,8 @@ __device__ inline LuLarge lu_factor_large(const Lds& L)
 
 // Solves (L U) x = P b (src/LUsolver/LUsolver.C:3922-4111: unit-L forward,
 // U backward scaled by 1/pivot); b is this thread's element in the original
-// row order; returns x_t for thread t.
+// row order; returns x_t for thread t.  The factors of the next panel are
+// loaded while the current one is solved.
 template <int NP>
 __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double b)
 {
@@ -293,20 +312,36 @@ __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double 
     const double* invDg = L.J + (size_t)NP * NP;
     const LuScratch S = lu_scratch<NP>(L);
     const int lane = lane_id();
+    const bool diag = t < PW;   // wavefront 0, lanes 0..15: the diagonal block
+    // row updates (lr: this thread's factors in the panel's columns) and the
+    // diagonal block's factors (dq: lane q = logical row k0 + q)
+    double lr[PW], dq[PW], lrN[PW], dqN[PW];
+    auto load_fwd = [&](int k0, double (&l)[PW], double (&d)[PW]) {
+#pragma unroll
+        for (int r = 0; r < PW; ++r) l[r] = (row && t >= k0 + PW) ? J[(size_t)(k0 + r) * NP + t] : 0.0;
+        const int ph = diag ? L.ip[k0 + lane] : 0;
+#pragma unroll
+        for (int r = 0; r < PW; ++r) d[r] = (diag && r < lane) ? J[(size_t)(k0 + r) * NP + ph] : 0.0;
+    };
+    auto load_bwd = [&](int k0, double (&l)[PW], double (&d)[PW]) {
+#pragma unroll
+        for (int r = 0; r < PW; ++r) l[r] = (row && t < k0) ? J[(size_t)(k0 + r) * NP + t] : 0.0;
+        const int ph = diag ? L.ip[k0 + lane] : 0;
+#pragma unroll
+        for (int r = 0; r < PW; ++r) d[r] = (diag && r > lane) ? J[(size_t)(k0 + r) * NP + ph] : 0.0;
+    };
     // forward: y = L^-1 P b, panel by panel in logical order
+    load_fwd(0, lr, dq);
     for (int k0 = 0; k0 < NP; k0 += PW) {
+        if (k0 + PW < NP) load_fwd(k0 + PW, lrN, dqN);
         if (t >= k0 && t < k0 + PW) S.v0[st.pos - k0] = b;
         __syncthreads();
-        if (t < PW) {   // wavefront 0, lanes 0..15: lane q holds logical row k0+q
-            const int ph = L.ip[k0 + lane];
-            double lq[PW];
-#pragma unroll
-            for (int r = 0; r < PW; ++r) lq[r] = r < lane ? J[(size_t)(k0 + r) * NP + ph] : 0.0;
+        if (diag) {
             double y = S.v0[lane];
 #pragma unroll
             for (int r = 0; r < PW; ++r) {
                 const double yr = __shfl(y, r, 16);
-                if (lane > r) y = y - lq[r] * yr;
+                if (lane > r) y = y - dq[r] * yr;
             }
             S.v1[lane] = y;
         }
@@ -314,30 +349,33 @@ __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double 
         if (row && t >= k0 + PW) {
             double acc = b;
 #pragma unroll
-            for (int r = 0; r < PW; ++r) acc = acc - J[(size_t)(k0 + r) * NP + t] * S.v1[r];
+            for (int r = 0; r < PW; ++r) acc = acc - lr[r] * S.v1[r];
             b = acc;
         } else if (t >= k0 && t < k0 + PW) {
             b = S.v1[st.pos - k0];
         }
+#pragma unroll
+        for (int r = 0; r < PW; ++r) {
+            lr[r] = lrN[r];
+            dq[r] = dqN[r];
+        }
     }
     // backward: x = U^-1 y, panels in descending logical order
     double x = 0.0;
+    load_bwd(NP - PW, lr, dq);
     for (int k0 = NP - PW; k0 >= 0; k0 -= PW) {
+        if (k0 >= PW) load_bwd(k0 - PW, lrN, dqN);
+        const double id = diag ? invDg[L.ip[k0 + lane]] : 0.0;
         __syncthreads();
         if (t >= k0 && t < k0 + PW) S.v0[st.pos - k0] = b;
         __syncthreads();
-        if (t < PW) {
-            const int ph = L.ip[k0 + lane];
-            double uq[PW];
-#pragma unroll
-            for (int r = 0; r < PW; ++r) uq[r] = r > lane ? J[(size_t)(k0 + r) * NP + ph] : 0.0;
-            const double id = invDg[ph];
+        if (diag) {
             double v = S.v0[lane];
 #pragma unroll
             for (int r = PW - 1; r >= 0; --r) {
                 if (lane == r) v = v * id;
                 const double xr = __shfl(v, r, 16);
-                if (lane < r) v = v - uq[r] * xr;
+                if (lane < r) v = v - dq[r] * xr;
             }
             S.v1[lane] = v;
         }
@@ -345,10 +383,15 @@ __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double 
         if (row && t < k0) {
             double acc = b;
 #pragma unroll
-            for (int r = PW - 1; r >= 0; --r) acc = acc - J[(size_t)(k0 + r) * NP + t] * S.v1[r];
+            for (int r = PW - 1; r >= 0; --r) acc = acc - lr[r] * S.v1[r];
             b = acc;
         }
         if (t >= k0 && t < k0 + PW) x = S.v1[t - k0];
+#pragma unroll
+        for (int r = 0; r < PW; ++r) {
+            lr[r] = lrN[r];
+            dq[r] = dqN[r];
+        }
     }
     __syncthreads();
     return x;

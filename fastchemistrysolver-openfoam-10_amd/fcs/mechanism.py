@@ -438,8 +438,9 @@ def mixture_fraction_field(mech: Mechanism, n_cells, seed, T_peak=2200.0, T_end=
 
 
 def counterflow_field(mech: Mechanism, n_side, seed, equilibrate, T_react=500.0, n_base=16,
-                      p=1.0e5, dtchem0=1e-7) -> Field:
-    """Synthetic 2-D counterflow field of n_side x n_side cells, shuffled.
+                      p=1.0e5, dtchem0=1e-7, n_rows=None) -> Field:
+    """Synthetic 2-D counterflow field of n_side x n_rows cells (n_rows
+    defaults to n_side), shuffled.
 
     Across the burner axis x in [0, 1] the cells fall into three bands: the
     two cold inflow streams |x - 0.5| > 0.275 (45 %, T in [300, 480) K, below
@@ -455,11 +456,12 @@ def counterflow_field(mech: Mechanism, n_side, seed, equilibrate, T_react=500.0,
     on Y, so a 1 us flow step takes the integrator only a few sub-steps.
     """
     rng = np.random.default_rng(seed)
-    n = n_side * n_side
+    rows = n_side if n_rows is None else n_rows
+    n = n_side * rows
     Yox = rng.dirichlet(np.ones(mech.n_species))
     Yfu = rng.dirichlet(np.ones(mech.n_species))
     x = (np.arange(n) % n_side + 0.5) / n_side
-    yy = (np.arange(n) // n_side + 0.5) / n_side
+    yy = (np.arange(n) // n_side + 0.5) / rows
     d = np.abs(x - 0.5)
     cold = d > 0.275
     flame = d <= 0.075
@@ -515,6 +517,14 @@ def config_field(cfg: int, mech: Mechanism, n_cells=None, equilibrate=None) -> F
         if equilibrate is None:
             raise ValueError("configuration 2/3 fields need an equilibrate callable")
         return counterflow_field(mech, side, seed=1054, equilibrate=equilibrate)
+    if cfg == 4:
+        # 512 x 256 counterflow field (131,072 cells) of the 160-species mechanism
+        n = n_cells or 131072
+        if n % 512:
+            raise ValueError("configuration 4 fields have 512 cells per row: n_cells must be a multiple of 512")
+        if equilibrate is None:
+            raise ValueError("configuration 4 fields need an equilibrate callable")
+        return counterflow_field(mech, 512, seed=1161, equilibrate=equilibrate, n_rows=n // 512)
     raise ValueError(cfg)
 
 
