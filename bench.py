@@ -171,49 +171,75 @@ def large_kernels():
 
 
 def large(args):
-    """Plain Rodas34 on the 160-species / 1,200-reaction mechanism over a
-    512-column counterflow field (131,072 cells by default): one workgroup
-    per cell, matrix slices in HBM, LU trailing updates on FP64 MFMA.  Runs
-    the exact and the fast Jacobian; the step kernel's roofline comes from an
-    event pair around every step launch of the exact run."""
-    from fcs import lib
+    """Configuration 4: the 160-species / 1,200-reaction mechanism over a
+    512-column counterflow field (131,072 cells by default), one workgroup
+    per cell, matrix slices in HBM, LU trailing updates on FP64 MFMA.
+    `value` is the learned-seeded Rodas34 run (network 163-256-256-256-162
+    trained first, untimed, on two 512 x 16 fields); plain Rodas34 with the
+    exact and the fast Jacobian and plain Seulex run beside it.  The step
+    kernel's roofline comes from an event pair around every step launch of
+    the plain exact run."""
+    from fcs import learned, lib
     from fcs.flops import attempt_flops
-    from fcs.mechanism import config_field, config_mechanism
+    from fcs.mechanism import config_field, config_mechanism, counterflow_field
     mech = config_mechanism(4)
     gm = lib.Mech(mech)
-    eqs = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
+    exact = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
+    eq = lib.equilibrator(exact)
     t0 = time.perf_counter()
-    field = config_field(4, mech, n_cells=args.large_cells, equilibrate=lib.equilibrator(eqs))
+    field = config_field(4, mech, n_cells=args.large_cells, equilibrate=eq)
     log(f"field: {field.n_cells} cells in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    Xs, Ts = zip(*(learned.samples(exact, counterflow_field(mech, 512, seed=sd, equilibrate=eq, n_rows=16), DT)
+                   for sd in (4001, 4002)))
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    net, hist = learned.train(X, T, hidden=(256, 256, 256), epochs=args.epochs, batch=512, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    log(f"training: {X.shape[0]} samples, loss {hist[0]:.3g} -> {hist[-1]:.3g}, {t_train:.1f} s")
     runs = {}
-    for jac in ("exact", "fast"):
-        s = lib.Solver(gm, solver="OptRodas34", jacobian=jac, absTol=1e-8, relTol=1e-4, Treact=TREACT)
+    for name, solver, jac, mlp in (("plain", "OptRodas34", "exact", None), ("learned", "OptRodas34", "exact", net),
+                                   ("fast", "OptRodas34", "fast", None), ("seulex", "OptSeulex", "exact", None)):
+        s = exact if (solver, jac) == ("OptRodas34", "exact") else \
+            lib.Solver(gm, solver=solver, jacobian=jac, absTol=1e-8, relTol=1e-4, Treact=TREACT)
         b = lib.Batch(s, field)
-        wall, launches, step_ms = timed_runs(b, args.steps, args.warmup, profile=(jac == "exact"))
+        if mlp is not None:
+            b.set_learned(mlp, "verify")
+        wall, launches, step_ms = timed_runs(b, args.steps, args.warmup, profile=(name == "plain"))
         r = b.download()
-        runs[jac] = dict(wall=wall, launches=launches, step_ms=step_ms, att=int(r["nAttempts"].sum()),
-                         failed=int(((r["status"] & 6) != 0).sum()))
-        log(f"{jac}: {wall / args.steps * 1e3:.1f} ms per step, {runs[jac]['att']} attempts")
-    ex = runs["exact"]
+        runs[name] = dict(wall=wall, launches=launches, step_ms=step_ms, att=int(r["nAttempts"].sum()),
+                          failed=int(((r["status"] & 6) != 0).sum()),
+                          value=field.n_cells / (wall / args.steps))
+        if mlp is not None:
+            runs[name]["stats"] = b.learned_stats()
+        log(f"{name}: {wall / args.steps * 1e3:.1f} ms per step, {runs[name]['att']} attempts")
+        del b
+    pl, ln = runs["plain"], runs["learned"]
     fpa = attempt_flops(mech, "OptRodas34")
-    avg_launch_s = 1e-3 * ex["step_ms"] / max(ex["launches"], 1)
-    achieved = fpa * ex["att"] * args.steps / max(ex["launches"], 1) / avg_launch_s / 1e12
-    line = {"metric": "cells·dt/s at 160-species mech (plain GPU Rodas34, workgroup-per-cell LU on FP64 MFMA)",
-            "value": field.n_cells / (ex["wall"] / args.steps), "unit": "cells·dt/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * ex["wall"] / args.steps,
+    avg_launch_s = 1e-3 * pl["step_ms"] / max(pl["launches"], 1)
+    achieved = fpa * pl["att"] / max(pl["launches"], 1) / avg_launch_s / 1e12
+    npred, nver, nacc = ln["stats"]
+    line = {"metric": "cells·dt/s at 160-species mech (learned-seeded vs plain GPU Rodas34, workgroup-per-cell "
+                      "LU on FP64 MFMA)",
+            "value": ln["value"], "unit": "cells·dt/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * ln["wall"] / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": f"plain OptRodas34, jacobian exact, 512x{field.n_cells // 512} synthetic "
-                                   f"counterflow field ({field.n_cells} cells), 160-species/1,200-reaction "
-                                   "synthetic mechanism, deltaT 1e-6 s, Treact 500 K",
-                       "cells": field.n_cells, "attempts_per_step": ex["att"] // args.steps,
-                       "failed_cells": ex["failed"]},
-            "fast_jacobian": {"value": field.n_cells / (runs["fast"]["wall"] / args.steps),
-                              "attempts_per_step": runs["fast"]["att"] // args.steps,
+            "config": {"workload": f"learned-seeded OptRodas34 (verify mode), jacobian exact, 512x"
+                                   f"{field.n_cells // 512} synthetic counterflow field ({field.n_cells} cells), "
+                                   "160-species/1,200-reaction synthetic mechanism, deltaT 1e-6 s, Treact 500 K",
+                       "cells": field.n_cells, "network": "163-256-256-256-162 SiLU, BF16 MFMA / FP32 accumulation",
+                       "training_samples": int(X.shape[0]), "training_s": t_train,
+                       "proposals": npred, "verified": nver, "accepted": nacc,
+                       "attempts_per_step": ln["att"] // args.steps, "failed_cells": ln["failed"]},
+            "plain": {"value": pl["value"], "attempts_per_step": pl["att"] // args.steps, "failed_cells": pl["failed"],
+                      "speedup_learned_vs_plain": ln["value"] / pl["value"]},
+            "fast_jacobian": {"value": runs["fast"]["value"], "attempts_per_step": runs["fast"]["att"] // args.steps,
                               "failed_cells": runs["fast"]["failed"]},
+            "seulex": {"value": runs["seulex"]["value"], "columns_per_step": runs["seulex"]["att"] // args.steps,
+                       "failed_cells": runs["seulex"]["failed"]},
             "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                         "kernel": "k_step<176,Rodas34> (one workgroup of 256 threads per active cell)",
-                         "flops_per_attempt": fpa, "launches": ex["launches"],
+                         "kernel": "k_step<176,Rodas34> (one workgroup of 256 threads per active cell), plain run",
+                         "flops_per_attempt": fpa, "launches": pl["launches"],
                          "avg_launch_ms": 1e3 * avg_launch_s}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mech, field, seconds=args.cpu_seconds)

@@ -355,29 +355,35 @@ __constant__ const SeulexConst kSeulex = make_seulex_const();
 
 // The Jacobian Jy of a Seulex cell persists across its steps (it is
 // re-evaluated only when theta > jacRedo, src/ODE/OptSeulex/OptSeulex.C:
-// 314-318, 483-490), so it lives in HBM between step launches, per cell, in
-// the lane-interleaved layout of the LDS matrix: element (i, j) at
-// JyG[j*64 + i].  Each seul() forms its iteration matrix from it.
+// 314-318, 483-490), so it lives in HBM between step launches, per cell,
+// column-major: element (i, j) at JyG[j*JYS + i] (JYS = 64 for the
+// wavefront form, NP for the workgroup form).  Each seul() forms its
+// iteration matrix from it.
+template <int NP>
+__device__ __forceinline__ constexpr int jy_stride() { return Cell<NP>::kLarge ? NP : WAVE; }
+
 template <int NP>
 __device__ __forceinline__ void jy_store(const Lds& L, double* JyG)
 {
-    const int lane = lane_id();
-    for (int j = 0; j < NP; ++j) JyG[j * WAVE + lane] = L.J[j * JS + lane];
+    const int lane = tid<NP>();
+    if (!Cell<NP>::kLarge || lane < NP)
+        for (int j = 0; j < NP; ++j) JyG[j * jy_stride<NP>() + lane] = L.J[j * Cell<NP>::JSt + lane];
 }
 
 // Iteration matrix diag*I - Jy in the LDS matrix, Jy read from HBM.
 template <int NP>
 __device__ __forceinline__ void form_matrix_from(const Lds& L, const double* JyG, double diag)
 {
-    const int lane = lane_id();
+    const int lane = tid<NP>();
     double* Jrow = L.J + lane;
-    for (int j0 = 0; j0 < NP; j0 += 4) {
-        double v[4];
+    if (!Cell<NP>::kLarge || lane < NP)
+        for (int j0 = 0; j0 < NP; j0 += 4) {
+            double v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = JyG[(j0 + u) * WAVE + lane];
+            for (int u = 0; u < 4; ++u) v[u] = JyG[(j0 + u) * jy_stride<NP>() + lane];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) Jrow[(j0 + u) * JS] = (lane == j0 + u ? diag : 0.0) - v[u];
-    }
+            for (int u = 0; u < 4; ++u) Jrow[(j0 + u) * Cell<NP>::JSt] = (lane == j0 + u ? diag : 0.0) - v[u];
+        }
     wave_sync();
 }
 
@@ -388,10 +394,10 @@ template <int NP>
 __device__ bool seul(const DevMech& M, const Lds& L, const double* JyG, double p, double lnp, double& y0s,
                      double dxTot, int k, int nSteps, double scale, double& theta, double& ySeq)
 {
-    const int lane = lane_id();
+    const int lane = tid<NP>();
     const double invdx = nSteps / dxTot;
     form_matrix_from<NP>(L, JyG, invdx);
-    const LuState st = lu_factor<NP>(L);
+    const auto st = lu_factor_any<NP>(L);
     double yT = 0.0, dy = 0.0;
     int nn = 0;
     bool thetaPhase = false;
@@ -400,11 +406,12 @@ __device__ bool seul(const DevMech& M, const Lds& L, const double* JyG, double p
         double ev = (nn == 0) ? y0s : yT;
         const double f = derivatives<NP>(M, L, p, lnp, ev);
         if (nn == 0) y0s = ev; else yT = ev;
-        dy = lu_solve<NP>(L, st, thetaPhase ? f - dy * invdx : f);
+        dy = lu_solve_any<NP>(L, st, thetaPhase ? f - dy * invdx : f);
         if (nn == 0) yT = y0s;
         if (thetaPhase) {
-            const bool huge = wave_any(lane < NP && fabs(dy) > sqrt(OF_VGREAT));
-            double dy2 = huge ? OF_VGREAT : wave_sum(lane < NP ? (dy / scale) * (dy / scale) : 0.0);
+            const bool huge = gany<NP>(lane < NP && fabs(dy) > sqrt(OF_VGREAT), L);
+            const double s2 = gsum<NP>(lane < NP ? (dy / scale) * (dy / scale) : 0.0, L);
+            double dy2 = huge ? OF_VGREAT : s2;
             dy2 = sqrt(dy2);
             const double den = 1.0 > dy1 + OF_SMALL ? 1.0 : dy1 + OF_SMALL;
             theta = dy2 / den;
@@ -416,7 +423,7 @@ __device__ bool seul(const DevMech& M, const Lds& L, const double* JyG, double p
         if (nn >= nSteps) break;
         yT += dy;
         if (nn == 1 && k <= 1) {
-            dy1 = sqrt(wave_sum(lane < NP ? (dy / scale) * (dy / scale) : 0.0));
+            dy1 = sqrt(gsum<NP>(lane < NP ? (dy / scale) * (dy / scale) : 0.0, L));
             thetaPhase = true;
         }
     }
@@ -448,7 +455,7 @@ __device__ int seulex_pass(const DevMech& M, const Lds& L, const DevOpts& o, dou
                            double* JyG, double& y, double& y0s, double& scale, SeulexCell& c)
 {
     const SeulexConst& S = kSeulex;
-    const int lane = lane_id();
+    const int lane = tid<NP>();
     const double lnp = log(p);
     const double kF1 = 0.7, kF2 = 0.9, sF1 = 0.8, sF2 = 0.93, sF3 = 0.1, sF4 = 4, sF5 = 0.5;
     const double jacRedo = o.relTol < 1e-4 ? o.relTol : 1e-4;
@@ -508,7 +515,7 @@ __device__ int seulex_pass(const DevMech& M, const Lds& L, const DevOpts& o, dou
         y = table[0] + S.coeff[k][0] * (table[0] - y);
         scale = o.absTol + o.relTol * fabs(y0s);
         const double dd = (y - table[0]) / scale;
-        const double err = sqrt(wave_sum(lane < NP ? dd * dd : 0.0) / (M.ns + 1));
+        const double err = sqrt(gsum<NP>(lane < NP ? dd * dd : 0.0, L) / (M.ns + 1));
         if (!(err == err)) return CELL_NONFINITE;
         if (err > 1 / OF_SMALL || (k > 1 && err >= errOld)) {
             reject = true;

@@ -180,13 +180,14 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step(DevMech M, DevOpts o, Cell
 // list (same list protocol as k_step).  The per-cell Jacobian lives in
 // a.Jy between launches.
 template <int NP>
-__global__ __launch_bounds__(64) void k_step_seulex(DevMech M, DevOpts o, CellArrays a, const int* listIn,
-                                                    const unsigned int* countIn, int* listOut,
-                                                    unsigned int* countOut, unsigned int* countZero)
+__global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(DevMech M, DevOpts o, CellArrays a,
+                                                             const int* listIn, const unsigned int* countIn,
+                                                             int* listOut, unsigned int* countOut,
+                                                             unsigned int* countZero)
 {
     extern __shared__ double lds_raw[];
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int lane = lane_id();
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
+    const int lane = tid<NP>();
     const int ns = M.ns;
     const size_t nc = (size_t)a.nc;
     const int row = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(64) void k_step_seulex(DevMech M, DevOpts o, CellAr
         sc.flags = a.flags[c];
         sc.nStep = a.nStep[c];
         sc.nAtt = a.nAtt[c];
-        double* JyG = a.Jy + (size_t)c * NP * WAVE;
+        double* JyG = a.Jy + (size_t)c * NP * jy_stride<NP>();
         const int st = seulex_pass<NP>(M, L, o, a.p0[c], a.dtCell ? a.dtCell[c] : a.dt, JyG, y, y0s, scale, sc);
         if (st == STEP_CONTINUE) {
             if (row >= 0) {
@@ -264,14 +265,14 @@ __global__ void k_features(int ns, CellArrays a, const int* list, const unsigned
 // network -- or, with seed != 0, starting from min(h, deltaT) instead of
 // deltaTChem.
 template <int NP>
-__global__ __launch_bounds__(64) void k_verify(DevMech M, DevOpts o, CellArrays a, const int* listIn,
-                                               const unsigned int* countIn, const float* P, int seed,
-                                               int* listOut, unsigned int* countOut,
-                                               unsigned long long* nAccepted)
+__global__ __launch_bounds__(Cell<NP>::G) void k_verify(DevMech M, DevOpts o, CellArrays a, const int* listIn,
+                                                        const unsigned int* countIn, const float* P, int seed,
+                                                        int* listOut, unsigned int* countOut,
+                                                        unsigned long long* nAccepted)
 {
     extern __shared__ double lds_raw[];
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
-    const int lane = lane_id();
+    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
+    const int lane = tid<NP>();
     const int ns = M.ns;
     const size_t nc = (size_t)a.nc;
     const int row = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
@@ -492,12 +493,13 @@ struct KernelSet {
      {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>, nullptr,         \
       (const void*)k_attempt<N, SOLVER_RODAS23>, (const void*)k_attempt<N, SOLVER_ROS23>},                 \
      (const void*)k_calculate<N>, (const void*)k_tc<N>},
-// workgroup form: Rosenbrock solvers only (no Seulex, no learned-seed verification)
+// workgroup form
 #define FCS_KSET_LARGE(N)                                                                                  \
     {N, group_of(N),                                                                                       \
-     {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>, nullptr,               \
-      (const void*)k_step<N, SOLVER_RODAS23>, (const void*)k_step<N, SOLVER_ROS23>},                       \
-     nullptr,                                                                                              \
+     {(const void*)k_step<N, SOLVER_RODAS34>, (const void*)k_step<N, SOLVER_ROS34>,                        \
+      (const void*)k_step_seulex<N>, (const void*)k_step<N, SOLVER_RODAS23>,                               \
+      (const void*)k_step<N, SOLVER_ROS23>},                                                               \
+     (const void*)k_verify<N>,                                                                             \
      (const void*)k_derivatives<N>, (const void*)k_jacobian<N>, (const void*)k_lu_solve<N>,                \
      {(const void*)k_attempt<N, SOLVER_RODAS34>, (const void*)k_attempt<N, SOLVER_ROS34>, nullptr,         \
       (const void*)k_attempt<N, SOLVER_RODAS23>, (const void*)k_attempt<N, SOLVER_ROS23>},                 \
@@ -1068,7 +1070,9 @@ int fcs_solver_destroy(fcs_solver* s)
 // Per-cell Jacobian storage of the Seulex solver (doubles), 0 otherwise.
 static size_t jy_doubles(const fcs_solver* s)
 {
-    return s->opts.solver == FCS_SOLVER_SEULEX ? (size_t)s->ks->NP * WAVE : 0;
+    // per-cell Jy of the Seulex solver: NP columns of 64 (wavefront form) or NP rows
+    const size_t NP = (size_t)s->ks->NP;
+    return s->opts.solver == FCS_SOLVER_SEULEX ? NP * (NP > 64 ? NP : (size_t)WAVE) : 0;
 }
 
 static DevOpts dev_opts(const fcs_solver_opts& o)
@@ -1138,7 +1142,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
             unsigned int* c0 = d.counts;
             unsigned int* c1 = d.counts + 1;
             void* args[] = {&M, &o, &a, &d.list0, &c0, &d.Pf, &seed, &d.list1, &c1, &d.nAccepted};
-            HIP_OK(hipLaunchKernel(s->ks->verify, dim3(std::max(1, std::min(s->grid, (int)n0))), dim3(64), args,
+            HIP_OK(hipLaunchKernel(s->ks->verify, dim3(std::max(1, std::min(grid, (int)n0))), dim3(blk), args,
                                    s->ldsBytes, st));
             it0 = 1;
         }

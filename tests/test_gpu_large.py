@@ -115,7 +115,55 @@ def test_large_calculate_and_tc(fcslib, mech4):
         assert tc[c] == pytest.approx(o.tc(rho[c], fld.T[c], fld.p[c], fld.Y[:, c]), rel=1e-10)
 
 
-def test_large_seulex_is_unsupported(fcslib, mech4):
-    from fcs.lib import FcsError
-    with pytest.raises(FcsError):
-        fcslib.Solver(fcslib.Mech(mech4), solver="OptSeulex")
+@pytest.mark.parametrize("which,n", [("mech4", 4), ("mech80", 8)])
+def test_large_seulex_matches_oracle(fcslib, which, n, request):
+    """OptSeulex in the workgroup form (per-cell Jacobian kept in HBM between
+    steps, NP x NP doubles per cell)."""
+    mech = request.getfixturevalue(which)
+    fld = M.uniform_field(mech, n, seed=21)
+    dt = 1e-6
+    g = fcslib.Solver(fcslib.Mech(mech), solver="OptSeulex").solve(fld, dt)
+    r = Oracle(mech).solve(fld, dt, threads=1, solver="OptSeulex")
+    assert g["rc"] == 0 and r["rc"] == 0
+    assert_solve_parity(g, r, fld, dt)
+
+
+def test_large_learned_seed(fcslib, mech4):
+    """The learned seed on the 160-species mechanism (network 163 -> 256 x 3
+    -> 162): fallback cells bit-identical to the plain integrator, accepted
+    proposals within the parity bar of it, attempts saved on the
+    near-equilibrium product cells."""
+    from fcs import learned
+    dt = 1e-6
+    s = fcslib.Solver(fcslib.Mech(mech4), Treact=500.0)
+    eq = fcslib.equilibrator(s)
+    fields = [M.counterflow_field(mech4, 512, seed=sd, equilibrate=eq, n_rows=2) for sd in (31, 32, 33)]
+    Xs, Ts = zip(*(learned.samples(s, f, dt) for f in fields[:2]))
+    net, hist = learned.train(np.concatenate(Xs), np.concatenate(Ts), hidden=(256, 256, 256), epochs=40,
+                              batch=256, lr=2e-3, seed=1)
+    assert hist[-1] < 0.5 * hist[0]
+    f = fields[2]
+
+    def run(mlp=None):
+        b = fcslib.Batch(s, f)
+        if mlp is not None:
+            b.set_learned(mlp, "verify")
+        b.run(dt)
+        return b.download(), b.learned_stats()
+    plain, _ = run()
+    lrn, (npred, nver, nacc) = run(net)
+    react = f.T > 500.0
+    acc = (lrn["status"] & 8) != 0
+    assert npred == react.sum() and acc.sum() == nacc
+    fb = react & ~acc
+    for k in ("RR", "Tend", "deltaTChem", "nAttempts", "status"):
+        a, b = lrn[k], plain[k]
+        np.testing.assert_array_equal(a[..., fb], b[..., fb])
+    rho0 = f.rho(mech4)
+    Ya, Yb = f.Y + lrn["RR"] * dt / rho0, f.Y + plain["RR"] * dt / rho0
+    d = np.abs(Ya - Yb)
+    ok = ((d <= 1e-6) | (d <= 1e-4 * np.abs(Yb))).all(axis=0) & (np.abs(lrn["Tend"] - plain["Tend"]) <= 1e-3)
+    assert ok[react].all()
+    prod = f.cls == 1
+    assert acc[prod].sum() >= 0.5 * prod.sum(), (acc[prod].sum(), prod.sum(), nver, nacc)
+    assert lrn["nAttempts"][prod].sum() + (nver - nacc) <= 0.75 * plain["nAttempts"][prod].sum()
