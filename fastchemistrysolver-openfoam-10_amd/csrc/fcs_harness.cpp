@@ -72,10 +72,10 @@ struct Reader {
     }
 };
 
-// Mechanism arrays in fcs_mech_desc order: int32 ns, nr, pmax; doubles W,
-// T_low, T_high, T_common [ns], coeffs_high, coeffs_low [7 ns]; int32
-// rxn_type, rxn_reversible, rxn_n_lhs, rxn_n_rhs [nr], rxn_lhs, rxn_rhs
-// [3 nr]; doubles A, beta, Ta, A0, beta0, Ta0 [nr], efficiencies [nr ns],
+// Mechanism arrays in fcs_mech_desc order: int32 ns, nr, pmax, slot stride
+// ss; doubles W, T_low, T_high, T_common [ns], coeffs_high, coeffs_low
+// [7 ns]; int32 rxn_type, rxn_reversible, rxn_n_lhs, rxn_n_rhs [nr],
+// rxn_lhs, rxn_rhs [ss nr]; doubles A, beta, Ta, A0, beta0, Ta0 [nr], efficiencies [nr ns],
 // troe [4 nr]; int32 plog_n [nr]; doubles plog_p, plog_A, plog_beta,
 // plog_Ta [nr pmax], sri [5 nr], A_rev, beta_rev, Ta_rev [nr].
 struct MechData {
@@ -87,15 +87,15 @@ struct MechData {
 void read_mech(const std::string& path, MechData& m)
 {
     Reader r(path);
-    const auto h = r.get<int32_t>(3);
-    const size_t ns = h[0], nr = h[1], pm = h[2];
+    const auto h = r.get<int32_t>(4);
+    const size_t ns = h[0], nr = h[1], pm = h[2], ss = h[3];
     int k = 0;
     for (int q = 0; q < 4; ++q) m.d[k++] = r.get<double>(ns);
     m.d[k++] = r.get<double>(7 * ns);
     m.d[k++] = r.get<double>(7 * ns);
     for (int q = 0; q < 4; ++q) m.i[q] = r.get<int32_t>(nr);
-    m.i[4] = r.get<int32_t>(3 * nr);
-    m.i[5] = r.get<int32_t>(3 * nr);
+    m.i[4] = r.get<int32_t>(ss * nr);
+    m.i[5] = r.get<int32_t>(ss * nr);
     for (int q = 0; q < 6; ++q) m.d[k++] = r.get<double>(nr);
     m.d[k++] = r.get<double>(nr * ns);
     m.d[k++] = r.get<double>(4 * nr);
@@ -106,6 +106,7 @@ void read_mech(const std::string& path, MechData& m)
     fcs_mech_desc& s = m.desc;
     s.n_species = (int32_t)ns;
     s.n_reactions = (int32_t)nr;
+    s.slot_stride = (int32_t)ss;
     s.W = m.d[0].data(); s.T_low = m.d[1].data(); s.T_high = m.d[2].data(); s.T_common = m.d[3].data();
     s.coeffs_high = m.d[4].data(); s.coeffs_low = m.d[5].data();
     s.rxn_type = m.i[0].data(); s.rxn_reversible = m.i[1].data(); s.rxn_n_lhs = m.i[2].data();

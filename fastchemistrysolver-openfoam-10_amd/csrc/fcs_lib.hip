@@ -738,13 +738,18 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         for (int r = 0; r < nr; ++r)
             if (d->rxn_type[r] == t) perm.push_back(r);
     if ((int)perm.size() != nr) return FCS_E_ARG;
+    // slot layout: up to 3 reactant slots; product sides longer than 3 slots
+    // (up to slot_stride) only on reactions without a Kc reverse rate
+    const int ss = d->slot_stride > 0 ? d->slot_stride : FCS_MAX_STOICH;
+    if (ss < FCS_MAX_STOICH || ss > FCS_MAX_SLOTS) return FCS_E_ARG;
     for (int r = 0; r < nr; ++r) {
-        if (d->rxn_n_lhs[r] < 1 || d->rxn_n_lhs[r] > 3 || d->rxn_n_rhs[r] < 1 || d->rxn_n_rhs[r] > 3)
-            return FCS_E_ARG;
-        for (int q = 0; q < d->rxn_n_lhs[r]; ++q)
-            if (d->rxn_lhs[3 * r + q] < 0 || d->rxn_lhs[3 * r + q] >= ns) return FCS_E_ARG;
-        for (int q = 0; q < d->rxn_n_rhs[r]; ++q)
-            if (d->rxn_rhs[3 * r + q] < 0 || d->rxn_rhs[3 * r + q] >= ns) return FCS_E_ARG;
+        const int nl = d->rxn_n_lhs[r], nrh = d->rxn_n_rhs[r];
+        if (nl < 1 || nl > FCS_MAX_STOICH || nrh < 1 || nrh > ss) return FCS_E_ARG;
+        if (nrh > FCS_MAX_STOICH && d->rxn_reversible[r] == 1) return FCS_E_UNSUPPORTED;
+        for (int q = 0; q < nl; ++q)
+            if (d->rxn_lhs[(size_t)ss * r + q] < 0 || d->rxn_lhs[(size_t)ss * r + q] >= ns) return FCS_E_ARG;
+        for (int q = 0; q < nrh; ++q)
+            if (d->rxn_rhs[(size_t)ss * r + q] < 0 || d->rxn_rhs[(size_t)ss * r + q] >= ns) return FCS_E_ARG;
     }
     const int pmax = d->plog_max_points > 0 ? d->plog_max_points : 1;
     // ---- host images of every table
@@ -780,9 +785,15 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         rrev[r] = d->rxn_reversible[f] == 2 ? 2 : (d->rxn_reversible[f] ? 1 : 0);
         rnl[r] = d->rxn_n_lhs[f];
         rnr[r] = d->rxn_n_rhs[f];
+        // the reference evaluates reactions with more than 3 slots on a side
+        // through RFG / JFG (ReactionRate.C:565-569, ReactionFunctionI.H:
+        // 648-720, JacobianFunctionI.H:2965-3060), where a nonEquilibrium row
+        // has no reverse rate; the products beyond slot 3 then enter only
+        // through the net stoichiometry (nu below) and the slot count
+        if (rnr[r] > FCS_MAX_STOICH && rrev[r] == 2) rrev[r] = 0;
         for (int q = 0; q < 3; ++q) {
-            rsp[q * nr + r] = q < rnl[r] ? d->rxn_lhs[3 * f + q] : 0;
-            rsp[(3 + q) * nr + r] = q < rnr[r] ? d->rxn_rhs[3 * f + q] : 0;
+            rsp[q * nr + r] = q < rnl[r] ? d->rxn_lhs[(size_t)ss * f + q] : 0;
+            rsp[(3 + q) * nr + r] = q < rnr[r] ? d->rxn_rhs[(size_t)ss * f + q] : 0;
         }
         rA[r] = d->A[f];
         rbeta[r] = d->beta[f];
@@ -838,8 +849,9 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     std::vector<std::vector<std::pair<int, double>>> lst(ns), lstTB(ns);
     for (int r = 0; r < nr; ++r) {
         std::vector<double> nu(ns, 0.0);
+        const int f = perm[r];
         for (int q = 0; q < rnl[r]; ++q) nu[rsp[q * nr + r]] -= 1.0;
-        for (int q = 0; q < rnr[r]; ++q) nu[rsp[(3 + q) * nr + r]] += 1.0;
+        for (int q = 0; q < rnr[r]; ++q) nu[d->rxn_rhs[(size_t)ss * f + q]] += 1.0;
         for (int s = 0; s < ns; ++s) {
             if (nu[s] == 0.0) continue;
             lst[s].push_back({r, nu[s]});

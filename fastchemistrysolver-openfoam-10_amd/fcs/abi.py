@@ -32,6 +32,7 @@ class MechDesc(C.Structure):
         ("plog_max_points", C.c_int32), ("plog_n", _ip), ("plog_p", _dp),
         ("plog_A", _dp), ("plog_beta", _dp), ("plog_Ta", _dp),
         ("sri", _dp), ("A_rev", _dp), ("beta_rev", _dp), ("Ta_rev", _dp),
+        ("slot_stride", C.c_int32),
     ]
 
 
@@ -57,6 +58,8 @@ def make_desc(arrays: dict):
     keep = {}
     d = MechDesc()
     for name, typ in MechDesc._fields_:
+        if name == "slot_stride" and name not in arrays:
+            continue                     # 0: three slots per reaction
         v = arrays[name]
         if typ in (C.c_int32,):
             setattr(d, name, int(v))

@@ -4,13 +4,14 @@ Reads the `reactions` and `speciesThermo` dictionaries the plug-in takes
 (the reaction forms OptReaction::readInfo accepts, src/Reaction/OptReaction.C:
 348-1175: (ir)reversibleArrhenius, (ir)reversibleThirdBodyArrhenius,
 (ir)reversibleArrheniusLindemannFallOff, (ir)reversibleArrheniusTroeFallOff,
-(ir)reversibleArrheniusSRIFallOff, (ir)reversiblePlog,
-nonEquilibriumReversible(ThirdBody)Arrhenius) into a fcs.mechanism.Mechanism, and volScalarField files
+(ir)reversibleArrheniusSRIFallOff, (ir)reversibleArrheniusPLOG (also read as
+(ir)reversiblePlog), nonEquilibriumReversible(ThirdBody)Arrhenius) into a
+fcs.mechanism.Mechanism, and volScalarField files
 (`internalField uniform v` or `nonuniform List<scalar> n (...)`) into arrays.
 
 The dictionary grammar handled is the subset these files use: `key value;`
 entries, `name { ... }` sub-dictionaries, `( ... )` lists optionally preceded
-by their length, double-quoted strings, and words that may contain balanced
+by their length, uniform lists `N{value}`, double-quoted strings, and words that may contain balanced
 parentheses (species names such as CH2(S)).  Comments are // and /* */.
 """
 from __future__ import annotations
@@ -20,7 +21,7 @@ import re
 import numpy as np
 
 from .mechanism import (RXN_ARRHENIUS, RXN_LINDEMANN, RXN_PLOG, RXN_SRI, RXN_THIRD_BODY, RXN_TROE,
-                        MAX_STOICH, Mechanism, Reaction)
+                        MAX_SLOTS, MAX_STOICH, Mechanism, Reaction)
 
 _PUNCT = set("{};")
 
@@ -103,6 +104,12 @@ class _Parser:
         if num is not None and self.peek() == "(" and float(num).is_integer():
             self.take()
             return self.list_body()      # counted list: "53 ( ... )"
+        if num is not None and self.peek() == "{" and float(num).is_integer():
+            self.take()                  # uniform list: "7{0}" is seven zeros
+            v = self.value()
+            if self.take() != "}":
+                raise ValueError("malformed uniform list")
+            return [v] * int(num)
         return num if num is not None else tok
 
     def list_body(self):
@@ -162,6 +169,7 @@ _FORMS = {
     "ArrheniusTroeFallOff": RXN_TROE,
     "ArrheniusSRIFallOff": RXN_SRI,
     "Plog": RXN_PLOG,
+    "ArrheniusPLOG": RXN_PLOG,   # OptReaction.C:1129-1166 (rows in ArrheniusData)
 }
 _NONEQ = "nonEquilibriumReversible"
 
@@ -181,8 +189,8 @@ def _side(text, index):
         if name not in index:
             raise ValueError(f"unknown species {name!r}")
         slots += [index[name]] * coef
-    if not 1 <= len(slots) <= MAX_STOICH:
-        raise ValueError(f"{len(slots)} species slots on one side of a reaction (1..{MAX_STOICH} supported)")
+    if not 1 <= len(slots) <= MAX_SLOTS:
+        raise ValueError(f"{len(slots)} species slots on one side of a reaction (1..{MAX_SLOTS} supported)")
     return slots
 
 
@@ -201,8 +209,10 @@ def _arrhenius(d):
 
 def mechanism(reactions: dict, thermo: dict) -> Mechanism:
     """Mechanism from parsed `reactions` and `speciesThermo` dictionaries
-    (species order of the `species` list of either file)."""
-    names = list(thermo.get("species") or reactions.get("species"))
+    (species order of the `species` list of the reactions file, else of the
+    thermo file; a thermo database may hold species the mechanism does not
+    use)."""
+    names = list(reactions.get("species") or thermo.get("species"))
     index = {n: i for i, n in enumerate(names)}
     ns = len(names)
     W = np.zeros(ns)
@@ -231,6 +241,12 @@ def mechanism(reactions: dict, thermo: dict) -> Mechanism:
         t = _FORMS[form]
         lhs_s, rhs_s = r["reaction"].split("=", 1)
         lhs, rhs = _side(lhs_s, index), _side(rhs_s, index)
+        # reactant sides of up to 3 slots; longer product sides only where
+        # the reaction has no Kc reverse rate (irreversible, nonEquilibrium)
+        if len(lhs) > MAX_STOICH or (len(rhs) > MAX_STOICH and rev and not noneq):
+            raise ValueError(f"{key}: {typ} {r['reaction']!r}: {len(lhs)} reactant and {len(rhs)} product "
+                             f"slots; reactant sides and the product side of a Kc-reversible reaction "
+                             f"take at most {MAX_STOICH}")
         rx = Reaction(t, rev, lhs, rhs, 0.0, 0.0, 0.0)
         if noneq:
             # forward / reverse sub-dictionaries; third-body efficiencies of

@@ -38,7 +38,8 @@ def write_case(path: str, mech: Mechanism, field: Field, options: dict, rho=None
             f.write(f"{k} {v}\n")
     a = mech.arrays()
     with open(os.path.join(path, "mech.bin"), "wb") as f:
-        f.write(np.array([a["n_species"], a["n_reactions"], a["plog_max_points"]], np.int32).tobytes())
+        f.write(np.array([a["n_species"], a["n_reactions"], a["plog_max_points"], a["slot_stride"]],
+                         np.int32).tobytes())
         for k in _DOUBLE_HEAD:
             f.write(np.ascontiguousarray(a[k], np.float64).tobytes())
         for k in _INT_MID:

@@ -27,7 +27,8 @@ PSTD = 1.0e5
 RXN_ARRHENIUS, RXN_THIRD_BODY, RXN_LINDEMANN, RXN_TROE, RXN_PLOG, RXN_SRI = 0, 1, 2, 3, 4, 5
 FALLOFF = (RXN_LINDEMANN, RXN_TROE, RXN_SRI)
 USES_M = (RXN_THIRD_BODY,) + FALLOFF
-MAX_STOICH = 3
+MAX_STOICH = 3     # reactant slots; product slots of Kc-reversible reactions
+MAX_SLOTS = 16     # product slots of irreversible / nonEquilibrium reactions
 ELEM_MASS = np.array([1.00794, 12.0107, 15.9994, 14.0067])  # H C O N [kg/kmol]
 
 # Heats of formation are drawn in +-HF_RANGE J/mol.  At +-3e5 J/mol a random
@@ -88,8 +89,9 @@ class Mechanism:
         """Flat, contiguous arrays matching ``fcs_mech_desc`` field by field."""
         ns, nr = self.n_species, self.n_reactions
         pmax = max([len(r.plog[0]) for r in self.reactions if r.rtype == RXN_PLOG] + [1])
+        ss = max([MAX_STOICH] + [len(r.rhs) for r in self.reactions])
         a = dict(
-            n_species=ns, n_reactions=nr, plog_max_points=pmax,
+            n_species=ns, n_reactions=nr, plog_max_points=pmax, slot_stride=ss,
             W=np.ascontiguousarray(self.W, np.float64),
             T_low=np.ascontiguousarray(self.T_low, np.float64),
             T_high=np.ascontiguousarray(self.T_high, np.float64),
@@ -98,7 +100,7 @@ class Mechanism:
             coeffs_low=np.ascontiguousarray(self.coeffs_low, np.float64).ravel(),
             rxn_type=np.zeros(nr, np.int32), rxn_reversible=np.zeros(nr, np.int32),
             rxn_n_lhs=np.zeros(nr, np.int32), rxn_n_rhs=np.zeros(nr, np.int32),
-            rxn_lhs=np.zeros(nr * MAX_STOICH, np.int32), rxn_rhs=np.zeros(nr * MAX_STOICH, np.int32),
+            rxn_lhs=np.zeros(nr * ss, np.int32), rxn_rhs=np.zeros(nr * ss, np.int32),
             A=np.zeros(nr), beta=np.zeros(nr), Ta=np.zeros(nr),
             A0=np.zeros(nr), beta0=np.zeros(nr), Ta0=np.zeros(nr),
             efficiencies=np.ones(nr * ns), troe=np.zeros(nr * 4),
@@ -115,8 +117,8 @@ class Mechanism:
                 a["sri"][r * 5:r * 5 + 5] = rx.sri
             a["rxn_n_lhs"][r] = len(rx.lhs)
             a["rxn_n_rhs"][r] = len(rx.rhs)
-            a["rxn_lhs"][r * 3:r * 3 + len(rx.lhs)] = rx.lhs
-            a["rxn_rhs"][r * 3:r * 3 + len(rx.rhs)] = rx.rhs
+            a["rxn_lhs"][r * ss:r * ss + len(rx.lhs)] = rx.lhs
+            a["rxn_rhs"][r * ss:r * ss + len(rx.rhs)] = rx.rhs
             a["A"][r], a["beta"][r], a["Ta"][r] = rx.A, rx.beta, rx.Ta
             a["A0"][r], a["beta0"][r], a["Ta0"][r] = rx.A0, rx.beta0, rx.Ta0
             if rx.eff is not None:
@@ -137,12 +139,13 @@ class Mechanism:
         """Inverse of arrays(): a Mechanism from the flat fcs_mech_desc arrays."""
         ns, nr = int(a["n_species"]), int(a["n_reactions"])
         pmax = int(a["plog_max_points"])
+        ss = int(a["slot_stride"]) if "slot_stride" in a and int(a["slot_stride"]) > 0 else MAX_STOICH
         rx = []
         for r in range(nr):
             t = int(a["rxn_type"][r])
             x = Reaction(t, bool(a["rxn_reversible"][r]),
-                         [int(v) for v in a["rxn_lhs"][r * 3:r * 3 + int(a["rxn_n_lhs"][r])]],
-                         [int(v) for v in a["rxn_rhs"][r * 3:r * 3 + int(a["rxn_n_rhs"][r])]],
+                         [int(v) for v in a["rxn_lhs"][r * ss:r * ss + int(a["rxn_n_lhs"][r])]],
+                         [int(v) for v in a["rxn_rhs"][r * ss:r * ss + int(a["rxn_n_rhs"][r])]],
                          float(a["A"][r]), float(a["beta"][r]), float(a["Ta"][r]),
                          float(a["A0"][r]), float(a["beta0"][r]), float(a["Ta0"][r]))
             if t in USES_M:

@@ -64,7 +64,9 @@ extern "C" {
 #define FCS_JACOBIAN_EXACT  0   /* jacobian exact; (default)                     */
 #define FCS_JACOBIAN_FAST   1   /* jacobian fast;  diagonal dC/dY only           */
 
-#define FCS_MAX_STOICH      3   /* reactant / product slots per reaction         */
+#define FCS_MAX_STOICH      3   /* reactant slots per reaction; product slots of a
+                                   reversible (rxn_reversible 1) reaction           */
+#define FCS_MAX_SLOTS      16   /* product slots of other reactions (slot_stride) */
 
 /*
  * Flat mechanism description (species thermo + reactions, file order).
@@ -83,6 +85,13 @@ extern "C" {
  * (nonEquilibriumReversible[ThirdBody]Arrhenius, OptReaction.C:617-700).
  * sri is [n_reactions x 5] = (a, b, c, d, e) read for SRI rows; sri, A_rev,
  * beta_rev and Ta_rev may be NULL when no row needs them.
+ * rxn_lhs / rxn_rhs hold slot_stride slots per reaction (slot_stride 0 means
+ * FCS_MAX_STOICH; at most FCS_MAX_SLOTS).  Reactant sides have 1..3 slots.
+ * Product sides of irreversible and nonEquilibrium reactions may have up to
+ * slot_stride slots; such a reaction with more than 3 slots on a side takes
+ * the reference's general path (RFG / JFG, ReactionFunctionI.H:648-720,
+ * JacobianFunctionI.H:2965-3060), which has no reverse rate for
+ * nonEquilibrium rows.  Kc-reversible reactions keep 1..3 slots a side.
  */
 typedef struct fcs_mech_desc {
     int32_t n_species;
@@ -96,9 +105,9 @@ typedef struct fcs_mech_desc {
     const int32_t* rxn_type;    /* [nr] FCS_RXN_*                         */
     const int32_t* rxn_reversible; /* [nr] 1: reverse rate from Kc         */
     const int32_t* rxn_n_lhs;   /* [nr] 1..3                              */
-    const int32_t* rxn_n_rhs;   /* [nr] 1..3                              */
-    const int32_t* rxn_lhs;     /* [nr*3] species index per slot          */
-    const int32_t* rxn_rhs;     /* [nr*3]                                 */
+    const int32_t* rxn_n_rhs;   /* [nr] 1..3 (1..slot_stride, see above)  */
+    const int32_t* rxn_lhs;     /* [nr*slot_stride] species index per slot */
+    const int32_t* rxn_rhs;     /* [nr*slot_stride]                       */
     const double* A;            /* [nr] forward Arrhenius / k_inf         */
     const double* beta;
     const double* Ta;
@@ -117,6 +126,7 @@ typedef struct fcs_mech_desc {
     const double* A_rev;        /* [nr] reverse Arrhenius (rxn_reversible == 2) */
     const double* beta_rev;
     const double* Ta_rev;
+    int32_t slot_stride;        /* slots per reaction in rxn_lhs / rxn_rhs (0: 3) */
 } fcs_mech_desc;
 
 /* Solver options: the chemistryProperties keys of the reference. */

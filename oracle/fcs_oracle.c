@@ -57,6 +57,7 @@ struct orc_mech {
     double *W, *invW, *Tc, *clo, *chi;
     double TlowMin, ThighMax, TcMin, TcMax;
     int32_t *type, *rev, *nl, *nrh, *lhs, *rhs;
+    int ss;                      /* slots per reaction in lhs / rhs */
     double *A, *beta, *Ta, *A0, *beta0, *Ta0, *eff, *troe, *sri, *Arev, *brev, *Tarev;
     int pmax;
     int32_t* plog_n;
@@ -100,8 +101,15 @@ orc_mech* orc_mech_create(const fcs_mech_desc* d)
     m->rev = dup_i(d->rxn_reversible, nr);
     m->nl = dup_i(d->rxn_n_lhs, nr);
     m->nrh = dup_i(d->rxn_n_rhs, nr);
-    m->lhs = dup_i(d->rxn_lhs, 3 * (size_t)nr);
-    m->rhs = dup_i(d->rxn_rhs, 3 * (size_t)nr);
+    m->ss = d->slot_stride > 0 ? d->slot_stride : 3;
+    m->lhs = dup_i(d->rxn_lhs, (size_t)m->ss * nr);
+    m->rhs = dup_i(d->rxn_rhs, (size_t)m->ss * nr);
+    /* a reaction with more than 3 slots on a side takes the general path
+       RFG / JFG (ReactionRate.C:565-569, ReactionFunctionI.H:648-720,
+       JacobianFunctionI.H:2965-3060), which gives nonEquilibrium rows
+       (isIrreversible == 2) no reverse rate */
+    for (int r = 0; r < nr; ++r)
+        if (m->rev[r] == 2 && (m->nl[r] > 3 || m->nrh[r] > 3)) m->rev[r] = 0;
     m->A = dup_d(d->A, nr); m->beta = dup_d(d->beta, nr); m->Ta = dup_d(d->Ta, nr);
     m->A0 = dup_d(d->A0, nr); m->beta0 = dup_d(d->beta0, nr); m->Ta0 = dup_d(d->Ta0, nr);
     m->eff = dup_d(d->efficiencies, (size_t)nr * ns);
@@ -320,8 +328,8 @@ static double kc_of(const orc_mech* m, int r, const double* expG, const double* 
                     double T, double* dlnKc)
 {
     double num = 1.0, den = 1.0, sdb = 0.0;
-    for (int s = 0; s < m->nrh[r]; ++s) { int k = m->rhs[3 * r + s]; num *= expG[k]; if (dBdT) sdb += dBdT[k]; }
-    for (int s = 0; s < m->nl[r]; ++s) { int k = m->lhs[3 * r + s]; den *= expG[k]; if (dBdT) sdb -= dBdT[k]; }
+    for (int s = 0; s < m->nrh[r]; ++s) { int k = m->rhs[m->ss * r + s]; num *= expG[k]; if (dBdT) sdb += dBdT[k]; }
+    for (int s = 0; s < m->nl[r]; ++s) { int k = m->lhs[m->ss * r + s]; den *= expG[k]; if (dBdT) sdb -= dBdT[k]; }
     const int dn = m->nrh[r] - m->nl[r];
     const double PbyRT = PSTD / (RU * T);
     double pw = 1.0;
@@ -380,11 +388,11 @@ void orc_rates(const orc_mech* m, double p, double Tin, const double* C, double*
         const double kf = rate_k(m, r, T, lnT, p, C, 0, &dk, &dm, &M);
         const double kr = reverse_k(m, r, kf, dk, dm, M, expG, NULL, T, lnT, &dkr, &dmr);
         double CF = 1.0, CR = 1.0;
-        for (int s = 0; s < m->nl[r]; ++s) CF *= C[m->lhs[3 * r + s]];
-        for (int s = 0; s < m->nrh[r]; ++s) CR *= C[m->rhs[3 * r + s]];
+        for (int s = 0; s < m->nl[r]; ++s) CF *= C[m->lhs[m->ss * r + s]];
+        for (int s = 0; s < m->nrh[r]; ++s) CR *= C[m->rhs[m->ss * r + s]];
         const double q = kf * CF - kr * CR;
-        for (int s = 0; s < m->nl[r]; ++s) omega[m->lhs[3 * r + s]] -= q;
-        for (int s = 0; s < m->nrh[r]; ++s) omega[m->rhs[3 * r + s]] += q;
+        for (int s = 0; s < m->nl[r]; ++s) omega[m->lhs[m->ss * r + s]] -= q;
+        for (int s = 0; s < m->nrh[r]; ++s) omega[m->rhs[m->ss * r + s]] += q;
     }
 }
 
@@ -431,8 +439,8 @@ double orc_tc(const orc_mech* m, double rho, double Tin, double p, const double*
             kr = kf / Kc;
         }
         double CF = 1.0, CR = 1.0;
-        for (int a = 0; a < m->nl[r]; ++a) CF *= C[m->lhs[3 * r + a]];
-        for (int a = 0; a < m->nrh[r]; ++a) CR *= C[m->rhs[3 * r + a]];
+        for (int a = 0; a < m->nl[r]; ++a) CF *= C[m->lhs[m->ss * r + a]];
+        for (int a = 0; a < m->nrh[r]; ++a) CR *= C[m->rhs[m->ss * r + a]];
         const double of = kf * CF, orv = kr * CR;
         double wf = 0.0, wr = 0.0;
         for (int a = 0; a < m->nrh[r]; ++a) wf += of;
@@ -531,8 +539,8 @@ void orc_jacobian(const orc_mech* m, double p, double* YT, double* dYTdt,
         const double kf = rate_k(m, r, T, lnT, p, C, 1, &dkfdT, &dkfdM, &M);
         const double kr = reverse_k(m, r, kf, dkfdT, dkfdM, M, expG, dBdT, T, lnT, &dkrdT, &dkrdM);
         const int nl = m->nl[r], nrr = m->nrh[r];
-        const int32_t* L = m->lhs + 3 * r;
-        const int32_t* R = m->rhs + 3 * r;
+        const int32_t* L = m->lhs + (size_t)m->ss * r;
+        const int32_t* R = m->rhs + (size_t)m->ss * r;
         double CF = 1.0, CR = 1.0;
         for (int a = 0; a < nl; ++a) CF *= C[L[a]];
         for (int a = 0; a < nrr; ++a) CR *= C[R[a]];

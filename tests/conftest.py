@@ -37,3 +37,31 @@ def gri30():
     mech = Mechanism.from_arrays(a, names=[str(s) for s in g["names"]])
     field = Field(g["T"].copy(), g["p"].copy(), g["Y"].copy(), np.full(g["T"].size, 1e-7))
     return mech, field
+
+
+TUTORIAL_CASES = ("NH3_Zhang_counterflow", "NC12H26_Yao_counterflow", "H2_Connaire2004",
+                  "Kerosene_Zettervall2020", "NC12H26_Gao2014", "DME_LLNL", "NH3CH4_Konnov")
+
+
+@pytest.fixture(scope="session")
+def tutorial_case():
+    """Loader of the reference-tutorial mechanisms and cell states of
+    tests/golden/tutorial_mechs.npz (written by tests/golden/
+    make_tutorial_mechs.py): tutorial_case(name) -> (Mechanism, Field)."""
+    import numpy as np
+    from fcs.mechanism import Field, Mechanism
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "tutorial_mechs.npz"))
+    cache = {}
+
+    def load(name):
+        if name not in cache:
+            pre = name + "__mech_"
+            a = {k[len(pre):]: g[k] for k in g.files if k.startswith(pre)}
+            mech = Mechanism.from_arrays(a, names=[str(s) for s in g[name + "__names"]])
+            T = g[name + "__T"]
+            cache[name] = (mech, Field(T.copy(), g[name + "__p"].copy(), g[name + "__Y"].copy(),
+                                       np.full(T.size, 1e-7)))
+        m, f = cache[name]
+        return m, Field(f.T.copy(), f.p.copy(), f.Y.copy(), f.deltaTChem.copy())
+
+    return load

