@@ -18,7 +18,9 @@ using namespace fcs;
 // Padded system sizes compiled into the library (T sits at NP-1).
 #define FCS_FOR_EACH_NP(X) X(12) X(56)
 // workgroup form (matrix in HBM): padded sizes, multiples of 16
-#define FCS_FOR_EACH_NP_LARGE(X) X(176)
+#define FCS_FOR_EACH_NP_LARGE(X) X(176) X(256)
+// LDS of one compute unit: the most one cell's workgroup (or wavefront) can take
+constexpr size_t kMaxLdsBytes = 160 * 1024;
 
 // ============================================================== kernels
 // Device arrays of one batch of cells: the flow-step inputs and outputs of
@@ -1052,6 +1054,10 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
     s->opts = *opts;
     s->ks = ks;
     s->ldsBytes = lds_doubles(ks->NP, mech->nr, mech->dm.nTB) * sizeof(double);
+    if (s->ldsBytes > kMaxLdsBytes) {   // reaction arrays beyond one CU's LDS
+        delete s;
+        return FCS_E_UNSUPPORTED;
+    }
     if (hipStreamCreate(&s->stream) != hipSuccess) {
         delete s;
         return FCS_E_HIP;

@@ -1,8 +1,8 @@
 """GPU parity on the reference tutorials' own mechanisms (tests/golden/
 tutorial_mechs.npz): ArrheniusPLOG rows, nonEquilibrium reversible and
 third-body rows, irreversible reactions with 4 to 7 product slots, SRI
-falloff, and 10 to 129 species (wavefront and workgroup forms), through the
-C ABI against the oracle.
+falloff, and 10 to 255 species (the wavefront form and both workgroup
+sizes), through the C ABI against the oracle.
 
 Bar (as for GRI-3.0): kernel outputs at FP64 round-off scaled by the row
 magnitude; after solve(deltaT) per-species |dY| <= 1e-6 absolute or 1e-4
@@ -89,8 +89,11 @@ def test_kernels(fcslib, tutorial_case, name):
 @pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("name", TUTORIAL_CASES)
 def test_solve(fcslib, tutorial_case, name, solver):
-    """solve(1e-5 s) over every state of the case."""
+    """solve(1e-5 s) over every state of the case (every 4th above 150
+    species, where the oracle takes seconds per cell)."""
     mech, field = tutorial_case(name)
+    if mech.n_species > 150:
+        field = field.subset(np.arange(0, field.n_cells, 4))
     print(f"[tutorial solve] {name} {solver}: {field.n_cells} cells", flush=True)
     dt = 1e-5
     g = fcslib.Solver(fcslib.Mech(mech), solver=solver).solve(field, dt)

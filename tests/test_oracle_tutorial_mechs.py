@@ -31,6 +31,8 @@ EXPECTED = {
     "NC12H26_Gao2014": (31, 193, {0: 170, 1: 5, 2: 1, 3: 17}, 0, 4),
     "DME_LLNL": (79, 351, {0: 314, 1: 18, 2: 5, 3: 14}, 332, 3),
     "NH3CH4_Konnov": (129, 1231, {0: 1140, 1: 54, 2: 13, 3: 22, 5: 2}, 0, 3),
+    "NC7H16_Yoo2011": (188, 939, {0: 905, 1: 19, 2: 5, 3: 10}, 924, 3),
+    "NC12H26_Narayanaswamy2014": (255, 1509, {0: 1465, 1: 10, 3: 34}, 3, 4),
 }
 
 
