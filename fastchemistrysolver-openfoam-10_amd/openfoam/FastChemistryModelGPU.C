@@ -26,6 +26,7 @@ struct Foam::FastChemistryModelGPU<ThermoType>::MechArrays
         Arev, betarev, Tarev, plogP, plogA, plogBeta, plogTa;
     std::vector<int32_t> type, rev, nl, nr, lhs, rhs, plogN;
     int32_t pmax = 1;
+    int32_t stride = FCS_MAX_STOICH;   // slots per reaction in lhs / rhs
     fcs_mech_desc desc{};
 };
 
@@ -117,6 +118,7 @@ void Foam::FastChemistryModelGPU<ThermoType>::readMechanism
     const dictionary& reactions = this->subDict("reactions");
     const label ns = species.size();
     std::vector<std::vector<double>> plog;
+    std::vector<std::vector<int32_t>> lhsSides, rhsSides;
     forAllConstIter(dictionary, reactions, iter)
     {
         const dictionary& r = reactions.subDict(iter().keyword());
@@ -131,7 +133,7 @@ void Foam::FastChemistryModelGPU<ThermoType>::readMechanism
         else if (form == "ArrheniusLindemannFallOff") t = FCS_RXN_LINDEMANN;
         else if (form == "ArrheniusTroeFallOff") t = FCS_RXN_TROE;
         else if (form == "ArrheniusSRIFallOff") t = FCS_RXN_SRI;
-        else if (form == "Plog") t = FCS_RXN_PLOG;
+        else if (form == "ArrheniusPLOG" || form == "Plog") t = FCS_RXN_PLOG;
         else
         {
             FatalErrorInFunction << "Unsupported reaction type " << type << exit(FatalError);
@@ -143,8 +145,8 @@ void Foam::FastChemistryModelGPU<ThermoType>::readMechanism
         m.rev.push_back(noneq ? 2 : irrev ? 0 : 1);
         m.nl.push_back(L.size());
         m.nr.push_back(Rs.size());
-        for (int q = 0; q < 3; ++q) m.lhs.push_back(q < int(L.size()) ? L[q] : 0);
-        for (int q = 0; q < 3; ++q) m.rhs.push_back(q < int(Rs.size()) ? Rs[q] : 0);
+        lhsSides.push_back(L);
+        rhsSides.push_back(Rs);
         auto arr = [](const dictionary& a, std::vector<double>& A, std::vector<double>& b, std::vector<double>& T)
         {
             A.push_back(a.lookup<scalar>("A"));
@@ -190,6 +192,17 @@ void Foam::FastChemistryModelGPU<ThermoType>::readMechanism
         plog.push_back(row);
     }
     const size_t nr = m.type.size();
+    // slots per reaction: 3, or the longest product side (irreversible and
+    // nonEquilibrium reactions may carry up to FCS_MAX_SLOTS products)
+    m.stride = FCS_MAX_STOICH;
+    for (const auto& Rs : rhsSides) m.stride = std::max<int32_t>(m.stride, Rs.size());
+    m.lhs.assign(nr*m.stride, 0);
+    m.rhs.assign(nr*m.stride, 0);
+    for (size_t k = 0; k < nr; ++k)
+    {
+        for (size_t q = 0; q < lhsSides[k].size() && q < size_t(m.stride); ++q) m.lhs[k*m.stride + q] = lhsSides[k][q];
+        for (size_t q = 0; q < rhsSides[k].size(); ++q) m.rhs[k*m.stride + q] = rhsSides[k][q];
+    }
     m.plogP.assign(nr*m.pmax, 0); m.plogA.assign(nr*m.pmax, 0);
     m.plogBeta.assign(nr*m.pmax, 0); m.plogTa.assign(nr*m.pmax, 0);
     m.plogN.assign(nr, 0);
@@ -211,6 +224,7 @@ void Foam::FastChemistryModelGPU<ThermoType>::readMechanism
     d.coeffs_high = m.hi.data(); d.coeffs_low = m.lo.data();
     d.rxn_type = m.type.data(); d.rxn_reversible = m.rev.data(); d.rxn_n_lhs = m.nl.data();
     d.rxn_n_rhs = m.nr.data(); d.rxn_lhs = m.lhs.data(); d.rxn_rhs = m.rhs.data();
+    d.slot_stride = m.stride;
     d.A = m.A.data(); d.beta = m.beta.data(); d.Ta = m.Ta.data();
     d.A0 = m.A0.data(); d.beta0 = m.beta0.data(); d.Ta0 = m.Ta0.data();
     d.efficiencies = m.eff.data(); d.troe = m.troe.data();
