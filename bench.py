@@ -119,6 +119,57 @@ def attempt(mech, gm):
                       "mean_error_norm": float(err.mean())}), flush=True)
 
 
+def traffic(mech, gm):
+    """The default workload's plain step with eager launches (no hipGraph):
+    the 262,144-cell counterflow field, one plain Rodas34 step of
+    k_step<56,Rodas34> launches after a k_setup, for HBM-byte counter
+    collection (FETCH_SIZE, WRITE_SIZE per dispatch)."""
+    from fcs import lib
+    from fcs.mechanism import config_field
+    solver = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
+    field = config_field(2, mech, equilibrate=lib.equilibrator(solver))
+    b = lib.Batch(solver, field)
+    b.set_launch(0, 0)
+    _, launches, step_ms = timed_runs(b, 1, 0, profile=True)
+    r = b.download()
+    print(json.dumps({"workload": "plain OptRodas34 step, eager launches, 262,144-cell counterflow field",
+                      "launches": launches, "attempts": int(r["nAttempts"].sum()),
+                      "avg_launch_ms": step_ms / max(launches, 1)}), flush=True)
+
+
+def copy_calibration():
+    """Three device-to-device copies of 256 MiB (the runtime's copy kernel)
+    and three elementwise negations of 256 MiB: known-size references for
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 256 << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)          # an elementwise kernel: reads and writes 256 MiB
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "3 x 256 MiB device copy, 3 x 256 MiB elementwise negation",
+                      "bytes_per_launch": n}), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per k_step<56,Rodas34> launch of the default workload, as
+    measured by two counter runs and committed with the profiles (None when
+    the summary is absent)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "traffic_k_step_rodas34_53sp_counterflow_262144cells.md")
+    try:
+        with open(path) as f:
+            m = re.search(r"hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def mixture(args, mech, gm):
     """Plain Rodas34 on the 65,536-cell 1-D mixture-fraction field, with the
     step kernel's roofline from an event pair around every step launch."""
@@ -255,13 +306,16 @@ def main():
     ap.add_argument("--no-secondary", action="store_true", help="skip the 65,536-cell plain run")
     ap.add_argument("--train-side", type=int, default=256, help="side of each training field")
     ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt", "large", "large-kernels"),
+    ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt", "large", "large-kernels",
+                                           "traffic", "copy"),
                     default="counterflow",
                     help="counterflow: learned-seeded vs plain Rodas34 on the 262,144-cell field (default); "
                          "mixture: plain Rodas34 on the 65,536-cell mixture-fraction field only; "
                          "attempt: two single-attempt launches on 1,024 cells (for counter collection); "
                          "large: plain Rodas34 on the 160-species mechanism's counterflow field; "
-                         "large-kernels: its components through the kernel-level entries (for profiling)")
+                         "large-kernels: its components through the kernel-level entries (for profiling); "
+                         "traffic: one plain step of the default field with eager launches (for byte counters); "
+                         "copy: 256 MiB device copies (the byte counters' calibration)")
     ap.add_argument("--large-cells", type=int, default=131072, help="cells of the large workload (x 512)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -274,6 +328,8 @@ def main():
         return large(args)
     if args.workload == "large-kernels":
         return large_kernels()
+    if args.workload == "copy":
+        return copy_calibration()
     from fcs import learned, lib
     from fcs.flops import attempt_flops
     from fcs.mechanism import config_field, config_mechanism, counterflow_field
@@ -284,6 +340,8 @@ def main():
         return mixture(args, mech, gm)
     if args.workload == "attempt":
         return attempt(mech, gm)
+    if args.workload == "traffic":
+        return traffic(mech, gm)
     solver = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
     eq = lib.equilibrator(solver)
     t0 = time.perf_counter()
@@ -396,7 +454,9 @@ def main():
         "error_sample": "48 product + 48 flame-front cells, relTol 1e-10 / absTol 1e-14 GPU solve as reference "
                         "(max_rel_Y: |dY| / max(|Y|, 1e-6))",
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
+                     "traffic_unit": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
+                                     "profiles/traffic_k_step_rodas34_53sp_counterflow_262144cells.md)",
                      "kernel": "k_step<56,Rodas34> (FP64 VALU; one Rosenbrock attempt per active cell)",
                      "flops_per_attempt": fpa, "launches": launches,
                      "avg_launch_ms": 1e3 * avg_launch_s},
