@@ -3,34 +3,66 @@ host harness of the C++ chemistry-model adapter (fcs_harness) in-tree."""
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(PKG_ROOT, "libfcs.so")
 HARNESS = os.path.join(PKG_ROOT, "fcs_harness")
-SRCS = [os.path.join(PKG_ROOT, "csrc", f) for f in ("fcs_lib.hip", "fcs_mlp.hip")]
+CSRC = os.path.join(PKG_ROOT, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are kept out of every code object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile csrc/*.hip into libfcs.so (always recompiles)."""
-    cmd = [HIPCC, *HIP_FLAGS, "-I" + os.path.join(REPO, "include"),
-           "-I" + os.path.join(PKG_ROOT, "csrc"), "-o", LIB + ".tmp", *SRCS]
+def padded_sizes() -> list:
+    """The padded system sizes fcs_kernels.h lists (FCS_FOR_EACH_NP and
+    FCS_FOR_EACH_NP_LARGE): one object of fcs_kernels_np.hip each."""
+    with open(os.path.join(CSRC, "fcs_kernels.h")) as f:
+        text = f.read()
+    sizes = []
+    for name in ("FCS_FOR_EACH_NP", "FCS_FOR_EACH_NP_LARGE"):
+        m = re.search(r"#define " + name + r"\(X\)((?: X\(\d+\))+)", text)
+        if not m:
+            raise RuntimeError(f"{name} not found in fcs_kernels.h")
+        sizes += [int(v) for v in re.findall(r"X\((\d+)\)", m.group(1))]
+    return sizes
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
                     and "unknown target feature" not in l.lower())
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{out}")
+        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{' '.join(cmd)}\n{out}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile csrc/*.hip into libfcs.so (always recompiles): the host side,
+    the network kernels and one object per padded size, compiled in
+    parallel, then linked."""
+    inc = ["-I" + os.path.join(REPO, "include"), "-I" + CSRC]
+    with tempfile.TemporaryDirectory(prefix="fcs_build_") as tmp:
+        units = [(os.path.join(CSRC, "fcs_lib.hip"), [], os.path.join(tmp, "fcs_lib.o")),
+                 (os.path.join(CSRC, "fcs_mlp.hip"), [], os.path.join(tmp, "fcs_mlp.o"))]
+        units += [(os.path.join(CSRC, "fcs_kernels_np.hip"), [f"-DFCS_NP={n}"], os.path.join(tmp, f"fcs_np{n}.o"))
+                  for n in padded_sizes()]
+        cmds = [[HIPCC, *HIP_FLAGS, *inc, *defs, "-c", src, "-o", obj] for src, defs, obj in units]
+        jobs = jobs or max(1, min(len(cmds), os.cpu_count() or 1, 8))
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            for f in [ex.submit(_run, c, verbose) for c in cmds]:
+                f.result()
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", LIB + ".tmp", *[obj for _, _, obj in units]], verbose)
     os.replace(LIB + ".tmp", LIB)
     build_harness(verbose)
     return LIB

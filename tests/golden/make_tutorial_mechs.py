@@ -16,8 +16,8 @@ The cases cover the reaction forms and layouts the tutorials use:
 ArrheniusPLOG (NH3_Zhang), nonEquilibrium reversible and third-body rows
 (H2_Connaire2004, DME_LLNL), irreversible reactions with 4 to 7 product
 slots (NC12H26_Yao, Kerosene_Zettervall2020, NC12H26_Gao2014), SRI falloff
-(NH3CH4_Konnov) and species counts of both execution forms and both
-workgroup sizes (10 to 255).
+(NH3CH4_Konnov) and species counts across the padded system sizes of both
+execution forms (10 to 255).
 
 The fixture is data only: mechanism coefficients and field values.
 """
@@ -41,6 +41,7 @@ CASES = {
     "NH3CH4_Konnov": "Ignition0D/NH3CH4_Konnov",
     "NC7H16_Yoo2011": "Ignition0D/NC7H16_Yoo2011",
     "NC12H26_Narayanaswamy2014": "Ignition0D/NC12H26_Narayanaswamy2014",
+    "C2H4_Sandiego": "Ignition0D/C2H4_Sandiego",
 }
 N_SNAPSHOT, N_IGNITION = 256, 64
 

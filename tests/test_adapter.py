@@ -116,3 +116,22 @@ def test_adapter_time_loop_matches_library_and_oracle(tmp_path, lts):
     np.testing.assert_array_equal(out["RR_calculate"], s.calculate(rho, f.T, f.p, f.Y))
     np.testing.assert_array_equal(out["tc"], s.tc(rho, f.T, f.p, f.Y))
     np.testing.assert_array_equal(out["Qdot"], s.qdot(r["RR"]))
+
+
+@pytest.mark.gpu
+def test_adapter_reads_long_product_sides(tmp_path, tutorial_case):
+    """A tutorial mechanism with 5-7-slot product sides (slot stride 7 in
+    mech.bin) through the harness: the rates equal the library's."""
+    from fcs import lib
+    mech, field = tutorial_case("Kerosene_Zettervall2020")
+    assert mech.arrays()["slot_stride"] == 7
+    field = field.subset(np.arange(16))
+    text = CHEMISTRY_PROPERTIES.replace("jacobian        fast;", "jacobian        exact;")
+    d = case(tmp_path, text, mech, field)
+    out = harness.run(d, 1e-5, 1, mech.n_species, field.n_cells)
+    s = lib.Solver(lib.Mech(mech), jacobian="exact", absTol=1e-8, relTol=1e-4, maxSteps=20000, Treact=500.0,
+                   deltaTChemMax=1e-4)
+    f = M.Field(field.T.copy(), field.p.copy(), field.Y.copy(), np.full(field.n_cells, 1e-7))
+    r = s.solve(f, 1e-5, rho=field.rho(mech))
+    np.testing.assert_array_equal(out["RR"], r["RR"])
+    np.testing.assert_array_equal(out["deltaTMin"], [r["deltaTMin"]])

@@ -33,6 +33,7 @@ EXPECTED = {
     "NH3CH4_Konnov": (129, 1231, {0: 1140, 1: 54, 2: 13, 3: 22, 5: 2}, 0, 3),
     "NC7H16_Yoo2011": (188, 939, {0: 905, 1: 19, 2: 5, 3: 10}, 924, 3),
     "NC12H26_Narayanaswamy2014": (255, 1509, {0: 1465, 1: 10, 3: 34}, 3, 4),
+    "C2H4_Sandiego": (59, 271, {0: 233, 1: 15, 3: 23}, 0, 3),
 }
 
 
