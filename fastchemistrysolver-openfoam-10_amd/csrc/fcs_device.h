@@ -41,10 +41,16 @@ constexpr int WAVE = 64;
 // instead of hitting one bank with every lane.
 constexpr int JS = 65;
 
+// Workgroup-form sizes up to this many rows keep the iteration matrix in LDS
+// (NP*NP + NP doubles: 98.9 KiB at NP = 112) instead of an HBM slice.
+constexpr int kJLdsMaxNP = 112;
+__host__ __device__ constexpr bool jlds_of(int NP) { return NP > 64 && NP <= kJLdsMaxNP; }
+
 // Geometry of one cell for a padded system size NP.
 template <int NP>
 struct Cell {
-    static constexpr bool kLarge = NP > 64;                       // workgroup per cell, matrix in HBM
+    static constexpr bool kLarge = NP > 64;                       // workgroup per cell
+    static constexpr bool kJLds = jlds_of(NP);                    // workgroup form with the matrix in LDS
     static constexpr int G = kLarge ? 256 : 64;                   // threads per cell (all 4 SIMDs of a CU)
     static constexpr int NW = G / 64;                             // wavefronts per cell
     static constexpr int SW = G;                                  // species stride of per-species tables
@@ -226,7 +232,8 @@ __host__ __device__ inline int group_of(int NP) { return NP > 64 ? 256 : 64; }
 
 // LDS doubles per cell.  Wavefront form: four wavefronts of the 53-species
 // mechanism fit the 160 KiB of a CU (40,112 B each).  Workgroup form: the
-// matrix is in HBM; the LU scratch reuses the reaction arrays.
+// LU scratch reuses the reaction arrays; the matrix (and its reciprocal
+// pivots) follows them in LDS up to kJLdsMaxNP rows, else it is in HBM.
 __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
 {
     const size_t nrP = round_up(nr, 2);
@@ -238,7 +245,8 @@ __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
     const size_t kq = 2 * nrP > 2 * G ? 2 * nrP : 2 * G;
     const size_t react = nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2);
     const size_t lu = lu_scratch_doubles(NP);
-    return 256 + 2 * G + (react > lu ? react : lu) + G / 2 + 16;
+    const size_t rest = 256 + 2 * G + (react > lu ? react : lu) + G / 2 + 16;
+    return jlds_of(NP) ? ((rest + 1) & ~(size_t)1) + (size_t)NP * NP + NP : rest;
 }
 
 __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jglobal = nullptr)
@@ -263,7 +271,8 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jg
         L.lu = L.rq;
         L.ip = (int*)(L.rq + (react > lu ? react : lu));
         L.red = L.rq + (react > lu ? react : lu) + G / 2;
-        L.J = Jglobal;
+        const size_t rest = 256 + 2 * (size_t)G + (react > lu ? react : lu) + G / 2 + 16;
+        L.J = jlds_of(NP) ? base + ((rest + 1) & ~(size_t)1) : Jglobal;
         return L;
     }
     const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);

@@ -16,7 +16,7 @@ using namespace fcs;
 // Padded system sizes compiled into the library (T sits at NP-1).
 #define FCS_FOR_EACH_NP(X) X(12) X(24) X(40) X(56) X(64)
 // workgroup form (matrix in HBM): padded sizes, multiples of 16
-#define FCS_FOR_EACH_NP_LARGE(X) X(96) X(128) X(176) X(256)
+#define FCS_FOR_EACH_NP_LARGE(X) X(80) X(96) X(112) X(128) X(176) X(256)
 // LDS of one compute unit: the most one cell's workgroup (or wavefront) can take
 constexpr size_t kMaxLdsBytes = 160 * 1024;
 
@@ -45,11 +45,12 @@ struct CellArrays {
     double* Jg;                      // workgroup form: matrix slices [grid][NP*NP + NP]
 };
 
-// Matrix slice of this workgroup (workgroup form), else null.
+// HBM matrix slice of this workgroup (workgroup form with the matrix in
+// HBM), else null.
 template <int NP>
 __device__ __forceinline__ double* jslice(double* Jg)
 {
-    if constexpr (Cell<NP>::kLarge) return Jg + (size_t)blockIdx.x * ((size_t)NP * NP + NP);
+    if constexpr (Cell<NP>::kLarge && !Cell<NP>::kJLds) return Jg + (size_t)blockIdx.x * ((size_t)NP * NP + NP);
     else return nullptr;
 }
 

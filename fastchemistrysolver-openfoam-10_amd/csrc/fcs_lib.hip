@@ -104,10 +104,17 @@ namespace {
 const KernelSet* const kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET_PTR) FCS_FOR_EACH_NP_LARGE(FCS_KSET_PTR)};
 #undef FCS_KSET_PTR
 
-const KernelSet* pick_kernels(int n)
+// Kernels of the smallest padded size holding n rows.  With the reaction
+// table sizes given (nr >= 0), a workgroup-form size that keeps the matrix in
+// LDS is passed over when its LDS would exceed a CU's; the next size, with
+// the same species-table layout, then takes the mechanism.
+const KernelSet* pick_kernels(int n, int nr = -1, int nTB = 0)
 {
-    for (const KernelSet* k : kKernels)
-        if (k->NP >= n) return k;
+    for (const KernelSet* k : kKernels) {
+        if (k->NP < n) continue;
+        if (nr >= 0 && jlds_of(k->NP) && lds_doubles(k->NP, nr, nTB) * sizeof(double) > kMaxLdsBytes) continue;
+        return k;
+    }
     return nullptr;
 }
 
@@ -640,7 +647,7 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
     if (opts->solver < 0 || opts->solver > FCS_SOLVER_ROSENBROCK23 || opts->jacobian < 0 || opts->jacobian > 1 ||
         opts->maxSteps < 1 || opts->maxSteps > 10000000 || !(opts->absTol > 0) || !(opts->relTol > 0))
         return FCS_E_ARG;
-    const KernelSet* ks = pick_kernels(mech->ns + 1);
+    const KernelSet* ks = pick_kernels(mech->ns + 1, mech->nr, mech->dm.nTB);
     if (!ks || !ks->step[opts->solver]) return FCS_E_UNSUPPORTED;   // no Seulex in the workgroup form
     fcs_solver* s = new (std::nothrow) fcs_solver();
     if (!s) return FCS_E_NOMEM;
@@ -727,7 +734,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
     const void* f = s->ks->step[s->opts.solver];
     const int grid = std::max(1, std::min(d.gridOverride > 0 ? d.gridOverride : s->grid, nc));
     const int blk = s->ks->G;
-    if (s->ks->NP > 64) {
+    if (s->ks->NP > 64 && !jlds_of(s->ks->NP)) {
         if (d.ensure_jg((size_t)grid * ((size_t)s->ks->NP * s->ks->NP + s->ks->NP)) != FCS_OK) return FCS_E_NOMEM;
         a.Jg = d.Jg;
     }
@@ -1153,11 +1160,12 @@ int fcs_batch_download(fcs_batch* b, double* RR, double* deltaTChem, double* del
 
 // ------------------------------------------------------------ kernel-level entries
 
-// HBM matrix doubles per workgroup of the workgroup form (0 otherwise).
+// HBM matrix doubles per workgroup of the workgroup form with the matrix in
+// HBM (0 otherwise).
 static size_t jg_per_block(const fcs_solver* s)
 {
     const size_t NP = (size_t)s->ks->NP;
-    return NP > 64 ? NP * NP + NP : 0;
+    return NP > 64 && !jlds_of((int)NP) ? NP * NP + NP : 0;
 }
 
 int fcs_eval_derivatives(fcs_solver* s, int64_t nS, const double* p, double* YT, double* dYTdt)

@@ -1,7 +1,9 @@
 """Parity of the workgroup form (systems of more than 64 rows: one
-workgroup per cell, matrix in HBM, block-4-pivoted LU with FP64 MFMA
-trailing updates) with the CPU oracle, on the 160-species / 1,200-reaction
-synthetic mechanism and an 80-species one (many padding rows)."""
+workgroup per cell, block-4-pivoted LU with FP64 MFMA trailing updates) with
+the CPU oracle: the 160-species / 1,200-reaction synthetic mechanism (matrix
+in HBM, 176 rows), an 80-species one (matrix in LDS, 96 rows, padding rows),
+a 104-species one (matrix in LDS, 112 rows) and a 70-species one whose
+5,000 reactions leave no LDS for the matrix (HBM, 128 rows)."""
 import numpy as np
 import pytest
 
@@ -23,12 +25,17 @@ def mech80():
     return M.synthetic_mechanism(80, 320, n_third_body=30, n_troe=10, n_plog=4, n_sri=3, n_noneq=4, seed=8080)
 
 
+@pytest.fixture(scope="module")
+def mech104():
+    return M.synthetic_mechanism(104, 420, n_third_body=40, n_troe=12, n_plog=4, n_sri=2, n_noneq=6, seed=1104)
+
+
 def yt_states(mech, n, seed):
     fld = M.uniform_field(mech, n, seed=seed)
     return fld, np.vstack([fld.Y, fld.T[None, :]])
 
 
-@pytest.mark.parametrize("which", ["mech4", "mech80"])
+@pytest.mark.parametrize("which", ["mech4", "mech80", "mech104"])
 def test_large_lu_matches_oracle(fcslib, which, request):
     """Block-4-pivoted LU + solve of the workgroup form against the oracle's
     restatement of the reference LU, on random matrices that need pivoting
@@ -50,7 +57,7 @@ def test_large_lu_matches_oracle(fcslib, which, request):
         assert np.abs(A[k] @ x[k] - b[k]).max() <= 1e-10 * np.abs(A[k]).max() * np.abs(x[k]).max()
 
 
-@pytest.mark.parametrize("which", ["mech4", "mech80"])
+@pytest.mark.parametrize("which", ["mech4", "mech80", "mech104"])
 def test_large_derivatives_and_jacobian(fcslib, which, request):
     mech = request.getfixturevalue(which)
     o = Oracle(mech)
@@ -91,15 +98,17 @@ def test_large_single_attempt(fcslib, mech4, solver):
         assert np.abs(YTn[:, c] - ynr).max() <= (1e-9 if er < 1.0 else 1e-3) * inc + 1e-15, c
 
 
-@pytest.mark.parametrize("which,jac", [("mech4", "exact"), ("mech4", "fast"), ("mech80", "exact")])
+@pytest.mark.parametrize("which,jac", [("mech4", "exact"), ("mech4", "fast"), ("mech80", "exact"),
+                                       ("mech104", "exact"), ("mech104", "fast")])
 def test_large_solve_matches_oracle(fcslib, which, jac, request):
     mech = request.getfixturevalue(which)
     fld = M.uniform_field(mech, 8, seed=21)
     dt = 1e-6
     g = fcslib.Solver(fcslib.Mech(mech), jacobian=jac).solve(fld, dt)
-    r = Oracle(mech).solve(fld, dt, threads=1, jacobian=jac)
+    o = Oracle(mech)
+    r = o.solve(fld, dt, threads=1, jacobian=jac)
     assert g["rc"] == 0 and r["rc"] == 0
-    assert_solve_parity(g, r, fld, dt)
+    assert_solve_parity(g, r, fld, dt, rerun=lambda f: o.solve(f, dt, threads=8, jacobian=jac))
 
 
 def test_large_calculate_and_tc(fcslib, mech4):
@@ -123,9 +132,10 @@ def test_large_seulex_matches_oracle(fcslib, which, n, request):
     fld = M.uniform_field(mech, n, seed=21)
     dt = 1e-6
     g = fcslib.Solver(fcslib.Mech(mech), solver="OptSeulex").solve(fld, dt)
-    r = Oracle(mech).solve(fld, dt, threads=1, solver="OptSeulex")
+    o = Oracle(mech)
+    r = o.solve(fld, dt, threads=1, solver="OptSeulex")
     assert g["rc"] == 0 and r["rc"] == 0
-    assert_solve_parity(g, r, fld, dt)
+    assert_solve_parity(g, r, fld, dt, rerun=lambda f: o.solve(f, dt, threads=8, solver="OptSeulex"))
 
 
 def test_large_learned_seed(fcslib, mech4):
@@ -204,3 +214,18 @@ def test_large_batch_graph_autotune_lts_and_cold_cells(fcslib, mech80):
     d = np.abs(Yg - Yo)
     assert ((d <= 1e-6) | (d <= 1e-4 * np.abs(Yo))).all()
     assert np.abs(gl["Tend"] - ol["Tend"]).max() <= 1e-3
+
+
+def test_reaction_tables_beyond_lds_take_the_hbm_matrix(fcslib):
+    """70 species and 5,000 reactions: with the matrix in LDS the 80-, 96-
+    and 112-row sizes would need more than a CU's 160 KiB, so the solver
+    takes the 128-row size with the matrix in HBM (creation would otherwise
+    fail with FCS_E_UNSUPPORTED); its solve matches the oracle."""
+    mech = M.synthetic_mechanism(70, 5000, n_third_body=60, n_troe=20, n_plog=4, seed=7050)
+    fld = M.uniform_field(mech, 8, seed=5)
+    dt = 1e-7
+    g = fcslib.Solver(fcslib.Mech(mech)).solve(fld, dt)
+    o = Oracle(mech)
+    r = o.solve(fld, dt, threads=8)
+    assert g["rc"] == 0 and r["rc"] == 0
+    assert_solve_parity(g, r, fld, dt, rerun=lambda f: o.solve(f, dt, threads=8))
