@@ -156,6 +156,36 @@ def copy_calibration():
                       "bytes_per_launch": n}), flush=True)
 
 
+def tutorials():
+    """Plain Rodas34 (exact Jacobian, absTol 1e-8, relTol 1e-4) on the
+    reference tutorials' mechanisms of tests/golden/tutorial_mechs.npz: each
+    case's states tiled to 16,384 cells, deltaT 1e-5 s, best of two timed
+    solves after a warm-up.  One JSON line per mechanism."""
+    from fcs import lib
+    from fcs.mechanism import Field, Mechanism
+    g = np.load(os.path.join(ROOT, "tests", "golden", "tutorial_mechs.npz"))
+    names = sorted({k.split("__")[0] for k in g.files}, key=lambda n: int(g[n + "__mech_n_species"]))
+    for name in names:
+        pre = name + "__mech_"
+        mech = Mechanism.from_arrays({k[len(pre):]: g[k] for k in g.files if k.startswith(pre)})
+        T, p, Y = g[name + "__T"], g[name + "__p"], g[name + "__Y"]
+        rep = 16384 // T.size
+        field = Field(np.tile(T, rep), np.tile(p, rep), np.tile(Y, (1, rep)), np.full(T.size * rep, 1e-7))
+        b = lib.Batch(lib.Solver(lib.Mech(mech), solver="OptRodas34"), field)
+        b.run(1e-5)
+        best = min(_timed_run(b) for _ in range(2))
+        att = int(b.download()["nAttempts"].sum())
+        print(json.dumps({"mechanism": name, "species": mech.n_species, "reactions": mech.n_reactions,
+                          "cells": field.n_cells, "cells_dt_per_s": field.n_cells / best,
+                          "attempts_per_s": att / best}), flush=True)
+
+
+def _timed_run(batch):
+    t0 = time.perf_counter()
+    batch.run(1e-5)
+    return time.perf_counter() - t0
+
+
 def measured_traffic():
     """HBM bytes per k_step<56,Rodas34> launch of the default workload, as
     measured by two counter runs and committed with the profiles (None when
@@ -307,7 +337,7 @@ def main():
     ap.add_argument("--train-side", type=int, default=256, help="side of each training field")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt", "large", "large-kernels",
-                                           "traffic", "copy"),
+                                           "traffic", "copy", "tutorials"),
                     default="counterflow",
                     help="counterflow: learned-seeded vs plain Rodas34 on the 262,144-cell field (default); "
                          "mixture: plain Rodas34 on the 65,536-cell mixture-fraction field only; "
@@ -315,7 +345,8 @@ def main():
                          "large: plain Rodas34 on the 160-species mechanism's counterflow field; "
                          "large-kernels: its components through the kernel-level entries (for profiling); "
                          "traffic: one plain step of the default field with eager launches (for byte counters); "
-                         "copy: 256 MiB device copies (the byte counters' calibration)")
+                         "copy: 256 MiB device copies (the byte counters' calibration); "
+                         "tutorials: plain Rodas34 on the reference tutorials' mechanisms")
     ap.add_argument("--large-cells", type=int, default=131072, help="cells of the large workload (x 512)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -330,6 +361,8 @@ def main():
         return large_kernels()
     if args.workload == "copy":
         return copy_calibration()
+    if args.workload == "tutorials":
+        return tutorials()
     from fcs import learned, lib
     from fcs.flops import attempt_flops
     from fcs.mechanism import config_field, config_mechanism, counterflow_field

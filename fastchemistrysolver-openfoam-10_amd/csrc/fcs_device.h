@@ -51,7 +51,9 @@ template <int NP>
 struct Cell {
     static constexpr bool kLarge = NP > 64;                       // workgroup per cell
     static constexpr bool kJLds = jlds_of(NP);                    // workgroup form with the matrix in LDS
-    static constexpr int G = kLarge ? 256 : 64;                   // threads per cell (all 4 SIMDs of a CU)
+    // threads per cell: one wavefront, or a workgroup of one thread per row
+    // (two wavefronts up to 128 rows, so two cells share a CU; four beyond)
+    static constexpr int G = !kLarge ? 64 : (NP <= 128 ? 128 : 256);
     static constexpr int NW = G / 64;                             // wavefronts per cell
     static constexpr int SW = G;                                  // species stride of per-species tables
     static constexpr int ONE = kLarge ? 255 : 64;                 // concentration slot holding 1.0
@@ -228,7 +230,7 @@ __host__ __device__ inline size_t lu_scratch_doubles(int NP)
     return 8 * LU_PW + LU_PW * LU_PW + 2 * (size_t)LU_PW * NP + 3 * LU_PW + LU_PW / 2 + 8;
 }
 
-__host__ __device__ inline int group_of(int NP) { return NP > 64 ? 256 : 64; }
+__host__ __device__ inline int group_of(int NP) { return NP > 64 ? (NP <= 128 ? 128 : 256) : 64; }
 
 // LDS doubles per cell.  Wavefront form: four wavefronts of the 53-species
 // mechanism fit the 160 KiB of a CU (40,112 B each).  Workgroup form: the
