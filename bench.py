@@ -424,6 +424,18 @@ def main():
     rp = plain.download()
     log(f"plain: {wall_p / args.steps * 1e3:.1f} ms per step")
 
+    # ---- the same plain step through fcs_solve over host arrays (the
+    # OpenFOAM call's boundary: per-species host pointers), so the rate
+    # includes the uploads and downloads over PCIe; reported beside value
+    solver.solve(field, DT)
+    t0 = time.perf_counter()
+    rh = solver.solve(field, DT)
+    wall_h = time.perf_counter() - t0
+    host = {"api": "fcs_solve over host arrays (per-species pointers), plain OptRodas34, one flow step",
+            "value": field.n_cells / wall_h, "ms_per_step": 1e3 * wall_h,
+            "RR_bit_identical_to_batch": bool(np.array_equal(rh["RR"], rp["RR"]))}
+    log(f"host arrays: {wall_h * 1e3:.1f} ms per step")
+
     # ---- learned-seeded Rodas34 on the same field (the measured workload)
     lrn = lib.Batch(solver, field)
     lrn.set_launch(0, GRAPH_LAUNCHES)
@@ -512,6 +524,7 @@ def main():
                      "flops_per_attempt": fpa, "launches": launches,
                      "avg_launch_ms": 1e3 * avg_launch_s},
     }
+    line["host_buffers"] = host
     if secondary:
         line["plain_mixture_fraction_65536"] = secondary
     if not args.no_cpu:
