@@ -445,6 +445,17 @@ def main():
     npred, nver, nacc = lrn.learned_stats()
     log(f"learned: {wall_l / args.steps * 1e3:.1f} ms per step, {nacc}/{npred} proposals accepted")
 
+    # ---- seed mode: proposals verified as above, and every other reacting
+    # cell starts from the network's first sub-step min(h, deltaT) instead of
+    # its deltaTChem (so its results differ from the plain run's within the
+    # tolerance); reported beside value
+    sdb = lib.Batch(solver, field)
+    sdb.set_launch(0, GRAPH_LAUNCHES)
+    sdb.set_learned(net, "seed")
+    wall_s, _, _ = timed_runs(sdb, args.steps, args.warmup, profile=False)
+    rs = sdb.download()
+    log(f"learned, seed mode: {wall_s / args.steps * 1e3:.1f} ms per step")
+
     # roofline of the step kernel on the learned run: an event pair around
     # every eager step launch on the solver's stream
     lrn.set_launch(0, 0)
@@ -525,6 +536,11 @@ def main():
                      "avg_launch_ms": 1e3 * avg_launch_s},
     }
     line["host_buffers"] = host
+    line["learned_seed_mode"] = {"value": field.n_cells / (wall_s / args.steps),
+                                 "speedup_vs_plain": (field.n_cells / (wall_s / args.steps)) / plain_value,
+                                 "attempts_per_step": int(rs["nAttempts"].sum()),
+                                 "failed_cells": int(((rs["status"] & 6) != 0).sum()),
+                                 "error_vs_tight": err(rs)}
     if secondary:
         line["plain_mixture_fraction_65536"] = secondary
     if not args.no_cpu:
