@@ -14,9 +14,10 @@ library's own BF16-MFMA kernels on samples the plain GPU integrator
 produces from two other seeded counterflow fields.  The same field is also
 run with the plain integrator (no network), and the 65,536-cell
 mixture-fraction field of the plain-integrator configuration is run too;
-their rates are reported beside `value`.  The error of the plain and
-learned runs is measured against a relTol 1e-10 / absTol 1e-14 solve of a
-stratified sample of cells.
+their rates are reported beside `value`.  With the CPU baseline, the error
+of the plain, learned and seed-mode runs on a stratified sample of product
+and flame-front cells is measured against the oracle's relTol 1e-10 /
+absTol 1e-14 solve of those cells (`error_vs_oracle_tight`).
 
 Prints ONE JSON line.  The CPU baseline is the oracle (a scalar C
 restatement of the reference algorithm) timed on a bounded, stratified
@@ -80,7 +81,7 @@ def cpu_baseline(mech, field, seconds=15.0):
     # The same sample on every host core this process may use (at most 16,
     # the CPU share of one GPU): the oracle's OpenMP driver over cells,
     # dynamic schedule, one class at a time.
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = host_threads()
     wall = 0.0
     for c in cost:
         idx = per[c][:done[c]]
@@ -97,6 +98,55 @@ def cpu_baseline(mech, field, seconds=15.0):
                 all_cores=dict(value=field.n_cells / total_mt, cores=threads,
                                sample=f"the same sample, {wall:.1f} s wall on {threads} threads "
                                       f"(OpenMP over cells, dynamic schedule)"))
+
+
+def host_threads():
+    """Host cores this process may use, at most 16 (one GPU's CPU share)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def error_vs_oracle(mech, field, idx, runs):
+    """Error of GPU results (runs: name -> download() dict over the whole
+    field) on the cells idx against the oracle's relTol 1e-10 / absTol 1e-14
+    solve of the same cells, per cell class, with the oracle's own relTol
+    1e-4 solve beside them.  Cells the tight solve fails on are left out."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    o = Oracle(mech)
+    sub = field.subset(idx)
+    th = host_threads()
+    t0 = time.perf_counter()
+    tight = o.solve(sub, DT, threads=th, Treact=TREACT, absTol=1e-14, relTol=1e-10, maxSteps=2000000)
+    t_tight = time.perf_counter() - t0
+    ok = (tight["status"] & 6) == 0
+    rho = tight["rho0"]
+    Yt = sub.Y + tight["RR"] * DT / rho[None, :]
+    ref = o.solve(sub, DT, threads=th, Treact=TREACT)
+    rows = {"oracle_relTol_1e-4": {k: v for k, v in ref.items()}}
+    for name, r in runs.items():
+        rows[name] = {"RR": r["RR"][:, idx], "Tend": r["Tend"][idx]}
+    out = {}
+    for name, r in rows.items():
+        Y = sub.Y + r["RR"] * DT / rho[None, :]
+        per = {}
+        for c, cname in ((1, "product"), (2, "flame_front")):
+            m = ok & (sub.cls == c)
+            dY = np.abs(Y - Yt)[:, m]
+            Ym = np.abs(Yt[:, m])
+            per[cname] = dict(max_abs_Y=float(dY.max()),
+                              max_rel_Y=float((dY / np.maximum(Ym, 1e-6)).max()),
+                              max_tol_scaled_Y=float((dY / (1e-8 + 1e-4 * Ym)).max()),
+                              max_rel_T=float((np.abs(r["Tend"][m] - tight["Tend"][m]) / tight["Tend"][m]).max()))
+        out[name] = per
+    worst = lambda k: max(v[k] for v in out["learned"].values())
+    out["sample"] = (f"{int((sub.cls == 1).sum())} product + {int((sub.cls == 2).sum())} flame-front cells "
+                     f"(stratified, seed 3); reference: oracle OptRodas34 at relTol 1e-10 / absTol 1e-14 "
+                     f"({int(ok.sum())} of {idx.size} cells integrated, {t_tight:.1f} s on {th} threads); "
+                     "max_rel_Y = |dY| / max(|Y|, 1e-6), max_tol_scaled_Y = |dY| / (absTol + relTol |Y|)")
+    out["headline_bar"] = dict(max_abs_Y=worst("max_abs_Y"), max_rel_T=worst("max_rel_T"),
+                               holds=bool(worst("max_abs_Y") <= 1e-4 and worst("max_rel_T") <= 1e-4),
+                               reading="|dY| <= 1e-4 of the unit mass and |dT| <= 1e-4 T, learned run")
+    return out
 
 
 def timed_runs(batch, steps, warmup, profile):
@@ -198,6 +248,44 @@ def tutorials():
                           "attempts_per_s": att / best}), flush=True)
 
 
+PHASES = ["jacobian: thermo + reaction round", "jacobian: sparse term walk", "jacobian: production + dq/dT walk",
+          "jacobian: third-body partials", "jacobian: dC/dY conversion + T row", "iteration matrix",
+          "LU factorisation", "stage solves (all)", "RHS: density + thermo", "RHS: third-body sums",
+          "RHS: reaction round", "RHS: production rates", "RHS: T equation", "stage sums", "error norm"]
+
+
+def phases(args):
+    """Phase split of the step kernel on the default field (plain Rodas34,
+    eager launches) from the profiling build libfcs_phases.so: shader-clock
+    cycles per phase, summed over cells by each cell's first thread, divided
+    by the attempts made.  Build it first with
+    `python fastchemistrysolver-openfoam-10_amd/fcs/build.py --phase-timing`."""
+    import ctypes as C
+    from fcs import build, lib
+    from fcs.mechanism import config_field, config_mechanism
+    L = lib.load(build.LIB_PHASES)
+    L.fcs_phase_read.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
+    mech = config_mechanism(2)
+    gm = lib.Mech(mech)
+    solver = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
+    field = config_field(2, mech, n_cells=args.phase_side ** 2, equilibrate=lib.equilibrator(solver))
+    b = lib.Batch(solver, field)
+    b.run(DT)
+    buf = (C.c_uint64 * 16)()
+    L.fcs_phase_read(gm._h, buf, 1)          # clear
+    t0 = time.perf_counter()
+    b.run(DT)
+    wall = time.perf_counter() - t0
+    L.fcs_phase_read(gm._h, buf, 0)
+    att = int(b.download()["nAttempts"].sum())
+    cyc = [buf[i] / att for i in range(len(PHASES))]
+    tot = sum(cyc)
+    print(json.dumps({"workload": f"phase split of k_step<56,Rodas34>, plain Rodas34 on a {args.phase_side}x"
+                                  f"{args.phase_side} counterflow field (profiling build)",
+                      "attempts": att, "wall_s": wall, "cycles_per_attempt": tot,
+                      "phases": {n: {"cycles": c, "share": c / tot} for n, c in zip(PHASES, cyc)}}), flush=True)
+
+
 def _timed_run(batch):
     t0 = time.perf_counter()
     batch.run(1e-5)
@@ -239,7 +327,7 @@ def mixture(args, mech, gm):
             "config": {"workload": "plain OptRodas34, 65,536-cell 1-D mixture-fraction field, 53-species/"
                                    "325-reaction synthetic mechanism, deltaT 1e-6 s", "cells": f1.n_cells,
                        "attempts_per_step": att, "failed_cells": int(((r["status"] & 6) != 0).sum())},
-            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                          "kernel": "k_step<56,Rodas34>", "flops_per_attempt": fpa, "launches": launches,
                          "avg_launch_ms": 1e3 * avg_launch_s}}
@@ -335,7 +423,7 @@ def large(args):
                               "failed_cells": runs["fast"]["failed"]},
             "seulex": {"value": runs["seulex"]["value"], "columns_per_step": runs["seulex"]["att"] // args.steps,
                        "failed_cells": runs["seulex"]["failed"]},
-            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                          "kernel": "k_step<176,Rodas34> (one workgroup of 256 threads per active cell), plain run",
                          "flops_per_attempt": fpa, "launches": pl["launches"],
@@ -355,7 +443,7 @@ def main():
     ap.add_argument("--train-side", type=int, default=256, help="side of each training field")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt", "large", "large-kernels",
-                                           "traffic", "copy", "tutorials"),
+                                           "traffic", "copy", "tutorials", "phases"),
                     default="counterflow",
                     help="counterflow: learned-seeded vs plain Rodas34 on the 262,144-cell field (default); "
                          "mixture: plain Rodas34 on the 65,536-cell mixture-fraction field only; "
@@ -364,7 +452,9 @@ def main():
                          "large-kernels: its components through the kernel-level entries (for profiling); "
                          "traffic: one plain step of the default field with eager launches (for byte counters); "
                          "copy: 256 MiB device copies (the byte counters' calibration); "
-                         "tutorials: plain Rodas34 on the reference tutorials' mechanisms")
+                         "tutorials: plain Rodas34 on the reference tutorials' mechanisms; "
+                         "phases: per-phase cycles of the step kernel (profiling build libfcs_phases.so)")
+    ap.add_argument("--phase-side", type=int, default=128, help="side of the phases workload's field")
     ap.add_argument("--large-cells", type=int, default=131072, help="cells of the large workload (x 512)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -381,6 +471,8 @@ def main():
         return copy_calibration()
     if args.workload == "tutorials":
         return tutorials()
+    if args.workload == "phases":
+        return phases(args)
     from fcs import learned, lib
     from fcs.flops import attempt_flops
     from fcs.mechanism import config_field, config_mechanism, counterflow_field
@@ -444,6 +536,20 @@ def main():
     rl = lrn.download()
     npred, nver, nacc = lrn.learned_stats()
     log(f"learned: {wall_l / args.steps * 1e3:.1f} ms per step, {nacc}/{npred} proposals accepted")
+    # Amdahl bound of the learned seed on this field: the attempts the plain
+    # run spends on the cells the gate admitted (accepted proposals) and on
+    # all product cells, against the plain run's total; the speed-up if every
+    # such cell took one attempt (plus one verification per rejected proposal)
+    acc = (rl["status"] & 8) != 0
+    att_total = int(rp["nAttempts"].sum())
+    att_acc = int(rp["nAttempts"][acc].sum())
+    att_prod = int(rp["nAttempts"][field.cls == 1].sum())
+    amdahl = {"attempt_share_of_accepted_cells": att_acc / att_total,
+              "attempt_share_of_product_cells": att_prod / att_total,
+              "attempt_share_of_flame_front_cells": int(rp["nAttempts"][field.cls == 2].sum()) / att_total,
+              "max_speedup_admitted_cells_in_one_attempt": att_total / (att_total - att_acc + nacc + (nver - nacc)),
+              "max_speedup_every_product_cell_in_one_attempt":
+                  att_total / (att_total - att_prod + int((field.cls == 1).sum()))}
 
     # ---- seed mode: proposals verified as above, and every other reacting
     # cell starts from the network's first sub-step min(h, deltaT) instead of
@@ -463,22 +569,6 @@ def main():
     att_steps = int(rl["nAttempts"][(rl["status"] & 8) == 0].sum())
     avg_launch_s = 1e-3 * step_ms / max(launches, 1)
     achieved = fpa * att_steps / max(launches, 1) / avg_launch_s / 1e12
-
-    # ---- error against a tight-tolerance solve of a stratified sample
-    idx = stratified(field, 48)
-    sub = field.subset(idx)
-    tight = lib.Solver(gm, Treact=TREACT, absTol=1e-14, relTol=1e-10, maxSteps=2000000).solve(sub, DT)
-    if tight["rc"] != 0:
-        log(f"tight reference: {(tight['status'] & 6 != 0).sum()} cells failed")
-    rho = tight["rho0"]
-    Yt = sub.Y + tight["RR"] * DT / rho[None, :]
-
-    def err(r):
-        Y = sub.Y + r["RR"][:, idx] * DT / rho[None, :]
-        dY = np.abs(Y - Yt)
-        return dict(max_rel_Y=float((dY / np.maximum(np.abs(Yt), 1e-6)).max()),
-                    max_abs_Y=float(dY.max()),
-                    max_rel_T=float((np.abs(r["Tend"][idx] - tight["Tend"]) / tight["Tend"]).max()))
 
     # ---- plain Rodas34 on the 65,536-cell mixture-fraction field
     secondary = None
@@ -515,8 +605,7 @@ def main():
                    "training_samples": int(X.shape[0]), "training_epochs": args.epochs,
                    "training_s": t_data + t_train},
         "plain": {"value": plain_value, "ms_per_step": 1e3 * wall_p / args.steps,
-                  "attempts_per_step": int(rp["nAttempts"].sum()),
-                  "error_vs_tight": err(rp)},
+                  "attempts_per_step": int(rp["nAttempts"].sum())},
         "learned": {"speedup_vs_plain": value / plain_value, "proposals": npred, "verified": nver,
                     "accepted": nacc, "acceptance_rate": nacc / max(npred, 1),
                     "attempts_per_step": int(rl["nAttempts"].sum()),
@@ -524,10 +613,8 @@ def main():
                         np.array_equal(rl[k][..., (rl["status"] & 8) == 0], rp[k][..., (rl["status"] & 8) == 0])
                         for k in ("RR", "Tend", "nAttempts"))),
                     "failed_cells": int(((rl["status"] & 6) != 0).sum()),
-                    "error_vs_tight": err(rl)},
-        "error_sample": "48 product + 48 flame-front cells, relTol 1e-10 / absTol 1e-14 GPU solve as reference "
-                        "(max_rel_Y: |dY| / max(|Y|, 1e-6))",
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "amdahl_bound": amdahl},
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "traffic_unit": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
                                      "profiles/traffic_k_step_rodas34_53sp_counterflow_262144cells.md)",
@@ -539,12 +626,13 @@ def main():
     line["learned_seed_mode"] = {"value": field.n_cells / (wall_s / args.steps),
                                  "speedup_vs_plain": (field.n_cells / (wall_s / args.steps)) / plain_value,
                                  "attempts_per_step": int(rs["nAttempts"].sum()),
-                                 "failed_cells": int(((rs["status"] & 6) != 0).sum()),
-                                 "error_vs_tight": err(rs)}
+                                 "failed_cells": int(((rs["status"] & 6) != 0).sum())}
     if secondary:
         line["plain_mixture_fraction_65536"] = secondary
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mech, field, seconds=args.cpu_seconds)
+        line["error_vs_oracle_tight"] = error_vs_oracle(mech, field, stratified(field, 48),
+                                                        {"plain": rp, "learned": rl, "learned_seed_mode": rs})
     print(json.dumps(line), flush=True)
 
 

@@ -116,6 +116,9 @@ struct DevMech {
     int maxTerms;             // per-species Jacobian term list length, multiple of 16
     const unsigned long long* sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*
     const double* Hf;         // [SW] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
+#ifdef FCS_PHASE_TIMING
+    unsigned long long* phase;  // [PH_SLOTS] shader-clock cycles per phase, summed over cells
+#endif
 };
 
 // Jacobian term of species row i: nu_ir * d q_r / d C_j for one reactant
@@ -192,6 +195,34 @@ __device__ __forceinline__ int first_lane(bool pred)
 }
 // Orders LDS traffic between the lanes of the (single-wave) workgroup.
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// ---------------------------------------------------------------- phase clock
+// Profiling builds (-DFCS_PHASE_TIMING) add the shader-clock cycles of each
+// phase of an attempt into M.phase[slot] (once per cell, by its first
+// thread); in other builds the clock compiles to nothing.
+enum {
+    PH_JAC_RATES, PH_JAC_TERMS, PH_JAC_PROD, PH_JAC_TB, PH_JAC_CONV,   // Jacobian evaluation
+    PH_FORM, PH_LU, PH_SOLVE,                                            // iteration matrix, LU, stage solves
+    PH_RHS_SETUP, PH_RHS_TB, PH_RHS_RATES, PH_RHS_PROD, PH_RHS_REST,    // derivatives()
+    PH_STAGE, PH_ERR,                                                    // stage sums, error norm
+    PH_SLOTS = 16
+};
+#ifdef FCS_PHASE_TIMING
+struct PhaseClock {
+    unsigned long long t;
+    __device__ PhaseClock() : t(__builtin_amdgcn_s_memtime()) {}
+    __device__ void mark(const DevMech& M, bool first, int slot)
+    {
+        const unsigned long long n = __builtin_amdgcn_s_memtime();
+        if (first) atomicAdd(M.phase + slot, n - t);
+        t = n;
+    }
+};
+#else
+struct PhaseClock {
+    __device__ void mark(const DevMech&, bool, int) {}
+};
+#endif
 
 // ---------------------------------------------------------------- LDS layout
 // The per-cell matrix lives in LDS, lane-interleaved column-major: element
@@ -801,6 +832,7 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
     const bool sp = lane < M.ns;
     const SpeciesData S = load_species<NP>(M, lane);  // tables are padded to the group size
     if (sp) y = y > 0.0 ? y : 0.0;
+    PhaseClock clk;
     const double T = gbcast<NP>(y, NP - 1, L);
     const double RuTByP = RU * T / p;                 // unclamped T for the density
     const double invW = sp ? S.invW : 0.0;
@@ -812,15 +844,20 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
     L.C[lane] = sp ? rho * invW * y : 0.0;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     wave_sync();
+    clk.mark(M, lane == 0, PH_RHS_SETUP);
     third_body_sums<NP>(M, L);
     wave_sync();
+    clk.mark(M, lane == 0, PH_RHS_TB);
     reaction_round<false, NP>(M, L, Tc, lnT, iT, lnp);
     wave_sync();
+    clk.mark(M, lane == 0, PH_RHS_RATES);
     const double om = sp ? production_rate<NP>(M, L) : 0.0;
+    clk.mark(M, lane == 0, PH_RHS_PROD);
     const double Yd = sp ? om * (S.W * invrho) : 0.0;
     const double cpM = gsum<NP>(sp ? y * th.cp : 0.0, L);
     const double hY = gsum<NP>(sp ? th.ha * Yd : 0.0, L);
     wave_sync();
+    clk.mark(M, lane == 0, PH_RHS_REST);
     return sp ? Yd : (lane == NP - 1 ? -hY / cpM : 0.0);
 }
 
@@ -842,6 +879,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     const int lane = tid<NP>();
     const int ns = M.ns;
     const bool sp = lane < ns;
+    PhaseClock clk;
     const SpeciesData S = load_species<NP>(M, lane);  // tables are padded to the group size
     if (sp) y = y > 0.0 ? y : 0.0;
     const double T = clampT(M, gbcast<NP>(y, NP - 1, L));
@@ -868,6 +906,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     wave_sync();
     reaction_round<true, NP>(M, L, T, lnT, iT, lnp);
     wave_sync();
+    clk.mark(M, lane == 0, PH_JAC_RATES);
     // species rows, sparse partials: row i is a list of terms nu_ir dq_r/dC_j
     // sorted by column (host order within a column follows the species'
     // reaction list).  Each lane accumulates one column at a time in a
@@ -909,6 +948,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
         }
         Jrow[cur * JSt] = acc;
     }
+    clk.mark(M, lane == 0, PH_JAC_TERMS);
     // production rate and d(omega)/dT of this lane's species (its reaction
     // list, loads and gathers issued per block of 8 entries)
     double om = 0.0, jT = 0.0;
@@ -949,6 +989,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
         }
     }
     wave_sync();
+    clk.mark(M, lane == 0, PH_JAC_PROD);
     // third-body / falloff partials: element (i, j) gains sum_r nu_ir w_r
     // eff_rj over the third-body/falloff entries of species i.  Column-wise
     // over a flat entry list sorted by row (uniform loop, 16 entries per
@@ -983,6 +1024,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
         if (sp) L.J[lane * JSt + cur] = old + acc;
     }
     wave_sync();
+    clk.mark(M, lane == 0, PH_JAC_TB);
     // dC/dY conversion of the species rows, T column
     const double WiByRho = sp ? S.W * invrho : 0.0;
     const double Yd = sp ? om * WiByRho : 0.0;
@@ -1049,6 +1091,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
         L.J[lane * JSt + (NP - 1)] = rowT;
     }
     wave_sync();
+    clk.mark(M, lane == 0, PH_JAC_CONV);
     return sp ? Yd : (lane == NP - 1 ? dTdt : 0.0);
 }
 

@@ -220,9 +220,13 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
                                      double p, double lnp, double dx, double& y0, double& yT)
 {
     const double invdx = 1.0 / dx;
+    const bool first = tid<NP>() == 0;
     const double dfdx = jacobian<NP>(M, L, p, lnp, o.jac, y0);
+    PhaseClock clk;
     form_matrix<NP>(L, 1.0 * invdx * tb.invGamma);
+    clk.mark(M, first, PH_FORM);
     const auto st = lu_factor_any<NP>(L);
+    clk.mark(M, first, PH_LU);
     double k[6] = {0, 0, 0, 0, 0, 0};
     double dydx = dfdx;   // a stage without its own f before any f evaluation takes f(y0)
     for (int s = 0; s < tb.stages; ++s) {
@@ -234,13 +238,17 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
                 double yS = y0;
 #pragma unroll
                 for (int j = 0; j < 5; ++j) yS += tb.A[s][j] * k[j];
+                clk.mark(M, first, PH_STAGE);
                 dydx = derivatives<NP>(M, L, p, lnp, yS);
+                clk = PhaseClock();
             }
             rhs = dydx + (dx * tb.D[s]) * dfdx;
 #pragma unroll
             for (int j = 0; j < 5; ++j) rhs += (tb.C[s][j] * invdx) * k[j];
         }
+        clk.mark(M, first, PH_STAGE);
         const double ks = lu_solve_any<NP>(L, st, rhs);
+        clk.mark(M, first, PH_SOLVE);
 #pragma unroll
         for (int j = 0; j < 6; ++j)
             if (j == s) k[j] = ks;
@@ -252,7 +260,9 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
         yT += tb.B[j] * k[j];
         err += tb.E[j] * k[j];
     }
-    return error_norm<NP>(o, L, y0, yT, err);
+    const double norm = error_norm<NP>(o, L, y0, yT, err);
+    clk.mark(M, first, PH_ERR);
+    return norm;
 }
 
 // Step-size proposal after an accepted step of size dx with error norm Err

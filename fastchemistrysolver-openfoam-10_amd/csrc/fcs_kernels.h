@@ -242,10 +242,9 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_verify(DevMech M, DevOpts o, Ce
             }
         }
         if (!accept && lane == 0) {
-            if (seed) {
-                const double hs = h < dt ? h : dt;
-                a.dxTry[c] = (hs > 0.0 && hs == hs) ? hs : a.dxTry[c];
-            }
+            // a proposal that is not a positive number (NaN, or exp underflow)
+            // leaves the cell's deltaTChem in place
+            if (seed && h == h && h > 0.0) a.dxTry[c] = h < dt ? h : dt;
             listOut[atomicAdd(countOut, 1u)] = c;
         }
     }

@@ -65,6 +65,25 @@ __global__ __launch_bounds__(256) void k_setup(DevOpts o, int ns, CellArrays a, 
     list[atomicAdd(count, 1u)] = c;
 }
 
+// Cells still in an active list when a run reaches its launch cap (a bound
+// that the per-cell attempt cap keeps out of reach) are finished as failed,
+// as finish_cell finishes a cell over maxSteps: status CELL_MAXSTEPS, the
+// input deltaTChem, RR = 0 and the temperature of the last accepted state.
+__global__ __launch_bounds__(256) void k_fail_list(DevOpts o, int ns, CellArrays a, const int* list,
+                                                   const unsigned int* count)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const int c = list[i];
+    const size_t nc = (size_t)a.nc;
+    for (int s = 0; s < ns; ++s) a.RR[s * nc + c] = 0.0;
+    const double dtc = a.dtcIn[c];
+    a.status[c] = CELL_MAXSTEPS;
+    a.Tend[c] = a.Yw[(size_t)ns * nc + c];
+    a.dtcOut[c] = dtc < o.dtChemMax ? dtc : o.dtChemMax;
+    if (dtc > 0.0) atomicMin(a.dtMinBits, (unsigned long long)__double_as_longlong(dtc));
+}
+
 // Network inputs of the cells of a list, one thread per entry: row i =
 // [T, log p, Y_0 .. Y_ns-1, log deltaT] in raw units (the network holds its
 // own normalisation).  Y is the clipped initial state k_setup prepared.
@@ -568,6 +587,14 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     }
     auto P = [&](size_t i) { return (void*)((char*)m->dbuf + pieces[i].off); };
     DevMech& dm = m->dm;
+#ifdef FCS_PHASE_TIMING
+    if (hipMalloc((void**)&dm.phase, PH_SLOTS * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(dm.phase, 0, PH_SLOTS * sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipFree(m->dbuf);
+        delete m;
+        return FCS_E_NOMEM;
+    }
+#endif
     dm.ns = ns;
     dm.nr = nr;
     dm.NPm = SW;
@@ -627,9 +654,25 @@ int fcs_mech_destroy(fcs_mech* m)
 {
     if (!m) return FCS_E_ARG;
     if (m->dbuf) (void)hipFree(m->dbuf);
+#ifdef FCS_PHASE_TIMING
+    if (m->dm.phase) (void)hipFree(m->dm.phase);
+#endif
     delete m;
     return FCS_OK;
 }
+
+#ifdef FCS_PHASE_TIMING
+// Profiling build only: the per-phase cycle sums of every attempt made with
+// this mechanism since the last reset (PH_* slots), optionally cleared.
+int fcs_phase_read(fcs_mech* m, unsigned long long* out, int reset)
+{
+    if (!m || !out) return FCS_E_ARG;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(out, m->dm.phase, PH_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (reset) HIP_OK(hipMemset(m->dm.phase, 0, PH_SLOTS * sizeof(unsigned long long)));
+    return FCS_OK;
+}
+#endif
 
 static int solver_grid(fcs_solver* s)
 {
@@ -777,6 +820,19 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
         cout = d.counts + (it + 1) % 3;
         czero = d.counts + (it + 2) % 3;
     };
+    // launch cap reached: let the queued launches drain, then finish the
+    // cells of the last output list as failed (FCS_E_CELLS)
+    auto abandon = [&](int* list, unsigned int* count) -> int {
+        HIP_OK(hipStreamSynchronize(st));
+        unsigned int left = 0;
+        HIP_OK(hipMemcpy(&left, count, sizeof left, hipMemcpyDeviceToHost));
+        if (left > 0) {
+            void* args[] = {&o, &ns, &a, &list, &count};
+            HIP_OK(hipLaunchKernel((const void*)k_fail_list, dim3((left + 255) / 256), dim3(256), args, 0, st));
+            HIP_OK(hipStreamSynchronize(st));
+        }
+        return FCS_E_CELLS;
+    };
     d.lastStepMs = 0.0;
     if (d.graphLaunches > 0) {
         // hipGraph replays of G step launches (G a multiple of 6, so every
@@ -845,7 +901,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
                 HIP_OK(hipEventSynchronize(d.ev[bs]));
                 if (d.hostCount[bs] == 0u) break;
             }
-            if (r > maxReplays) return FCS_E_HIP;
+            if (r > maxReplays) return abandon((G - 1) & 1 ? d.list0 : d.list1, lastOut);
         }
         HIP_OK(hipStreamSynchronize(st));
         return FCS_OK;
@@ -876,7 +932,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
             HIP_OK(hipEventSynchronize(d.ev[bs]));
             if (d.hostCount[bs] == 0u) break;
         }
-        if (k > maxLaunches) return FCS_E_HIP;
+        if (k > maxLaunches) return abandon(lout, cout);
     }
     d.lastStepMs = 0.0;
     if (d.profile) {
@@ -922,7 +978,7 @@ static int solve_host(fcs_solver* s, int64_t nCells, double deltaT, const double
     for (int i = 0; i < ns; ++i)
         HIP_OK(hipMemcpyAsync(d.Y0 + (size_t)i * nc, Y0[i], nc * 8, hipMemcpyHostToDevice, st));
     int rc = run_cells(s, d, (int)nc, deltaT, dtField != nullptr);
-    if (rc != FCS_OK) return rc;
+    if (rc != FCS_OK && rc != FCS_E_CELLS) return rc;   // failed cells: results are still written
     for (int i = 0; i < ns; ++i)
         HIP_OK(hipMemcpyAsync(RR[i], d.RR + (size_t)i * nc, nc * 8, hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(deltaTChem, d.dtcOut, nc * 8, hipMemcpyDeviceToHost, st));
@@ -1019,11 +1075,11 @@ int fcs_batch_run(fcs_batch* b, double deltaT, float* elapsed_ms)
     b->lastDt = deltaT;
     HIP_OK(hipEventRecord(b->e0, s->stream));
     int rc = run_cells(s, b->cells, (int)b->nc, deltaT, false, b->mlp, b->learnedMode);
-    if (rc != FCS_OK) return rc;
+    if (rc != FCS_OK && rc != FCS_E_CELLS) return rc;
     HIP_OK(hipEventRecord(b->e1, s->stream));
     HIP_OK(hipEventSynchronize(b->e1));
     if (elapsed_ms) HIP_OK(hipEventElapsedTime(elapsed_ms, b->e0, b->e1));
-    return FCS_OK;
+    return rc;
 }
 
 int fcs_batch_set_launch(fcs_batch* b, int32_t grid, int32_t graphLaunches)

@@ -12,6 +12,9 @@ from concurrent.futures import ThreadPoolExecutor
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(PKG_ROOT, "libfcs.so")
+# profiling variant: the same library with the per-phase cycle counters of
+# the attempt compiled in (-DFCS_PHASE_TIMING, fcs_phase_read exported)
+LIB_PHASES = os.path.join(PKG_ROOT, "libfcs_phases.so")
 HARNESS = os.path.join(PKG_ROOT, "fcs_harness")
 CSRC = os.path.join(PKG_ROOT, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -47,11 +50,15 @@ def _run(cmd, verbose):
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{' '.join(cmd)}\n{out}")
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
+def build(verbose: bool = False, jobs: int | None = None, phase_timing: bool = False) -> str:
     """Compile csrc/*.hip into libfcs.so (always recompiles): the host side,
     the network kernels and one object per padded size, compiled in
-    parallel, then linked."""
+    parallel, then linked.  phase_timing builds the profiling variant
+    libfcs_phases.so instead."""
     inc = ["-I" + os.path.join(REPO, "include"), "-I" + CSRC]
+    if phase_timing:
+        inc.append("-DFCS_PHASE_TIMING")
+    out = LIB_PHASES if phase_timing else LIB
     with tempfile.TemporaryDirectory(prefix="fcs_build_") as tmp:
         units = [(os.path.join(CSRC, "fcs_lib.hip"), [], os.path.join(tmp, "fcs_lib.o")),
                  (os.path.join(CSRC, "fcs_mlp.hip"), [], os.path.join(tmp, "fcs_mlp.o"))]
@@ -62,10 +69,11 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         with ThreadPoolExecutor(max_workers=jobs) as ex:
             for f in [ex.submit(_run, c, verbose) for c in cmds]:
                 f.result()
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", LIB + ".tmp", *[obj for _, _, obj in units]], verbose)
-    os.replace(LIB + ".tmp", LIB)
-    build_harness(verbose)
-    return LIB
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", out + ".tmp", *[obj for _, _, obj in units]], verbose)
+    os.replace(out + ".tmp", out)
+    if not phase_timing:
+        build_harness(verbose)
+    return out
 
 
 def build_harness(verbose: bool = False) -> str:
@@ -83,5 +91,5 @@ def build_harness(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, phase_timing="--phase-timing" in sys.argv[1:]))
     sys.exit(0)

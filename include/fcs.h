@@ -235,7 +235,12 @@ int fcs_batch_upload(fcs_batch* batch, const double* T0, const double* p0,
 /* Runs the integration of every cell of the batch over deltaT on the
  * batch's stream, restarting from the uploaded state each call.  When
  * elapsed_ms is non-NULL the call waits and returns the time between HIP
- * events recorded on that stream around the kernel launches. */
+ * events recorded on that stream around the kernel launches.  Per-cell
+ * failures are reported through the downloaded cellStatus; FCS_E_CELLS is
+ * returned only if the run reaches its launch cap (101 * maxSteps step
+ * launches, out of reach while every cell obeys its own attempt cap), after
+ * the queued launches have drained and the cells still in flight have been
+ * finished as FCS_CELL_MAXSTEPS. */
 int fcs_batch_run(fcs_batch* batch, double deltaT, float* elapsed_ms);
 /* Profiling of fcs_batch_run (no counterpart in the reference): with
  * enable != 0 each step launch is bracketed by a pair of HIP events on the
