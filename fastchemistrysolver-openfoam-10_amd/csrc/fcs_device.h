@@ -117,7 +117,7 @@ struct DevMech {
     const unsigned long long* sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*
     const double* Hf;         // [SW] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
 #ifdef FCS_PHASE_TIMING
-    unsigned long long* phase;  // [PH_SLOTS] shader-clock cycles per phase, summed over cells
+    unsigned long long* phase;  // [PH_SLOTS * PH_SPREAD] shader-clock cycles per phase
 #endif
 };
 
@@ -198,14 +198,16 @@ __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // ---------------------------------------------------------------- phase clock
 // Profiling builds (-DFCS_PHASE_TIMING) add the shader-clock cycles of each
-// phase of an attempt into M.phase[slot] (once per cell, by its first
-// thread); in other builds the clock compiles to nothing.
+// phase of an attempt into M.phase[slot * PH_SPREAD + block % PH_SPREAD]
+// (once per cell, by its first thread; spread over PH_SPREAD counters per
+// slot so the atomics do not queue on one address); in other builds the
+// clock compiles to nothing.
 enum {
     PH_JAC_RATES, PH_JAC_TERMS, PH_JAC_PROD, PH_JAC_TB, PH_JAC_CONV,   // Jacobian evaluation
     PH_FORM, PH_LU, PH_SOLVE,                                            // iteration matrix, LU, stage solves
     PH_RHS_SETUP, PH_RHS_TB, PH_RHS_RATES, PH_RHS_PROD, PH_RHS_REST,    // derivatives()
     PH_STAGE, PH_ERR,                                                    // stage sums, error norm
-    PH_SLOTS = 16
+    PH_SLOTS = 16, PH_SPREAD = 256
 };
 #ifdef FCS_PHASE_TIMING
 struct PhaseClock {
@@ -214,7 +216,7 @@ struct PhaseClock {
     __device__ void mark(const DevMech& M, bool first, int slot)
     {
         const unsigned long long n = __builtin_amdgcn_s_memtime();
-        if (first) atomicAdd(M.phase + slot, n - t);
+        if (first) atomicAdd(M.phase + slot * PH_SPREAD + (blockIdx.x & (PH_SPREAD - 1)), n - t);
         t = n;
     }
 };

@@ -588,8 +588,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     auto P = [&](size_t i) { return (void*)((char*)m->dbuf + pieces[i].off); };
     DevMech& dm = m->dm;
 #ifdef FCS_PHASE_TIMING
-    if (hipMalloc((void**)&dm.phase, PH_SLOTS * sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(dm.phase, 0, PH_SLOTS * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc((void**)&dm.phase, PH_SLOTS * PH_SPREAD * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(dm.phase, 0, PH_SLOTS * PH_SPREAD * sizeof(unsigned long long)) != hipSuccess) {
         (void)hipFree(m->dbuf);
         delete m;
         return FCS_E_NOMEM;
@@ -668,8 +668,13 @@ int fcs_phase_read(fcs_mech* m, unsigned long long* out, int reset)
 {
     if (!m || !out) return FCS_E_ARG;
     HIP_OK(hipDeviceSynchronize());
-    HIP_OK(hipMemcpy(out, m->dm.phase, PH_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    if (reset) HIP_OK(hipMemset(m->dm.phase, 0, PH_SLOTS * sizeof(unsigned long long)));
+    std::vector<unsigned long long> h((size_t)PH_SLOTS * PH_SPREAD);
+    HIP_OK(hipMemcpy(h.data(), m->dm.phase, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int k = 0; k < PH_SLOTS; ++k) {
+        out[k] = 0;
+        for (int j = 0; j < PH_SPREAD; ++j) out[k] += h[(size_t)k * PH_SPREAD + j];
+    }
+    if (reset) HIP_OK(hipMemset(m->dm.phase, 0, h.size() * sizeof(unsigned long long)));
     return FCS_OK;
 }
 #endif
