@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1215,6 +1216,156 @@ int fcs_batch_download(fcs_batch* b, double* RR, double* deltaTChem, double* del
         std::memcpy(&dmin, &bits, 8);
         dmin = std::min(dmin, OF_GREAT);
         *deltaTMin = std::min(dmin, 2 * b->lastDt);
+    }
+    return FCS_OK;
+}
+
+// ------------------------------------------------------------ no-allocation solve
+// fcs_solve_noalloc keeps the integrator's per-cell state, the active lists
+// and the counters in this array, one copy per GPU, held until the process
+// ends.  Host code must not pass the array's name to a kernel (it is a
+// host-side stand-in there): the device address comes from
+// hipGetSymbolAddress, once per device, and is kept in gScratchPtr.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+void* gScratchPtr[kMaxDevices] = {};
+
+int scratch_of_current_device(char** out)
+{
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return FCS_E_HIP;
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchPtr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FCS_E_HIP;
+        gScratchPtr[dev] = p;
+    }
+    *out = (char*)gScratchPtr[dev];
+    return FCS_OK;
+}
+
+// Carves the per-cell state of nc cells (and the Seulex extras when
+// jyDoubles > 0) out of the scratch at base; returns the bytes used.
+size_t carve_scratch(char* base, size_t nc, int ns, size_t jyDoubles, CellArrays& a, int*& list0, int*& list1,
+                     unsigned int*& counts, unsigned long long*& nAccepted)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) { char* r = base + off; off += align256(bytes); return r; };
+    counts = (unsigned int*)take(4 * sizeof(unsigned int));
+    a.dtMinBits = (unsigned long long*)take(8);
+    nAccepted = (unsigned long long*)take(16);
+    a.Yw = (double*)take(nc * (ns + 1) * 8);
+    a.x = (double*)take(nc * 8);
+    a.dxTry = (double*)take(nc * 8);
+    a.dx = (double*)take(nc * 8);
+    a.dxTry0 = (double*)take(nc * 8);
+    a.dxFirst = (double*)take(nc * 8);
+    a.flags = (int*)take(nc * 4);
+    a.nStep = (int*)take(nc * 4);
+    a.nAtt = (int*)take(nc * 4);
+    list0 = (int*)take(nc * 4);
+    list1 = (int*)take(nc * 4);
+    a.Y0s = a.Scl = a.theta = a.Jy = nullptr;
+    a.kTarg = nullptr;
+    if (jyDoubles) {
+        a.Y0s = (double*)take(nc * (ns + 1) * 8);
+        a.Scl = (double*)take(nc * (ns + 1) * 8);
+        a.theta = (double*)take(nc * 8);
+        a.kTarg = (int*)take(nc * 4);
+        a.Jy = (double*)take(nc * jyDoubles * 8);
+    }
+    return off;
+}
+}  // namespace
+
+int fcs_solve_noalloc(fcs_solver* s, int64_t nCells, double deltaT, const double* T0, const double* p0,
+                      const double* rho0, const double* Y0, const double* deltaTChemIn, double* deltaTChemOut,
+                      double* RR, int32_t* cellStatus, double* Tend, double* deltaTMin)
+{
+    if (!s || nCells < 0 || nCells > 0x7fffffff || !(deltaT > 0) ||
+        (nCells > 0 && (!T0 || !p0 || !rho0 || !Y0 || !deltaTChemIn || !deltaTChemOut || !RR || !cellStatus ||
+                        !Tend)))
+        return FCS_E_ARG;
+    // the workgroup form with its matrix in HBM needs per-workgroup slices
+    // beyond this scratch
+    if (s->ks->NP > 64 && !jlds_of(s->ks->NP)) return FCS_E_UNSUPPORTED;
+    const int ns = s->mech->ns;
+    const size_t nc = (size_t)nCells;
+    char* base = nullptr;
+    const int rc0 = scratch_of_current_device(&base);
+    if (rc0 != FCS_OK) return rc0;
+    CellArrays a{};
+    int *list0 = nullptr, *list1 = nullptr;
+    unsigned int* counts = nullptr;
+    unsigned long long* nAccepted = nullptr;
+    if (carve_scratch(base, nc, ns, jy_doubles(s), a, list0, list1, counts, nAccepted) > kScratchBytes)
+        return FCS_E_SCRATCH;
+    a.nc = (int)nc;
+    a.dt = deltaT;
+    a.dtCell = nullptr;
+    a.T0 = T0; a.p0 = p0; a.rho0 = rho0; a.Y0 = Y0; a.dtcIn = deltaTChemIn;
+    a.dtcOut = deltaTChemOut; a.RR = RR; a.Tend = Tend; a.status = cellStatus;
+    a.Jg = nullptr;
+    hipStream_t st = s->stream;
+    DevMech M = s->mech->dm;
+    DevOpts o = dev_opts(s->opts);
+    int nsp = ns;
+    {
+        void* args[] = {&counts, &a.dtMinBits, &nAccepted};
+        HIP_OK(hipLaunchKernel((const void*)k_reset, dim3(1), dim3(1), args, 0, st));
+    }
+    if (nc > 0) {
+        unsigned int* c0 = counts;
+        void* args[] = {&o, &nsp, &a, &list0, &c0};
+        HIP_OK(hipLaunchKernel((const void*)k_setup, dim3((unsigned)((nc + 255) / 256)), dim3(256), args, 0, st));
+        const void* f = s->ks->step[s->opts.solver];
+        const int grid = std::max(1, std::min(s->grid, (int)nc));
+        // same list / counter rotation as run_cells; the host reads the
+        // active-list length after every kPollEvery launches with a
+        // synchronous copy into a stack word (no pinned buffer, no events)
+        constexpr int kPollEvery = 8;
+        const long long maxLaunches = 101LL * s->opts.maxSteps + 2 * kPollEvery + 2;
+        for (long long k = 0;; ++k) {
+            int* lin = (k & 1) ? list1 : list0;
+            int* lout = (k & 1) ? list0 : list1;
+            unsigned int* cin = counts + k % 3;
+            unsigned int* cout = counts + (k + 1) % 3;
+            unsigned int* czero = counts + (k + 2) % 3;
+            void* args[] = {&M, &o, &a, &lin, &cin, &lout, &cout, &czero};
+            HIP_OK(hipLaunchKernel(f, dim3(grid), dim3(s->ks->G), args, s->ldsBytes, st));
+            if ((k + 1) % kPollEvery == 0) {
+                unsigned int left = 0;
+                HIP_OK(hipMemcpyAsync(&left, cout, sizeof left, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                if (left == 0u) break;
+            }
+            if (k > maxLaunches) {
+                // out of reach while every cell obeys its attempt cap: finish
+                // the cells still in flight as failed, as run_cells does
+                HIP_OK(hipStreamSynchronize(st));
+                unsigned int left = 0;
+                HIP_OK(hipMemcpy(&left, cout, sizeof left, hipMemcpyDeviceToHost));
+                void* fargs[] = {&o, &nsp, &a, &lout, &cout};
+                if (left > 0)
+                    HIP_OK(hipLaunchKernel((const void*)k_fail_list, dim3((left + 255) / 256), dim3(256), fargs, 0,
+                                           st));
+                break;
+            }
+        }
+    }
+    unsigned long long bits = 0;
+    HIP_OK(hipMemcpyAsync(&bits, a.dtMinBits, 8, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (deltaTMin) {
+        double dmin;
+        std::memcpy(&dmin, &bits, 8);
+        dmin = std::min(dmin, OF_GREAT);
+        *deltaTMin = std::min(dmin, 2 * deltaT);
     }
     return FCS_OK;
 }

@@ -8,7 +8,7 @@ import numpy as np
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
-FCS_OK, FCS_E_ARG, FCS_E_HIP, FCS_E_UNSUPPORTED, FCS_E_NOMEM, FCS_E_CELLS = 0, -1, -2, -3, -4, -5
+FCS_OK, FCS_E_ARG, FCS_E_HIP, FCS_E_UNSUPPORTED, FCS_E_NOMEM, FCS_E_CELLS, FCS_E_SCRATCH = 0, -1, -2, -3, -4, -5, -6
 FCS_CELL_OK, FCS_CELL_SKIPPED, FCS_CELL_MAXSTEPS, FCS_CELL_NONFINITE = 0, 1, 2, 4
 FCS_SOLVER_RODAS34, FCS_SOLVER_ROSENBROCK34, FCS_SOLVER_SEULEX = 0, 1, 2
 FCS_SOLVER_RODAS23, FCS_SOLVER_ROSENBROCK23 = 3, 4

@@ -24,7 +24,7 @@ EXPORTS = [
     "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_download",
     "fcs_eval_derivatives", "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
     "fcs_batch_set_learned", "fcs_batch_learned_stats", "fcs_batch_first_steps", "fcs_batch_set_launch",
-    "fcs_batch_autotune", "fcs_calculate", "fcs_tc", "fcs_qdot",
+    "fcs_batch_autotune", "fcs_calculate", "fcs_tc", "fcs_qdot", "fcs_solve_noalloc",
     "fcs_mlp_create", "fcs_mlp_destroy", "fcs_mlp_param_count", "fcs_mlp_get_params", "fcs_mlp_set_params",
     "fcs_mlp_set_norm", "fcs_mlp_forward", "fcs_mlp_loss_grad", "fcs_mlp_train",
 ]
@@ -78,6 +78,8 @@ def load(path: str | None = None):
     L.fcs_batch_first_steps.argtypes = [_vp, _dp]
     L.fcs_batch_set_launch.argtypes = [_vp, C.c_int32, C.c_int32]
     L.fcs_batch_autotune.argtypes = [_vp, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    # device pointers (integers) for the arrays of the no-allocation solve
+    L.fcs_solve_noalloc.argtypes = [_vp, C.c_int64, C.c_double] + [_vp] * 9 + [_dp]
     L.fcs_mlp_create.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.c_uint64, C.POINTER(_vp)]
     L.fcs_mlp_destroy.argtypes = [_vp]
     L.fcs_mlp_param_count.argtypes = [_vp]

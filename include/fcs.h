@@ -38,6 +38,7 @@ extern "C" {
 #define FCS_E_UNSUPPORTED  -3   /* mechanism outside the compiled size range    */
 #define FCS_E_NOMEM        -4   /* device or host allocation failed             */
 #define FCS_E_CELLS        -5   /* one or more cells failed (see cellStatus)    */
+#define FCS_E_SCRATCH      -6   /* input beyond fcs_solve_noalloc's scratch     */
 
 /* ---- per-cell status bits (cellStatus output) ------------------------- */
 #define FCS_CELL_OK            0
@@ -251,6 +252,28 @@ int fcs_batch_stats(fcs_batch* batch, int64_t* stepLaunches, double* stepKernelM
 int fcs_batch_download(fcs_batch* batch, double* RR /* [ns*nCells] */,
                        double* deltaTChem, double* deltaTMin,
                        int32_t* cellStatus, double* Tend, int32_t* nAttempts);
+
+/*
+ * No-allocation solve(deltaT) (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The variant of
+ * fcs_batch_upload + fcs_batch_run + fcs_batch_download over arrays the
+ * caller already holds in device memory, in the batch layout
+ * (Y0 and RR [ns*nCells], species-major); plain integrator only, same
+ * kernels and results as fcs_batch_run.  It calls no hipMalloc, hipFree or
+ * other allocating routine: the integrator state, active lists and counters
+ * live in a library-owned 32 MiB device array, one per GPU, held until the
+ * process ends; every call on a GPU shares it, so calls that overlap on one
+ * GPU race.  About (ns+1)*8 + 64 bytes per cell (Rosenbrock solvers), so
+ * some 70,000 cells at 53 species; larger inputs return FCS_E_SCRATCH.
+ * FCS_E_UNSUPPORTED for mechanisms whose workgroup form keeps its matrix in
+ * HBM (more than 111 species).  deltaTMin (host, may be NULL) as
+ * fcs_batch_download returns it. */
+int fcs_solve_noalloc(fcs_solver* solver, int64_t nCells, double deltaT,
+                      const double* T0, const double* p0, const double* rho0,
+                      const double* Y0 /* [ns*nCells] */, const double* deltaTChemIn,
+                      double* deltaTChemOut, double* RR /* [ns*nCells] */,
+                      int32_t* cellStatus, double* Tend, double* deltaTMin);
 
 /*
  * Launch shape of a batch's step kernel (no counterpart in the reference:
