@@ -252,6 +252,7 @@ PHASES = ["jacobian: thermo + reaction round", "jacobian: sparse term walk", "ja
           "jacobian: third-body partials", "jacobian: dC/dY conversion + T row", "iteration matrix",
           "LU factorisation", "stage solves (all)", "RHS: density + thermo", "RHS: third-body sums",
           "RHS: reaction round", "RHS: production rates", "RHS: T equation", "stage sums", "error norm"]
+LU_PARTS = ["LU: panels", "LU: U rows", "LU: trailing updates"]
 
 
 def phases(args):
@@ -271,7 +272,7 @@ def phases(args):
     field = config_field(2, mech, n_cells=args.phase_side ** 2, equilibrate=lib.equilibrator(solver))
     b = lib.Batch(solver, field)
     b.run(DT)
-    buf = (C.c_uint64 * 16)()
+    buf = (C.c_uint64 * 24)()
     L.fcs_phase_read(gm._h, buf, 1)          # clear
     t0 = time.perf_counter()
     b.run(DT)
@@ -283,7 +284,9 @@ def phases(args):
     print(json.dumps({"workload": f"phase split of k_step<56,Rodas34>, plain Rodas34 on a {args.phase_side}x"
                                   f"{args.phase_side} counterflow field (profiling build)",
                       "attempts": att, "wall_s": wall, "cycles_per_attempt": tot,
-                      "phases": {n: {"cycles": c, "share": c / tot} for n, c in zip(PHASES, cyc)}}), flush=True)
+                      "phases": {n: {"cycles": c, "share": c / tot} for n, c in zip(PHASES, cyc)},
+                      "lu_parts": {n: {"cycles": buf[15 + i] / att, "share": buf[15 + i] / att / tot}
+                                   for i, n in enumerate(LU_PARTS)}}), flush=True)
 
 
 def _timed_run(batch):

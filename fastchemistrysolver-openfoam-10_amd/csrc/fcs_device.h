@@ -139,6 +139,10 @@ __device__ __forceinline__ int pk_species(unsigned long long w, int q) { return 
 __device__ __forceinline__ int pk_nl(unsigned long long w) { return (int)((w >> 48) & 255u); }
 __device__ __forceinline__ int pk_nr(unsigned long long w) { return (int)(w >> 56); }
 
+// LDS doubles at the end of a cell's LDS that hold a copy of its DevMech
+// (see stage_mech in fcs_kernels.h).
+constexpr int kMechLdsDoubles = (int)((sizeof(DevMech) + 7) / 8);
+
 struct DevOpts {
     int solver, jac, maxSteps;
     double absTol, relTol, Treact, dtChemMax;
@@ -207,7 +211,8 @@ enum {
     PH_FORM, PH_LU, PH_SOLVE,                                            // iteration matrix, LU, stage solves
     PH_RHS_SETUP, PH_RHS_TB, PH_RHS_RATES, PH_RHS_PROD, PH_RHS_REST,    // derivatives()
     PH_STAGE, PH_ERR,                                                    // stage sums, error norm
-    PH_SLOTS = 16, PH_SPREAD = 256
+    PH_LU_PANEL, PH_LU_UROWS, PH_LU_TRAIL,                               // parts of PH_LU
+    PH_SLOTS = 24, PH_SPREAD = 256
 };
 #ifdef FCS_PHASE_TIMING
 struct PhaseClock {
@@ -265,8 +270,9 @@ __host__ __device__ inline size_t lu_scratch_doubles(int NP)
 
 __host__ __device__ inline int group_of(int NP) { return NP > 64 ? (NP <= 128 ? 128 : 256) : 64; }
 
-// LDS doubles per cell.  Wavefront form: four wavefronts of the 53-species
-// mechanism fit the 160 KiB of a CU (40,112 B each).  Workgroup form: the
+// LDS doubles per cell, the staged DevMech last.  Wavefront form: four
+// wavefronts of the 53-species mechanism fit the 160 KiB of a CU (40,472 B
+// each).  Workgroup form: the
 // LU scratch reuses the reaction arrays; the matrix (and its reciprocal
 // pivots) follows them in LDS up to kJLdsMaxNP rows, else it is in HBM.
 __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
@@ -274,14 +280,15 @@ __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
     const size_t nrP = round_up(nr, 2);
     if (NP <= 64) {
         const size_t kq = 2 * nrP > 128 ? 2 * nrP : 128;   // rk + rdT, later H + Q
-        return 3 * 64 + 2 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32;
+        return 3 * 64 + 2 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32 +
+               kMechLdsDoubles;
     }
     const size_t G = group_of(NP);
     const size_t kq = 2 * nrP > 2 * G ? 2 * nrP : 2 * G;
     const size_t react = nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2);
     const size_t lu = lu_scratch_doubles(NP);
     const size_t rest = 256 + 2 * G + (react > lu ? react : lu) + G / 2 + 16;
-    return jlds_of(NP) ? ((rest + 1) & ~(size_t)1) + (size_t)NP * NP + NP : rest;
+    return (jlds_of(NP) ? ((rest + 1) & ~(size_t)1) + (size_t)NP * NP + NP : rest) + kMechLdsDoubles;
 }
 
 __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jglobal = nullptr)
@@ -995,35 +1002,43 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     // third-body / falloff partials: element (i, j) gains sum_r nu_ir w_r
     // eff_rj over the third-body/falloff entries of species i.  Column-wise
     // over a flat entry list sorted by row (uniform loop, 16 entries per
-    // batch, coalesced efficiency rows): lane j forms row i's sum and adds it
-    // once; the row's old value is read when the row starts, so the read
-    // overlaps the row's sum.
+    // batch, coalesced efficiency rows): lane j forms row i's sum and writes
+    // old + sum once when the row ends.  A batch's entries are fetched with
+    // one per-lane load (entry k0 + (lane mod 16)) and broadcast with
+    // readlane, so its efficiency rows, w_t and the current value of each
+    // entry's row are all requested before any of them is used; a row first
+    // seen in this batch still holds its sparse-partial value (rows are
+    // visited once, in ascending order), and the value read for a row
+    // continuing from the last batch is not used.
     if (M.nTBz > 0) {
-        double acc = 0.0;
-        int cur = M.tbz_row[0];
-        double old = sp ? L.J[lane * JSt + cur] : 0.0;
-        for (int k0 = 0; k0 < M.nTBz; k0 += 16) {
+        double acc = 0.0, oldv = 0.0;
+        int cur = -1;
+        for (int k0 = 0; k0 < M.nTBz; k0 += 16) {   // nTBz % 16 == 0
+            const int e = k0 + (lane & 15);
+            const int vt = M.tbz_tb[e], vrow = M.tbz_row[e];
+            const double vnu = M.tbz_nu[e];
             int rw_[16];
-            double cf[16], ef[16];
+            double cf[16], ef[16], ov[16];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {   // unconditional: nTBz % 16 == 0
-                const int t = M.tbz_tb[k0 + u];
-                rw_[u] = M.tbz_row[k0 + u];
-                cf[u] = M.tbz_nu[k0 + u] * L.rw[t];
+            for (int u = 0; u < 16; ++u) {
+                const int t = readlane_i(vt, u);
+                rw_[u] = readlane_i(vrow, u);
+                cf[u] = readlane_d(vnu, u) * L.rw[t];
                 ef[u] = M.eff[t * SW + lane];
+                ov[u] = sp ? L.J[lane * JSt + rw_[u]] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 if (rw_[u] != cur) {
-                    if (sp) L.J[lane * JSt + cur] = old + acc;
+                    if (sp && cur >= 0) L.J[lane * JSt + cur] = oldv + acc;
                     acc = 0.0;
                     cur = rw_[u];
-                    old = sp ? L.J[lane * JSt + cur] : 0.0;
+                    oldv = ov[u];
                 }
                 acc += cf[u] * ef[u];
             }
         }
-        if (sp) L.J[lane * JSt + cur] = old + acc;
+        if (sp) L.J[lane * JSt + cur] = oldv + acc;
     }
     wave_sync();
     clk.mark(M, lane == 0, PH_JAC_TB);
@@ -1051,7 +1066,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             for (int u = 0; u < 8; ++u)
                 if (j0 + u < ns) {
                     v[u] = Jrow[(j0 + u) * JSt];
-                    iw[u] = M.invW[j0 + u];
+                    iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : readlane_d(S.invW, j0 + u);   // lane j holds 1/W_j
                 }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
@@ -1099,163 +1114,107 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 
 // ---------------------------------------------------------------- LU
 // Block-4 LU with pivoting restricted to each 4-row diagonal block
-// (src/LUsolver/LUsolver.C:106-246 full blocks, :503-551 / :553-650 tails)
-// on the LDS matrix.  Rows never move: pos is the logical row a lane holds,
-// swapped as the reference swaps rows, and ip[r] is the lane holding
-// logical row r when the factorisation ends.  Right-looking elimination:
-// the pivot row is read by all lanes at one address.
+// (src/LUsolver/LUsolver.C:106-246 full blocks, :503-551 / :553-650 tails).
+// Wavefront form: each lane holds its matrix row in registers for the whole
+// factorisation (loaded from LDS, factors stored back for the solves).
+// Rows never move: pos is the logical row a lane holds, swapped as the
+// reference swaps rows, and ip[r] is the lane holding logical row r when the
+// factorisation ends.  Right-looking elimination; a pivot row's entry is
+// broadcast with readlane.
 struct LuState { double invD; int pos; int ipl; };   // ipl: lane holding logical row `lane`
 
-// |value| of column-k entry in the lane that holds logical row r.
-__device__ __forceinline__ double cand(double ak, int pos, int r)
-{
-    return fabs(readlane_d(ak, first_lane(pos == r)));
-}
-
 template <int NP>
-__device__ inline LuState lu_factor(const Lds& L)
+__device__ inline LuState lu_factor(const DevMech& M, const Lds& L)
 {
     static_assert(NP % 4 == 0 && NP <= 64, "wavefront-form LU: padded size a multiple of 4, at most 64");
     const int lane = lane_id();
+    PhaseClock clk;
     const bool row = lane < NP;
     double* Jrow = L.J + lane;
+    double R[NP];            // this lane's row
+#pragma unroll
+    for (int j = 0; j < NP; ++j) R[j] = Jrow[j * JS];
     int pos = lane;          // lanes >= NP hold no row
     double invD = 0.0;
-    for (int k0 = 0; k0 < NP; k0 += 4) {
-        // ---- panel: columns k0..k0+3 of every row, in registers.  Each
-        // column takes the pivot choice, swap, eps replacement and
-        // elimination of the reference, restricted to the panel columns.
-        double pa[4];
 #pragma unroll
-        for (int d = 0; d < 4; ++d) pa[d] = Jrow[(k0 + d) * JS];
-        int pl[4];           // lane holding logical row k0+d
+    for (int k0 = 0; k0 < NP; k0 += 4) {
+        // ---- panel: columns k0..k0+3.  Each column takes the pivot choice,
+        // swap, eps replacement and elimination of the reference, restricted
+        // to the panel columns.
+        // Swaps only ever exchange rows within a block, so at the block's
+        // start lane k0+q holds logical row k0+q; lr tracks the lane of each
+        // of the block's logical rows (uniform) through the block's swaps.
+        int lr[4] = {k0, k0 + 1, k0 + 2, k0 + 3};
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             const int k = k0 + d;
-            const double ak = pa[d];
-            int pick = k;
+            const double ak = R[k];
+            double v[4];     // |column-k entry| of logical rows k0+q, q >= d
+#pragma unroll
+            for (int q = d; q < 4; ++q) v[q] = fabs(readlane_d(ak, lr[q]));
+            int im = d;
             if (d == 0) {
                 double temp = 0.0;
-                int im = 0;
+                im = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double v = cand(ak, pos, k0 + q);
-                    if (temp < v) { im = q; temp = v; }
-                }
-                pick = k0 + im;
+                for (int q = 0; q < 4; ++q)
+                    if (temp < v[q]) { im = q; temp = v[q]; }
             } else if (d == 1) {
-                double t = cand(ak, pos, k0 + 1);
-                int im = 1;
-                const double v2 = cand(ak, pos, k0 + 2);
-                if (t < v2) {
+                double t = v[1];
+                if (t < v[2]) {
                     im = 2;
-                    t = readlane_d(ak, first_lane(pos == k0 + 2));   // signed, LUsolver.C:177
+                    t = readlane_d(ak, lr[2]);   // signed, LUsolver.C:177
                 }
-                const double v3 = cand(ak, pos, k0 + 3);
-                if (t < v3) im = 3;
-                pick = k0 + im;
+                if (t < v[3]) im = 3;
             } else if (d == 2) {
-                const double v2 = cand(ak, pos, k0 + 2);
-                const double v3 = cand(ak, pos, k0 + 3);
-                if (v2 < v3) pick = k0 + 3;
+                if (v[2] < v[3]) im = 3;
             }
+            const int pick = k0 + im;
             if (pick != k) {
                 if (pos == k) pos = pick;
                 else if (pos == pick) pos = k;
+#pragma unroll
+                for (int q = d + 1; q < 4; ++q)
+                    if (q == im) { const int t = lr[d]; lr[d] = lr[q]; lr[q] = t; }
             }
-            const int p = first_lane(pos == k);
+            const int p = lr[d];
             double piv = readlane_d(ak, p);
             if (piv == 0.0) {                        // LUsolver.C:155
                 piv = PIVOT_EPS;
-                if (lane == p) pa[d] = PIVOT_EPS;
+                if (lane == p) R[k] = PIVOT_EPS;
             }
             const double inv = 1.0 / piv;
             if (lane == p) invD = inv;
             const bool elim = row && pos > k;
             const double l = ak * inv;
-            if (elim) pa[d] = l;
+            if (elim) R[k] = l;
 #pragma unroll
             for (int e = d + 1; e < 4; ++e) {
-                const double pe = readlane_d(pa[e], p);
-                if (elim) pa[e] = pa[e] - l * pe;
-            }
-            pl[d] = p;
-        }
-#pragma unroll
-        for (int d = 0; d < 4; ++d) Jrow[(k0 + d) * JS] = pa[d];
-        // ---- U rows of the block, column-wise: lane j forms the pivot rows'
-        // entries of trailing column j with the operations (and order) the
-        // column-by-column elimination applies to them, and stores them.
-        const double l10 = readlane_d(pa[0], pl[1]);
-        const double l20 = readlane_d(pa[0], pl[2]), l21 = readlane_d(pa[1], pl[2]);
-        const double l30 = readlane_d(pa[0], pl[3]), l31 = readlane_d(pa[1], pl[3]),
-                     l32 = readlane_d(pa[2], pl[3]);
-        if (lane >= k0 + 4 && lane < NP) {
-            const double* cj = L.J + lane * JS;
-            const double u0 = cj[pl[0]], a1 = cj[pl[1]], a2 = cj[pl[2]], a3 = cj[pl[3]];
-            const double u1 = a1 - l10 * u0;
-            double u2 = a2 - l20 * u0;
-            u2 = u2 - l21 * u1;
-            double u3 = a3 - l30 * u0;
-            u3 = u3 - l31 * u1;
-            u3 = u3 - l32 * u2;
-            L.J[lane * JS + pl[1]] = u1;
-            L.J[lane * JS + pl[2]] = u2;
-            L.J[lane * JS + pl[3]] = u3;
-        }
-        wave_sync();
-        // ---- trailing update of the rows below the block: a_ij -= sum_q
-        // l_iq u_qj in the elimination's order.  The other rows take zero
-        // multipliers, which leaves them unchanged exactly.
-        const bool below = row && pos > k0 + 3;
-        const double m0 = below ? pa[0] : 0.0, m1 = below ? pa[1] : 0.0, m2 = below ? pa[2] : 0.0,
-                     m3 = below ? pa[3] : 0.0;
-        // (software pipelined: the next block of 4 columns is loaded before
-        // the current one is updated; blocks are disjoint, so this is safe)
-        double nv[4], n0[4], n1[4], n2[4], n3[4];
-        if (k0 + 4 < NP) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int j = k0 + 4 + u;
-                nv[u] = Jrow[j * JS];
-                n0[u] = L.J[j * JS + pl[0]];
-                n1[u] = L.J[j * JS + pl[1]];
-                n2[u] = L.J[j * JS + pl[2]];
-                n3[u] = L.J[j * JS + pl[3]];
+                const double pe = readlane_d(R[k0 + e], p);
+                if (elim) R[k0 + e] = R[k0 + e] - l * pe;
             }
         }
-        for (int j0 = k0 + 4; j0 < NP; j0 += 4) {
-            double v[4], u0[4], u1[4], u2[4], u3[4];
+        const int pl[4] = {lr[0], lr[1], lr[2], lr[3]};   // lane holding logical row k0+d
+        clk.mark(M, lane == 0, PH_LU_PANEL);
+        // ---- trailing columns: step q subtracts l_iq u_qj from every row
+        // below logical row k0+q, u_qj read from that row after steps 0..q-1
+        // -- the column-by-column elimination's operations in its order (the
+        // pivot rows' own entries become the U rows on the way).  Rows not
+        // below take a zero multiplier, which leaves them unchanged exactly.
+        double m[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                v[u] = nv[u];
-                u0[u] = n0[u];
-                u1[u] = n1[u];
-                u2[u] = n2[u];
-                u3[u] = n3[u];
-            }
-            if (j0 + 4 < NP) {
+        for (int q = 0; q < 4; ++q) m[q] = (row && pos > k0 + q) ? R[k0 + q] : 0.0;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int j = j0 + 4 + u;
-                    nv[u] = Jrow[j * JS];
-                    n0[u] = L.J[j * JS + pl[0]];
-                    n1[u] = L.J[j * JS + pl[1]];
-                    n2[u] = L.J[j * JS + pl[2]];
-                    n3[u] = L.J[j * JS + pl[3]];
-                }
-            }
+        for (int j = k0 + 4; j < NP; ++j) {
+            double a = R[j];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                double w = v[u] - m0 * u0[u];
-                w = w - m1 * u1[u];
-                w = w - m2 * u2[u];
-                w = w - m3 * u3[u];
-                Jrow[(j0 + u) * JS] = w;
-            }
+            for (int q = 0; q < 4; ++q) a = a - m[q] * readlane_d(a, pl[q]);
+            R[j] = a;
         }
-        wave_sync();
+        clk.mark(M, lane == 0, PH_LU_TRAIL);
     }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) Jrow[j * JS] = R[j];
     if (row) L.ip[pos] = lane;
     wave_sync();
     const int ipl = row ? L.ip[lane] : lane;

@@ -91,10 +91,10 @@ __device__ inline void form_matrix(const Lds& L, double diag)
 
 // LU factor / solve of either form.
 template <int NP>
-__device__ inline auto lu_factor_any(const Lds& L)
+__device__ inline auto lu_factor_any(const DevMech& M, const Lds& L)
 {
     if constexpr (Cell<NP>::kLarge) return lu_factor_large<NP>(L);
-    else return lu_factor<NP>(L);
+    else return lu_factor<NP>(M, L);
 }
 template <int NP, class St>
 __device__ inline double lu_solve_any(const Lds& L, const St& st, double b)
@@ -225,7 +225,7 @@ __device__ double rosenbrock_attempt(const DevMech& M, const Lds& L, const DevOp
     PhaseClock clk;
     form_matrix<NP>(L, 1.0 * invdx * tb.invGamma);
     clk.mark(M, first, PH_FORM);
-    const auto st = lu_factor_any<NP>(L);
+    const auto st = lu_factor_any<NP>(M, L);
     clk.mark(M, first, PH_LU);
     double k[6] = {0, 0, 0, 0, 0, 0};
     double dydx = dfdx;   // a stage without its own f before any f evaluation takes f(y0)
@@ -407,7 +407,7 @@ __device__ bool seul(const DevMech& M, const Lds& L, const double* JyG, double p
     const int lane = tid<NP>();
     const double invdx = nSteps / dxTot;
     form_matrix_from<NP>(L, JyG, invdx);
-    const auto st = lu_factor_any<NP>(L);
+    const auto st = lu_factor_any<NP>(M, L);
     double yT = 0.0, dy = 0.0;
     int nn = 0;
     bool thetaPhase = false;

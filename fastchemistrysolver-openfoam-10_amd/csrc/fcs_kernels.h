@@ -45,6 +45,24 @@ struct CellArrays {
     double* Jg;                      // workgroup form: matrix slices [grid][NP*NP + NP]
 };
 
+// Copies the mechanism descriptor from device memory into the last
+// kMechLdsDoubles of the cell's LDS and returns the copy.  The kernels take
+// the descriptor by pointer and read its fields from LDS where they use them:
+// held in SGPRs from the kernel arguments instead (45 pointers and sizes),
+// they outgrow the scalar register file, and the compiler spills them to
+// VGPR lanes and reloads them inside every loop.
+template <int NP>
+__device__ __forceinline__ const DevMech& stage_mech(const DevMech* Mg, double* lds_raw)
+{
+    const int nr = Mg->nr, nTB = Mg->nTB;
+    DevMech* Ms = reinterpret_cast<DevMech*>(lds_raw + lds_doubles(NP, nr, nTB) - kMechLdsDoubles);
+    const unsigned int* src = reinterpret_cast<const unsigned int*>(Mg);
+    unsigned int* dst = reinterpret_cast<unsigned int*>(Ms);
+    for (int w = tid<NP>(); w < (int)(sizeof(DevMech) / 4); w += Cell<NP>::G) dst[w] = src[w];
+    __syncthreads();
+    return *Ms;
+}
+
 // HBM matrix slice of this workgroup (workgroup form with the matrix in
 // HBM), else null.
 template <int NP>
@@ -89,11 +107,12 @@ __device__ void finish_cell(const DevMech& M, const DevOpts& o, const Lds& L, co
 // counter *countZero (the list after next) is cleared for the next launch.
 // Every loop here is bounded by the list length read at entry.
 template <int NP, int SOLVER>
-__global__ __launch_bounds__(Cell<NP>::G) void k_step(DevMech M, DevOpts o, CellArrays a, const int* listIn,
+__global__ __launch_bounds__(Cell<NP>::G) void k_step(const DevMech* Mg, DevOpts o, CellArrays a, const int* listIn,
                                                       const unsigned int* countIn, int* listOut,
                                                       unsigned int* countOut, unsigned int* countZero)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
     const int lane = tid<NP>();
     const int ns = M.ns;
@@ -134,12 +153,13 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step(DevMech M, DevOpts o, Cell
 // list (same list protocol as k_step).  The per-cell Jacobian lives in
 // a.Jy between launches.
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(DevMech M, DevOpts o, CellArrays a,
+__global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(const DevMech* Mg, DevOpts o, CellArrays a,
                                                              const int* listIn, const unsigned int* countIn,
                                                              int* listOut, unsigned int* countOut,
                                                              unsigned int* countZero)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
     const int lane = tid<NP>();
     const int ns = M.ns;
@@ -203,12 +223,13 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(DevMech M, DevOpts 
 // network -- or, with seed != 0, starting from min(h, deltaT) instead of
 // deltaTChem.
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_verify(DevMech M, DevOpts o, CellArrays a, const int* listIn,
+__global__ __launch_bounds__(Cell<NP>::G) void k_verify(const DevMech* Mg, DevOpts o, CellArrays a, const int* listIn,
                                                         const unsigned int* countIn, const float* P, int seed,
                                                         int* listOut, unsigned int* countOut,
                                                         unsigned long long* nAccepted)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
     const int lane = tid<NP>();
     const int ns = M.ns;
@@ -253,10 +274,11 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_verify(DevMech M, DevOpts o, Ce
 // One wavefront per state: derivatives.  External layout [n][nS] with T at
 // row ns.
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_derivatives(DevMech M, int nS, const double* p, double* YT,
+__global__ __launch_bounds__(Cell<NP>::G) void k_derivatives(const DevMech* Mg, int nS, const double* p, double* YT,
                                                              double* out)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
@@ -269,10 +291,11 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_derivatives(DevMech M, int nS, 
 }
 
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_jacobian(DevMech M, int jacType, int nS, const double* p,
+__global__ __launch_bounds__(Cell<NP>::G) void k_jacobian(const DevMech* Mg, int jacType, int nS, const double* p,
                                                           double* YT, double* f, double* J, double* Jg)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
@@ -295,11 +318,12 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_jacobian(DevMech M, int jacType
 // OptRosenbrock34.C:281-677).  External layout [n][nS], T at row ns; the
 // start state is clipped in place as the Jacobian evaluation does.
 template <int NP, int SOLVER>
-__global__ __launch_bounds__(Cell<NP>::G) void k_attempt(DevMech M, DevOpts o, int nS, const double* p,
+__global__ __launch_bounds__(Cell<NP>::G) void k_attempt(const DevMech* Mg, DevOpts o, int nS, const double* p,
                                                          const double* dx, double* YT, double* YTnew, double* err,
                                                          double* Jg)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
@@ -315,10 +339,11 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_attempt(DevMech M, DevOpts o, i
 }
 
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_lu_solve(DevMech M, int nSys, const double* A, double* b,
+__global__ __launch_bounds__(Cell<NP>::G) void k_lu_solve(const DevMech* Mg, int nSys, const double* A, double* b,
                                                           double* Jg)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
@@ -329,7 +354,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_lu_solve(DevMech M, int nSys, c
                 (ext >= 0 && ej >= 0) ? A[((size_t)s * n + ext) * n + ej] : ((lane == j && lane < NP) ? 1.0 : 0.0);
         }
     wave_sync();
-    const auto st = lu_factor_any<NP>(L);
+    const auto st = lu_factor_any<NP>(M, L);
     const double x = lu_solve_any<NP>(L, st, ext >= 0 ? b[(size_t)s * n + ext] : 0.0);
     if (ext >= 0) b[(size_t)s * n + ext] = x;
 }
@@ -363,10 +388,11 @@ __device__ inline double cell_rate_state(const DevMech& M, const Lds& L, int nc,
 // 700-740): RR_i = W_i dNdtByV_i(p, T, c), one wavefront per cell, fields
 // [ns][nc].
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_calculate(DevMech M, int nc, const double* rho, const double* T,
+__global__ __launch_bounds__(Cell<NP>::G) void k_calculate(const DevMech* Mg, int nc, const double* rho, const double* T,
                                                            const double* p, const double* Y, double* RR)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
     const int c = blockIdx.x, lane = tid<NP>();
     double Tcl, lnT, iT;
@@ -380,10 +406,11 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_calculate(DevMech M, int nc, co
 // with OptReaction::Tc (src/Reaction/Tc.C:4-834): tc = sumW /
 // sumWRateByCTot * sum(c), or vGreat when no reaction proceeds.
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_tc(DevMech M, int nc, const double* rho, const double* T,
+__global__ __launch_bounds__(Cell<NP>::G) void k_tc(const DevMech* Mg, int nc, const double* rho, const double* T,
                                                     const double* p, const double* Y, double* tc)
 {
     extern __shared__ double lds_raw[];
+    const DevMech& M = stage_mech<NP>(Mg, lds_raw);
     const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
     const int c = blockIdx.x;
     double Tcl, lnT, iT;

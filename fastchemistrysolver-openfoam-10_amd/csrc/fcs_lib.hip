@@ -167,6 +167,7 @@ int with_device(size_t bytes, F&& f)
 
 struct fcs_mech {
     DevMech dm;
+    DevMech* dmDev = nullptr;   // device copy of dm, read by the kernels
     void* dbuf = nullptr;
     int ns = 0, nr = 0;
     std::vector<int> perm;   // device order -> file order
@@ -644,6 +645,11 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.tbz_nu = (const double*)P(iTbzNu);
     dm.rpack = (const unsigned long long*)P(iRpack);
     dm.sp_pack = (const unsigned long long*)P(iSpPack);
+    if (hipMalloc((void**)&m->dmDev, sizeof(DevMech)) != hipSuccess ||
+        hipMemcpy(m->dmDev, &dm, sizeof(DevMech), hipMemcpyHostToDevice) != hipSuccess) {
+        fcs_mech_destroy(m);
+        return FCS_E_NOMEM;
+    }
     m->ns = ns;
     m->nr = nr;
     m->perm = perm;
@@ -655,6 +661,7 @@ int fcs_mech_destroy(fcs_mech* m)
 {
     if (!m) return FCS_E_ARG;
     if (m->dbuf) (void)hipFree(m->dbuf);
+    if (m->dmDev) (void)hipFree(m->dmDev);
 #ifdef FCS_PHASE_TIMING
     if (m->dm.phase) (void)hipFree(m->dm.phase);
 #endif
@@ -768,7 +775,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
 {
     hipStream_t st = s->stream;
     CellArrays a = d.view(nc, dt, lts);
-    DevMech M = s->mech->dm;
+    const DevMech* M = s->mech->dmDev;   // kernels stage it in LDS
     DevOpts o = dev_opts(s->opts);
     int ns = s->mech->ns;
     {
@@ -1312,7 +1319,7 @@ int fcs_solve_noalloc(fcs_solver* s, int64_t nCells, double deltaT, const double
     a.dtcOut = deltaTChemOut; a.RR = RR; a.Tend = Tend; a.status = cellStatus;
     a.Jg = nullptr;
     hipStream_t st = s->stream;
-    DevMech M = s->mech->dm;
+    const DevMech* M = s->mech->dmDev;   // kernels stage it in LDS
     DevOpts o = dev_opts(s->opts);
     int nsp = ns;
     {
@@ -1391,7 +1398,7 @@ int fcs_eval_derivatives(fcs_solver* s, int64_t nS, const double* p, double* YT,
         double* dd = dy + nv;
         HIP_OK(hipMemcpy(dp, p, nS * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(dy, YT, nv * 8, hipMemcpyHostToDevice));
-        DevMech M = s->mech->dm;
+        const DevMech* M = s->mech->dmDev;   // kernels stage it in LDS
         int ns32 = (int)nS;
         void* args[] = {&M, &ns32, &dp, &dy, &dd};
         HIP_OK(hipLaunchKernel(s->ks->deriv, dim3((unsigned)nS), dim3(s->ks->G), args, s->ldsBytes, s->stream));
@@ -1415,7 +1422,7 @@ int fcs_eval_jacobian(fcs_solver* s, int64_t nS, const double* p, double* YT, do
         double* dG = ng ? dJ + nm : nullptr;
         HIP_OK(hipMemcpy(dp, p, nS * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(dy, YT, nv * 8, hipMemcpyHostToDevice));
-        DevMech M = s->mech->dm;
+        const DevMech* M = s->mech->dmDev;   // kernels stage it in LDS
         int jt = s->opts.jacobian, ns32 = (int)nS;
         void* args[] = {&M, &jt, &ns32, &dp, &dy, &df, &dJ, &dG};
         HIP_OK(hipLaunchKernel(s->ks->jac, dim3((unsigned)nS), dim3(s->ks->G), args, s->ldsBytes, s->stream));
@@ -1444,7 +1451,7 @@ int fcs_eval_attempt(fcs_solver* s, int64_t nS, const double* p, const double* d
         HIP_OK(hipMemcpy(dp, p, nS * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(ddx, dx, nS * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(dy, YT, nv * 8, hipMemcpyHostToDevice));
-        DevMech M = s->mech->dm;
+        const DevMech* M = s->mech->dmDev;   // kernels stage it in LDS
         DevOpts o = dev_opts(s->opts);
         int ns32 = (int)nS;
         void* args[] = {&M, &o, &ns32, &dp, &ddx, &dy, &dn, &de, &dG};
@@ -1483,7 +1490,7 @@ static int field_kernel(fcs_solver* s, const void* kern, int64_t nCells, const d
         HIP_OK(hipMemcpyAsync(dp, p, nc * 8, hipMemcpyHostToDevice, st));
         for (int i = 0; i < ns; ++i)
             HIP_OK(hipMemcpyAsync(dY + (size_t)i * nc, Y[i], nc * 8, hipMemcpyHostToDevice, st));
-        DevMech M = s->mech->dm;
+        const DevMech* M = s->mech->dmDev;   // kernels stage it in LDS
         int nc32 = (int)nc;
         void* args[] = {&M, &nc32, &dRho, &dT, &dp, &dY, &dOut};
         HIP_OK(hipLaunchKernel(kern, dim3((unsigned)nc), dim3(s->ks->G), args, s->ldsBytes, st));
@@ -1544,7 +1551,7 @@ int fcs_lu_solve(fcs_solver* s, int64_t nSys, const double* A, double* b)
         double* dG = ng ? db + nv : nullptr;
         HIP_OK(hipMemcpy(dA, A, nm * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(db, b, nv * 8, hipMemcpyHostToDevice));
-        DevMech M = s->mech->dm;
+        const DevMech* M = s->mech->dmDev;   // kernels stage it in LDS
         int n32 = (int)nSys;
         void* args[] = {&M, &n32, &dA, &db, &dG};
         HIP_OK(hipLaunchKernel(s->ks->lu, dim3((unsigned)nSys), dim3(s->ks->G), args, s->ldsBytes, s->stream));
