@@ -650,11 +650,10 @@ __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
                 eb[u] = M.effT[(size_t)(j0 + u) * M.nTBpad + tb];
             }
 #pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (j0 + u < M.ns) {
-                    ma += ea[u] * c[u];
-                    mb += eb[u] * c[u];
-                }
+            for (int u = 0; u < 16; ++u) {   // no species test: padding adds 0 * 0
+                ma += ea[u] * c[u];
+                mb += eb[u] * c[u];
+            }
         }
         if (ina) L.Mtb[ta] = ma;
         if (inb) L.Mtb[tb] = mb;
@@ -782,7 +781,7 @@ __device__ __forceinline__ double production_rate(const DevMech& M, const Lds& L
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int nu = rn_nu(w[u]);
-            if (nu != 0) om += (double)nu * q[u];
+            om += (double)nu * q[u];   // padding entries: nu = 0
         }
     }
     return om;
@@ -945,7 +944,6 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 const int nu = jterm_nu(w[u]);
-                if (nu == 0) continue;
                 const int col = jterm_col(w[u]);
                 if (col != cur) {
                     Jrow[cur * JSt] = acc;
@@ -983,7 +981,6 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int nui = rn_nu(wv[u]);
-                if (nui == 0) continue;
                 const double nu = (double)nui;
                 const int nl = pk_nl(pk[u]), nrh = pk_nr(pk[u]);
                 double CF = 1.0, CR = 1.0;
@@ -1045,37 +1042,35 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     // dC/dY conversion of the species rows, T column
     const double WiByRho = sp ? S.W * invrho : 0.0;
     const double Yd = sp ? om * WiByRho : 0.0;
+    // (every column up to NP without a species test: the padding columns
+    // and C are zero there, and the T column is rewritten after the loop)
     if (sp) {
         double sC = 0.0;
-        for (int j0 = 0; j0 < ns; j0 += 8) {
-            double v[8], c[8];
+        for (int j0 = 0; j0 < NP; j0 += 4) {
+            double v[4], c[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (j0 + u < ns) {
-                    v[u] = Jrow[(j0 + u) * JSt];
-                    c[u] = L.C[j0 + u];
-                }
+            for (int u = 0; u < 4; ++u) {
+                v[u] = Jrow[(j0 + u) * JSt];
+                c[u] = L.C[j0 + u];
+            }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (j0 + u < ns) sC += v[u] * c[u];
+            for (int u = 0; u < 4; ++u) sC += v[u] * c[u];
         }
         const double sCx = jacType == 0 ? sC : 0.0;   // fast: diagonal dC/dY only
-        for (int j0 = 0; j0 < ns; j0 += 8) {
-            double v[8], iw[8];
+        for (int j0 = 0; j0 < NP; j0 += 4) {
+            double v[4], iw[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (j0 + u < ns) {
-                    v[u] = Jrow[(j0 + u) * JSt];
-                    iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : readlane_d(S.invW, j0 + u);   // lane j holds 1/W_j
-                }
+            for (int u = 0; u < 4; ++u) {
+                v[u] = Jrow[(j0 + u) * JSt];
+                iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : readlane_d(S.invW, j0 + u);   // lane j holds 1/W_j
+            }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (j0 + u < ns) {
-                    const double iWj = iw[u];
-                    Jrow[(j0 + u) * JSt] = jacType == 0
-                        ? WiByRho * (v[u] * (rho * iWj) - (MW * iWj) * sCx) + (MW * iWj) * Yd
-                        : WiByRho * (v[u] * (rho * iWj)) + (MW * iWj) * Yd;
-                }
+            for (int u = 0; u < 4; ++u) {
+                const double iWj = iw[u];
+                Jrow[(j0 + u) * JSt] = jacType == 0
+                    ? WiByRho * (v[u] * (rho * iWj) - (MW * iWj) * sCx) + (MW * iWj) * Yd
+                    : WiByRho * (v[u] * (rho * iWj)) + (MW * iWj) * Yd;
+            }
         }
         Jrow[(NP - 1) * JSt] = WiByRho * (jT - sC * iT) + iT * Yd;
     }
@@ -1089,19 +1084,17 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     if (lane < NP) {
         const bool tcol = lane == NP - 1;
         double colsum = 0.0;
-        for (int q0 = 0; q0 < ns; q0 += 8) {
-            double v[8], h[8], qq[8];
+        for (int q0 = 0; q0 < NP; q0 += 4) {   // every row: H and Q are zero beyond ns
+            double v[4], h[4], qq[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (q0 + u < ns) {
-                    const int i = q0 + u;
-                    v[u] = L.J[lane * JSt + i];
-                    h[u] = L.H[i];
-                    qq[u] = tcol ? L.Q[i] : 0.0;
-                }
+            for (int u = 0; u < 4; ++u) {
+                const int i = q0 + u;
+                v[u] = L.J[lane * JSt + i];
+                h[u] = L.H[i];
+                qq[u] = tcol ? L.Q[i] : 0.0;
+            }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (q0 + u < ns) colsum += v[u] * h[u] + qq[u];
+            for (int u = 0; u < 4; ++u) colsum += v[u] * h[u] + qq[u];
         }
         const double rowT = !tcol ? (sp ? (-colsum - th.cp * dTdt) / cpM : 0.0)
                                   : (-colsum - dTdt * dcpMdT) / cpM;
