@@ -271,7 +271,7 @@ __host__ __device__ inline size_t lu_scratch_doubles(int NP)
 __host__ __device__ inline int group_of(int NP) { return NP > 64 ? (NP <= 128 ? 128 : 256) : 64; }
 
 // LDS doubles per cell, the staged DevMech last.  Wavefront form: four
-// wavefronts of the 53-species mechanism fit the 160 KiB of a CU (40,472 B
+// wavefronts of the 53-species mechanism fit the 160 KiB of a CU (40,504 B
 // each).  Workgroup form: the
 // LU scratch reuses the reaction arrays; the matrix (and its reciprocal
 // pivots) follows them in LDS up to kJLdsMaxNP rows, else it is in HBM.
@@ -280,7 +280,7 @@ __host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
     const size_t nrP = round_up(nr, 2);
     if (NP <= 64) {
         const size_t kq = 2 * nrP > 128 ? 2 * nrP : 128;   // rk + rdT, later H + Q
-        return 3 * 64 + 2 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32 +
+        return 3 * 66 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32 +
                kMechLdsDoubles;
     }
     const size_t G = group_of(NP);
@@ -320,9 +320,9 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jg
     const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
     const int kq = 2 * nrP > 128 ? 2 * nrP : 128;
     L.C = base;              // C[64] = 1.0: the "no other species" slot of the Jacobian terms
-    L.expG = base + 66;
-    L.dBdT = base + 130;
-    L.rq = base + 194;
+    L.expG = base + 66;      // expG[64] = 1.0 and dBdT[64] = 0: the identity slot unused
+    L.dBdT = base + 132;     // reaction slots name (wavefront form)
+    L.rq = base + 198;
     L.rk = L.rq + nrP;
     L.rdT = L.rk + nrP;
     L.H = L.rk;
@@ -401,6 +401,15 @@ __device__ __forceinline__ double gbcast(double v, int src, const Lds& L)
         __syncthreads();
         return L.red[8];
     }
+}
+
+// The wavefront form's identity slot 64 (C 1, exp(-G/RT) 1, d/dT 0), which
+// the unused reaction slots name.
+__device__ __forceinline__ void set_identity_slot(const Lds& L)
+{
+    L.C[64] = 1.0;
+    L.expG[64] = 1.0;
+    L.dBdT[64] = 0.0;
 }
 
 // ---------------------------------------------------------------- thermo
@@ -666,11 +675,12 @@ __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 // Kc in concentration units follows src/Reaction/ReactionFunctionI.H:1-52
 // and ChemicalEquilibriumFunctionI.H:14-25, d ln Kc / dT
 // src/Reaction/JacobianFunctionI.H:1150-1158.  The species of all six slots
-// are gathered unconditionally (unused slots name species 0) and the
-// products take the used slots in slot order.
+// are gathered unconditionally (unused slots name the identity slot 64 in
+// the wavefront form, species 0 in the workgroup form) and the products take
+// the used slots in slot order.
 // One reaction of a round: rates from the preloaded record D and the
 // gathered slot values c (concentrations), g (exp(-G/RT)), db (d(-G/RT)/dT).
-template <bool JAC>
+template <bool JAC, bool ONES>
 __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, const RxnData& D, int r,
                                               const double* c, const double* g, const double* db, double T,
                                               double lnT, double iT, double lnp, double PbyRT)
@@ -692,16 +702,27 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
         dkrdM = tb ? kra : 0.0;
     } else if (D.rev) {
         double num = 1.0, den = 1.0, sdb = 0.0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            if (a < nrh) { num *= g[3 + a]; if (JAC) sdb += db[3 + a]; }
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            if (a < nl) { den *= g[a]; if (JAC) sdb -= db[a]; }
         const int dn = nrh - nl;
         double pw = 1.0;
-        if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
-        if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
+        if constexpr (ONES) {
+            // unused slots read the identity slot (1, 1, 0): the products and
+            // sums below equal the used-slots-only ones exactly, and the
+            // Kc-reversible form has at most 3 slots a side (|dn| <= 2)
+            num = (g[3] * g[4]) * g[5];
+            den = (g[0] * g[1]) * g[2];
+            if (JAC) sdb = ((((db[3] + db[4]) + db[5]) - db[0]) - db[1]) - db[2];
+            const double P2 = PbyRT * PbyRT, M1 = 1.0 / PbyRT, M2 = M1 / PbyRT;
+            pw = dn == 1 ? PbyRT : dn == 2 ? P2 : dn == -1 ? M1 : dn == -2 ? M2 : 1.0;
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (a < nrh) { num *= g[3 + a]; if (JAC) sdb += db[3 + a]; }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (a < nl) { den *= g[a]; if (JAC) sdb -= db[a]; }
+            if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
+            if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
+        }
         double Kc = (num / den) * pw;
         const bool floored = !(Kc > KC_FLOOR);
         Kc = floored ? KC_FLOOR : Kc;
@@ -711,10 +732,15 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
         dkrdM = k.dkfdM * invKc;
     }
     double CF = 1.0, CR = 1.0;
+    if constexpr (ONES) {
+        CF = (c[0] * c[1]) * c[2];
+        CR = (c[3] * c[4]) * c[5];
+    } else {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (a < nl) CF *= c[a];
-        if (a < nrh) CR *= c[3 + a];
+        for (int a = 0; a < 3; ++a) {
+            if (a < nl) CF *= c[a];
+            if (a < nrh) CR *= c[3 + a];
+        }
     }
     if (!JAC) {
         L.rq[r] = k.kf * CF - kr * CR;
@@ -734,8 +760,9 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
 // Kc in concentration units follows src/Reaction/ReactionFunctionI.H:1-52
 // and ChemicalEquilibriumFunctionI.H:14-25, d ln Kc / dT
 // src/Reaction/JacobianFunctionI.H:1150-1158.  The species of all six slots
-// are gathered unconditionally (unused slots name species 0) and the
-// products take the used slots in slot order.
+// are gathered unconditionally (unused slots name the identity slot 64 in
+// the wavefront form, species 0 in the workgroup form) and the products take
+// the used slots in slot order.
 template <bool JAC, int NP>
 __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, double lnT, double iT,
                                       double lnp)
@@ -758,8 +785,8 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
             g1[q] = L.expG[s1];
             db1[q] = JAC ? L.dBdT[s1] : 0.0;
         }
-        reaction_eval<JAC>(M, L, D0, r0, c0, g0, db0, T, lnT, iT, lnp, PbyRT);
-        if (has1) reaction_eval<JAC>(M, L, D1, r1, c1, g1, db1, T, lnT, iT, lnp, PbyRT);
+        reaction_eval<JAC, !Cell<NP>::kLarge>(M, L, D0, r0, c0, g0, db0, T, lnT, iT, lnp, PbyRT);
+        if (has1) reaction_eval<JAC, !Cell<NP>::kLarge>(M, L, D1, r1, c1, g1, db1, T, lnT, iT, lnp, PbyRT);
     }
 }
 
@@ -851,6 +878,7 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
     if (sp) th = species_thermo(M, S, Tc, lnT, iT);
     L.C[lane] = sp ? rho * invW * y : 0.0;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
+    if (!Cell<NP>::kLarge && lane == 0) set_identity_slot(L);
     wave_sync();
     clk.mark(M, lane == 0, PH_RHS_SETUP);
     third_body_sums<NP>(M, L);
@@ -904,6 +932,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     const double Cl = sp ? rho * y * invW : 0.0;
     L.C[lane] = lane == ONE ? 1.0 : Cl;
     if (ONE >= G && lane == 0) L.C[ONE] = 1.0;
+    if (!Cell<NP>::kLarge && lane == 0) set_identity_slot(L);
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
     L.dBdT[lane] = th.dBdT;
     double* Jrow = L.J + lane;   // element j of this thread's row at Jrow[j*JSt]
@@ -943,17 +972,16 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             }
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
+                // running sum of the current column, stored after every term
+                // (the column's last store holds its sum): no branch
                 const int nu = jterm_nu(w[u]);
                 const int col = jterm_col(w[u]);
-                if (col != cur) {
-                    Jrow[cur * JSt] = acc;
-                    acc = 0.0;
-                    cur = col;
-                }
-                acc += (double)nu * ((k[u] * c1[u]) * c2[u]);
+                const double t = (double)nu * ((k[u] * c1[u]) * c2[u]);
+                acc = col == cur ? acc + t : 0.0 + t;
+                cur = col;
+                Jrow[col * JSt] = acc;
             }
         }
-        Jrow[cur * JSt] = acc;
     }
     clk.mark(M, lane == 0, PH_JAC_TERMS);
     // production rate and d(omega)/dT of this lane's species (its reaction
@@ -984,10 +1012,15 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
                 const double nu = (double)nui;
                 const int nl = pk_nl(pk[u]), nrh = pk_nr(pk[u]);
                 double CF = 1.0, CR = 1.0;
+                if constexpr (!Cell<NP>::kLarge) {   // unused slots: the identity slot
+                    CF = (c[u][0] * c[u][1]) * c[u][2];
+                    CR = (c[u][3] * c[u][4]) * c[u][5];
+                } else {
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    if (q < nl) CF *= c[u][q];
-                    if (q < nrh) CR *= c[u][3 + q];
+                    for (int q = 0; q < 3; ++q) {
+                        if (q < nl) CF *= c[u][q];
+                        if (q < nrh) CR *= c[u][3 + q];
+                    }
                 }
                 om += nu * (kf[u] * CF - kr[u] * CR);
                 jT += nu * rd[u];

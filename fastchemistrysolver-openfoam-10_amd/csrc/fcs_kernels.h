@@ -378,6 +378,7 @@ __device__ inline double cell_rate_state(const DevMech& M, const Lds& L, int nc,
     if (sp) th = species_thermo(M, S, Tcl, lnT, iT);
     L.C[lane] = Cl;
     L.expG[lane] = sp ? exp(th.negG) : 0.0;
+    if (!Cell<NP>::kLarge && lane == 0) set_identity_slot(L);
     wave_sync();
     third_body_sums<NP>(M, L);
     wave_sync();

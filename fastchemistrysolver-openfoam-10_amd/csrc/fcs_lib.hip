@@ -397,6 +397,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         const double Ts = 298.15;
         Hf[s] = (RU / d->W[s]) * (((((a[4] / 5 * Ts + a[3] / 4) * Ts + a[2] / 3) * Ts + a[1] / 2) * Ts + a[0]) * Ts + a[5]);
     }
+    std::vector<char> slotUnused(6 * (size_t)std::max(nr, 1), 1);
     std::vector<int> rtype(nr), rrev(nr), rnl(nr), rnr(nr), rsp(6 * std::max(nr, 1), 0), rtb(nr, -1), raux(nr, -1);
     std::vector<double> rA(nr), rbeta(nr), rTa(nr), rA0(nr), rbeta0(nr), rTa0(nr);
     std::vector<double> troe, eff, plogLnp, plogLnA, plogBeta, plogTa, rtroe(5 * (size_t)std::max(nr, 1), 0.0);
@@ -418,6 +419,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         for (int q = 0; q < 3; ++q) {
             rsp[q * nr + r] = q < rnl[r] ? d->rxn_lhs[(size_t)ss * f + q] : 0;
             rsp[(3 + q) * nr + r] = q < rnr[r] ? d->rxn_rhs[(size_t)ss * f + q] : 0;
+            slotUnused[q * nr + r] = q >= rnl[r];
+            slotUnused[(3 + q) * nr + r] = q >= rnr[r];
         }
         rA[r] = d->A[f];
         rbeta[r] = d->beta[f];
@@ -489,7 +492,11 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     std::vector<unsigned long long> rpack(std::max(nr, 1), 0ull);
     for (int r = 0; r < nr; ++r) {
         unsigned long long w = 0;
-        for (int q = 0; q < 6; ++q) w |= (unsigned long long)(rsp[q * nr + r] & 255) << (8 * q);
+        for (int q = 0; q < 6; ++q) {
+            // the wavefront form names its identity slot (64) in unused slots
+            const int sq = (kset->NP <= 64 && slotUnused[q * nr + r]) ? 64 : rsp[q * nr + r];
+            w |= (unsigned long long)(sq & 255) << (8 * q);
+        }
         w |= (unsigned long long)rnl[r] << 48;
         w |= (unsigned long long)rnr[r] << 56;
         rpack[r] = w;
