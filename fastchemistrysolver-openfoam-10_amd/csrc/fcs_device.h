@@ -405,11 +405,11 @@ __device__ __forceinline__ double gbcast(double v, int src, const Lds& L)
 
 // The wavefront form's identity slot 64 (C 1, exp(-G/RT) 1, d/dT 0), which
 // the unused reaction slots name.
-__device__ __forceinline__ void set_identity_slot(const Lds& L)
+__device__ __forceinline__ void set_identity_slot(const Lds& L, double dslot = 0.0)
 {
     L.C[64] = 1.0;
     L.expG[64] = 1.0;
-    L.dBdT[64] = 0.0;
+    L.dBdT[64] = dslot;
 }
 
 // ---------------------------------------------------------------- thermo
@@ -704,7 +704,17 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
         double num = 1.0, den = 1.0, sdb = 0.0;
         const int dn = nrh - nl;
         double pw = 1.0;
-        if constexpr (ONES) {
+        if constexpr (ONES && !JAC) {
+            // 1/Kc as a product: db holds exp(+G/RT) here; unused slots read
+            // the identity slot; the floor test Kc > floor reads as
+            // 1/Kc < 1/floor
+            const double P2 = PbyRT * PbyRT, M1 = 1.0 / PbyRT, M2 = M1 / PbyRT;
+            const double ipw = dn == 1 ? M1 : dn == 2 ? M2 : dn == -1 ? PbyRT : dn == -2 ? P2 : 1.0;
+            const double inv = ((((db[3] * db[4]) * db[5]) * ((g[0] * g[1]) * g[2]))) * ipw;
+            const double invKc = inv < 1.0 / KC_FLOOR ? inv : 1.0 / KC_FLOOR;
+            kr = k.kf * invKc;
+            dkrdM = k.dkfdM * invKc;
+        } else if constexpr (ONES) {
             // unused slots read the identity slot (1, 1, 0): the products and
             // sums below equal the used-slots-only ones exactly, and the
             // Kc-reversible form has at most 3 slots a side (|dn| <= 2)
@@ -723,13 +733,15 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
             if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;
             if (dn < 0) for (int q = 0; q < -dn; ++q) pw /= PbyRT;
         }
-        double Kc = (num / den) * pw;
-        const bool floored = !(Kc > KC_FLOOR);
-        Kc = floored ? KC_FLOOR : Kc;
-        const double invKc = 1.0 / Kc;
-        kr = k.kf * invKc;
-        if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * (sdb - dn * iT));
-        dkrdM = k.dkfdM * invKc;
+        if constexpr (!(ONES && !JAC)) {
+            double Kc = (num / den) * pw;
+            const bool floored = !(Kc > KC_FLOOR);
+            Kc = floored ? KC_FLOOR : Kc;
+            const double invKc = 1.0 / Kc;
+            kr = k.kf * invKc;
+            if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * (sdb - dn * iT));
+            dkrdM = k.dkfdM * invKc;
+        }
     }
     double CF = 1.0, CR = 1.0;
     if constexpr (ONES) {
@@ -780,10 +792,10 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
             const int s0 = pk_species(D0.pk, q), s1 = pk_species(D1.pk, q);
             c0[q] = L.C[s0];
             g0[q] = L.expG[s0];
-            db0[q] = JAC ? L.dBdT[s0] : 0.0;
+            db0[q] = (JAC || !Cell<NP>::kLarge) ? L.dBdT[s0] : 0.0;
             c1[q] = L.C[s1];
             g1[q] = L.expG[s1];
-            db1[q] = JAC ? L.dBdT[s1] : 0.0;
+            db1[q] = (JAC || !Cell<NP>::kLarge) ? L.dBdT[s1] : 0.0;
         }
         reaction_eval<JAC, !Cell<NP>::kLarge>(M, L, D0, r0, c0, g0, db0, T, lnT, iT, lnp, PbyRT);
         if (has1) reaction_eval<JAC, !Cell<NP>::kLarge>(M, L, D1, r1, c1, g1, db1, T, lnT, iT, lnp, PbyRT);
@@ -877,8 +889,14 @@ __device__ inline double derivatives(const DevMech& M, const Lds& L, double p, d
     Thermo th{0, 0, 0, 0, 0};
     if (sp) th = species_thermo(M, S, Tc, lnT, iT);
     L.C[lane] = sp ? rho * invW * y : 0.0;
-    L.expG[lane] = sp ? exp(th.negG) : 0.0;
-    if (!Cell<NP>::kLarge && lane == 0) set_identity_slot(L);
+    const double eg = sp ? exp(th.negG) : 0.0;
+    L.expG[lane] = eg;
+    if constexpr (!Cell<NP>::kLarge) {
+        // exp(+G/RT) in the dBdT slots (no derivative is formed here), for
+        // the rate round's division-free 1/Kc
+        L.dBdT[lane] = sp ? 1.0 / eg : 0.0;
+        if (lane == 0) set_identity_slot(L, 1.0);
+    }
     wave_sync();
     clk.mark(M, lane == 0, PH_RHS_SETUP);
     third_body_sums<NP>(M, L);
@@ -1043,10 +1061,17 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     if (M.nTBz > 0) {
         double acc = 0.0, oldv = 0.0;
         int cur = -1;
+        int nt = M.tbz_tb[lane & 15], nrow = M.tbz_row[lane & 15];
+        double nnu = M.tbz_nu[lane & 15];
         for (int k0 = 0; k0 < M.nTBz; k0 += 16) {   // nTBz % 16 == 0
-            const int e = k0 + (lane & 15);
-            const int vt = M.tbz_tb[e], vrow = M.tbz_row[e];
-            const double vnu = M.tbz_nu[e];
+            const int vt = nt, vrow = nrow;
+            const double vnu = nnu;
+            if (k0 + 16 < M.nTBz) {   // the next batch's entries, one load round trip ahead
+                const int e = k0 + 16 + (lane & 15);
+                nt = M.tbz_tb[e];
+                nrow = M.tbz_row[e];
+                nnu = M.tbz_nu[e];
+            }
             int rw_[16];
             double cf[16], ef[16], ov[16];
 #pragma unroll

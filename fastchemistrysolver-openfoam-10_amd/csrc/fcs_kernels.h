@@ -377,8 +377,12 @@ __device__ inline double cell_rate_state(const DevMech& M, const Lds& L, int nc,
     Thermo th{0, 0, 0, 0, 0};
     if (sp) th = species_thermo(M, S, Tcl, lnT, iT);
     L.C[lane] = Cl;
-    L.expG[lane] = sp ? exp(th.negG) : 0.0;
-    if (!Cell<NP>::kLarge && lane == 0) set_identity_slot(L);
+    const double eg = sp ? exp(th.negG) : 0.0;
+    L.expG[lane] = eg;
+    if constexpr (!Cell<NP>::kLarge) {
+        L.dBdT[lane] = sp ? 1.0 / eg : 0.0;   // exp(+G/RT) for the division-free 1/Kc
+        if (lane == 0) set_identity_slot(L, 1.0);
+    }
     wave_sync();
     third_body_sums<NP>(M, L);
     wave_sync();
