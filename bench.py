@@ -100,6 +100,20 @@ def cpu_baseline(mech, field, seconds=15.0):
                                       f"(OpenMP over cells, dynamic schedule)"))
 
 
+def counted_flops(mech, solver="OptRodas34", jacobian="exact"):
+    """FP64 operations of one attempt: the frozen count of the op-counting
+    oracle build (tests/golden/flops.json, rank-1 dC/dY conversion) for the
+    benchmark mechanisms, else the hand model of fcs/flops.py.  Returns
+    (flops, source)."""
+    from fcs.flops import attempt_flops
+    name = f"synthetic_{mech.n_species}sp_{mech.n_reactions}rxn"
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "flops.json")) as f:
+            return json.load(f)[name][f"{solver}/{jacobian}"]["flops"], "op-counting oracle build (tests/golden/flops.json)"
+    except (OSError, KeyError):
+        return attempt_flops(mech, solver), "hand model (fcs/flops.py)"
+
+
 def host_threads():
     """Host cores this process may use, at most 16 (one GPU's CPU share)."""
     return max(1, min(16, len(os.sched_getaffinity(0))))
@@ -313,14 +327,13 @@ def mixture(args, mech, gm):
     """Plain Rodas34 on the 65,536-cell 1-D mixture-fraction field, with the
     step kernel's roofline from an event pair around every step launch."""
     from fcs import lib
-    from fcs.flops import attempt_flops
     from fcs.mechanism import config_field
     f1 = config_field(1, mech)
     s1 = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4)
     b1 = lib.Batch(s1, f1)
     wall, launches, step_ms = timed_runs(b1, args.steps, args.warmup, profile=True)
     r = b1.download()
-    fpa = attempt_flops(mech, "OptRodas34")
+    fpa, fsrc = counted_flops(mech, "OptRodas34")
     att = int(r["nAttempts"].sum())
     avg_launch_s = 1e-3 * step_ms / max(launches, 1)
     achieved = fpa * att * args.steps / max(launches, 1) / avg_launch_s / 1e12
@@ -332,7 +345,7 @@ def mixture(args, mech, gm):
                        "attempts_per_step": att, "failed_cells": int(((r["status"] & 6) != 0).sum())},
             "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                         "kernel": "k_step<56,Rodas34>", "flops_per_attempt": fpa, "launches": launches,
+                         "kernel": "k_step<56,Rodas34>", "flops_per_attempt": fpa, "flops_source": fsrc, "launches": launches,
                          "avg_launch_ms": 1e3 * avg_launch_s}}
     print(json.dumps(line), flush=True)
 
@@ -370,7 +383,6 @@ def large(args):
     kernel's roofline comes from an event pair around every step launch of
     the plain exact run."""
     from fcs import learned, lib
-    from fcs.flops import attempt_flops
     from fcs.mechanism import config_field, config_mechanism, counterflow_field
     mech = config_mechanism(4)
     gm = lib.Mech(mech)
@@ -404,7 +416,7 @@ def large(args):
         log(f"{name}: {wall / args.steps * 1e3:.1f} ms per step, {runs[name]['att']} attempts")
         del b
     pl, ln = runs["plain"], runs["learned"]
-    fpa = attempt_flops(mech, "OptRodas34")
+    fpa, fsrc = counted_flops(mech, "OptRodas34")
     avg_launch_s = 1e-3 * pl["step_ms"] / max(pl["launches"], 1)
     achieved = fpa * pl["att"] / max(pl["launches"], 1) / avg_launch_s / 1e12
     npred, nver, nacc = ln["stats"]
@@ -429,7 +441,7 @@ def large(args):
             "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                          "kernel": "k_step<176,Rodas34> (one workgroup of 256 threads per active cell), plain run",
-                         "flops_per_attempt": fpa, "launches": pl["launches"],
+                         "flops_per_attempt": fpa, "flops_source": fsrc, "launches": pl["launches"],
                          "avg_launch_ms": 1e3 * avg_launch_s}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mech, field, seconds=args.cpu_seconds)
@@ -477,7 +489,6 @@ def main():
     if args.workload == "phases":
         return phases(args)
     from fcs import learned, lib
-    from fcs.flops import attempt_flops
     from fcs.mechanism import config_field, config_mechanism, counterflow_field
 
     mech = config_mechanism(2)
@@ -510,7 +521,7 @@ def main():
     log(f"training: {X.shape[0]} samples in {t_data:.1f} s, {args.epochs} epochs in {t_train:.1f} s, "
         f"loss {hist[0]:.3g} -> {hist[-1]:.3g}")
 
-    fpa = attempt_flops(mech, "OptRodas34")
+    fpa, fsrc = counted_flops(mech, "OptRodas34")
 
     # ---- plain Rodas34 on the field
     plain = lib.Batch(solver, field)
@@ -622,7 +633,7 @@ def main():
                      "traffic_unit": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
                                      "profiles/traffic_k_step_rodas34_53sp_counterflow_262144cells.md)",
                      "kernel": "k_step<56,Rodas34> (FP64 VALU; one Rosenbrock attempt per active cell)",
-                     "flops_per_attempt": fpa, "launches": launches,
+                     "flops_per_attempt": fpa, "flops_source": fsrc, "launches": launches,
                      "avg_launch_ms": 1e3 * avg_launch_s},
     }
     line["host_buffers"] = host

@@ -579,6 +579,20 @@ void orc_jacobian(const orc_mech* m, double p, double* YT, double* dYTdt,
     /* conversions */
     double* Yd = (double*)alloca(ns * sizeof(double));
     for (int i = 0; i < ns; ++i) Yd[i] = om[i] * (m->W[i] * invrho);
+#ifdef ORC_CONVERSION_RANK1
+    /* op-counting build only (oracle/flop_count.cpp): the exact conversion
+     * in its rank-1 form, the algorithmic work the device does,
+     * sum_k Jc_ik dC_k/dY_j = Jc_ij rho/W_j - (MW/W_j) sum_k Jc_ik C_k */
+    if (jac == FCS_JACOBIAN_EXACT) {
+        for (int i = 0; i < ns; ++i) {
+            double sC = 0.0;
+            for (int k = 0; k < ns; ++k) sC += Jc[i * n + k] * C[k];
+            for (int j = 0; j < ns; ++j)
+                Jy[i * n + j] = (m->W[i] * invrho) * (Jc[i * n + j] * (rho * m->invW[j]) - (MW * m->invW[j]) * sC)
+                              + (MW * m->invW[j]) * Yd[i];
+        }
+    } else
+#endif
     if (jac == FCS_JACOBIAN_EXACT) {
         double* dCdY = (double*)alloca(ns * sizeof(double));
         for (int j = 0; j < ns; ++j) {
