@@ -48,56 +48,72 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mech, field, seconds=15.0):
-    """Times the oracle on one core over a stratified random sample of the
-    field's classes, growing the sample until ~`seconds` of CPU work, and
-    converts the per-class cost into the whole field's cells·dt/s; then
-    times the same sample on all usable host cores (`all_cores`)."""
+def cpu_model():
+    """Host CPU model name (/proc/cpuinfo, as lscpu prints it) and the core
+    counts: nproc of the machine and the cores this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(model=name, nproc=os.cpu_count(), usable=len(os.sched_getaffinity(0)))
+
+
+def cpu_baseline(mech, field, n_cells=4096, gpu=None, runs=3):
+    """Times the oracle's AVX2 build (-O3 -mavx2 -mfma, libmvec exp/log) on a
+    stratified random sample of n_cells cells of the field, drawn in the
+    field's class proportions so that cells/s of the sample is the field's
+    cells·dt/s: once on one core, then the median of `runs` runs on every
+    usable host core (at most 16).  With gpu (a download() dict of the GPU
+    run over the whole field) it also reports the largest per-species |dY|
+    and |dT| between the CPU and GPU results on the sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
-    o = Oracle(mech)
+    o = Oracle(mech, variant="avx2")
     rng = np.random.default_rng(7)
-    classes = np.unique(field.cls)
-    cost, count = {}, {}
-    per = {c: rng.permutation(np.nonzero(field.cls == c)[0]) for c in classes}
-    used, n = 0.0, 4
-    done = {c: 0 for c in classes}
-    while used < seconds and any(done[c] < per[c].size for c in classes):
-        for c in classes:
-            idx = per[c][done[c]:done[c] + n]
-            if idx.size == 0:
-                continue
-            t0 = time.perf_counter()
-            o.solve(field.subset(idx), DT, threads=1, Treact=TREACT)
-            dt = time.perf_counter() - t0
-            cost[c] = cost.get(c, 0.0) + dt
-            count[c] = count.get(c, 0) + idx.size
-            done[c] += idx.size
-            used += dt
-        n = min(2 * n, 64)
-    per_cell = {c: cost[c] / count[c] for c in cost}
-    total = sum(per_cell[c] * (field.cls == c).sum() for c in per_cell)
-    desc = ", ".join(f"class {int(c)}: {count[c]} cells" for c in sorted(count))
-    # The same sample on every host core this process may use (at most 16,
-    # the CPU share of one GPU): the oracle's OpenMP driver over cells,
-    # dynamic schedule, one class at a time.
+    classes, counts = np.unique(field.cls, return_counts=True)
+    n_cells = min(n_cells, field.n_cells)
+    take = {int(c): int(round(n_cells * k / field.n_cells)) for c, k in zip(classes, counts)}
+    idx = np.sort(np.concatenate([rng.choice(np.nonzero(field.cls == c)[0], take[c], replace=False)
+                                  for c in take]))
+    sub = field.subset(idx)
+    warm = field.subset(np.concatenate([np.nonzero(field.cls == c)[0][:2] for c in take]))
+    o.solve(warm, DT, threads=1, Treact=TREACT)
+    t0 = time.perf_counter()
+    r1 = o.solve(sub, DT, threads=1, Treact=TREACT)
+    t1 = time.perf_counter() - t0
     threads = host_threads()
-    wall = 0.0
-    for c in cost:
-        idx = per[c][:done[c]]
+    walls = []
+    for _ in range(runs):
         t0 = time.perf_counter()
-        o.solve(field.subset(idx), DT, threads=threads, Treact=TREACT)
-        dt = time.perf_counter() - t0
-        wall += dt
-        per_cell[c] = dt / idx.size
-    total_mt = sum(per_cell[c] * (field.cls == c).sum() for c in per_cell)
-    return dict(value=field.n_cells / total, unit="cells·dt/s", cores=1, kind="port",
-                sample=f"stratified random sample of the {field.n_cells}-cell counterflow field ({desc}; "
-                       f"0 cold, 1 product, 2 flame front), {used:.1f} s on one core, plain OptRodas34 "
-                       f"oracle (C, -O3); per-class cost scaled to the field's class counts",
-                all_cores=dict(value=field.n_cells / total_mt, cores=threads,
-                               sample=f"the same sample, {wall:.1f} s wall on {threads} threads "
-                                      f"(OpenMP over cells, dynamic schedule)"))
+        o.solve(sub, DT, threads=threads, Treact=TREACT)
+        walls.append(time.perf_counter() - t0)
+    tm = float(np.median(walls))
+    desc = ", ".join(f"class {c}: {take[c]}" for c in sorted(take))
+    out = dict(value=idx.size / t1, unit="cells·dt/s", cores=1, kind="port",
+               sample=f"stratified random sample of {idx.size} cells of the {field.n_cells}-cell field in its "
+                      f"class proportions ({desc}; 0 cold, 1 product, 2 flame front), {t1:.1f} s on one core; "
+                      "plain OptRodas34 oracle built -O3 -mavx2 -mfma -ffast-math with libmvec 4-wide exp/log "
+                      "(oracle/liboracle_avx2.so), after a warm-up solve of 2 cells per class",
+               host=cpu_model(),
+               all_cores=dict(value=idx.size / tm, cores=threads,
+                              sample=f"the same sample, median of {runs} runs ({', '.join(f'{w:.2f}' for w in walls)} "
+                                     f"s wall) on {threads} threads (OpenMP over cells, dynamic schedule)"))
+    if gpu is not None:
+        rho = r1["rho0"]
+        dY = np.abs(r1["RR"] - gpu["RR"][:, idx]) * DT / rho[None, :]
+        Yc = np.abs(sub.Y + r1["RR"] * DT / rho[None, :])
+        ok = (r1["status"] & 6) == 0
+        over = (dY > np.maximum(1e-6, 1e-4 * Yc))[:, ok]
+        dT = float(np.abs(r1["Tend"] - gpu["Tend"][idx])[ok].max())
+        out["vs_gpu"] = dict(max_abs_dY=float(dY[:, ok].max()), entries_over_dY_bar=int(over.sum()),
+                             max_abs_dT_K=dT, holds=bool(not over.any() and dT <= 1e-3),
+                             bar="per-species |dY| <= 1e-6 absolute or <= 1e-4 relative, |dT| <= 1e-3 K")
+    return out
 
 
 def counted_flops(mech, solver="OptRodas34", jacobian="exact"):
@@ -413,6 +429,8 @@ def large(args):
                           value=field.n_cells / (wall / args.steps))
         if mlp is not None:
             runs[name]["stats"] = b.learned_stats()
+        if name == "plain":
+            runs[name]["res"] = r
         log(f"{name}: {wall / args.steps * 1e3:.1f} ms per step, {runs[name]['att']} attempts")
         del b
     pl, ln = runs["plain"], runs["learned"]
@@ -444,7 +462,7 @@ def large(args):
                          "flops_per_attempt": fpa, "flops_source": fsrc, "launches": pl["launches"],
                          "avg_launch_ms": 1e3 * avg_launch_s}}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(mech, field, seconds=args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(mech, field, n_cells=args.cpu_cells_large, gpu=pl.get("res"))
     print(json.dumps(line), flush=True)
 
 
@@ -471,7 +489,10 @@ def main():
                          "phases: per-phase cycles of the step kernel (profiling build libfcs_phases.so)")
     ap.add_argument("--phase-side", type=int, default=128, help="side of the phases workload's field")
     ap.add_argument("--large-cells", type=int, default=131072, help="cells of the large workload (x 512)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-cells", type=int, default=4096,
+                    help="cells of the CPU baseline's stratified sample (headline field)")
+    ap.add_argument("--cpu-cells-large", type=int, default=512,
+                    help="cells of the CPU baseline's sample on the 160-species workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); cells are independent, "
@@ -644,7 +665,7 @@ def main():
     if secondary:
         line["plain_mixture_fraction_65536"] = secondary
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(mech, field, seconds=args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(mech, field, n_cells=args.cpu_cells, gpu=rp)
         line["error_vs_oracle_tight"] = error_vs_oracle(mech, field, stratified(field, 48),
                                                         {"plain": rp, "learned": rl, "learned_seed_mode": rs})
     print(json.dumps(line), flush=True)

@@ -29,21 +29,27 @@ class StepRec(C.Structure):
                 ("accepted", C.c_int32), ("order", C.c_int32)]
 
 
-def build(force=False):
-    lib = os.path.join(HERE, "liboracle.so")
+# "strict": the checker (-ffp-contract=off, IEEE libm); "avx2": the same
+# source built as the reference builds its CPU path (-O3 -mavx2 -mfma,
+# fast math, so that exp/log loops call libmvec's 4-wide AVX2 variants),
+# used only as the timed CPU baseline.
+VARIANTS = {"strict": "liboracle.so", "avx2": "liboracle_avx2.so"}
+
+
+def build(force=False, variant="strict"):
+    lib = os.path.join(HERE, VARIANTS[variant])
     if force or not os.path.exists(lib):
         subprocess.check_call(["make", "-s", "-C", HERE, "all"])
     return lib
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        _lib = C.CDLL(build())
-        L = _lib
+def lib(variant="strict"):
+    if variant not in _libs:
+        L = C.CDLL(build(variant=variant))
+        _libs[variant] = L
         L.orc_mech_create.restype = C.c_void_p
         L.orc_mech_create.argtypes = [C.POINTER(MechDesc)]
         L.orc_mech_destroy.argtypes = [C.c_void_p]
@@ -70,57 +76,58 @@ def lib():
                                 _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
         L.orc_solve_lts.argtypes = [C.c_void_p, C.POINTER(SolverOpts), C.c_int64, _dp,
                                     _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp, _ip, C.c_int]
-    return _lib
+    return _libs[variant]
 
 
 class Oracle:
     """One mechanism loaded into the CPU oracle."""
 
-    def __init__(self, mech):
+    def __init__(self, mech, variant="strict"):
         self.mech = mech
         self.ns = mech.n_species
         self.n = self.ns + 1
+        self._L = lib(variant)
         self._desc, self._keep = make_desc(mech.arrays())
-        self._h = lib().orc_mech_create(C.byref(self._desc))
+        self._h = self._L.orc_mech_create(C.byref(self._desc))
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.orc_mech_destroy(self._h)
+        if getattr(self, "_h", None):
+            self._L.orc_mech_destroy(self._h)
             self._h = None
 
     def thermo(self, T):
         out = [np.zeros(self.ns) for _ in range(5)]
-        lib().orc_thermo(self._h, float(T), *[dptr(o) for o in out])
+        self._L.orc_thermo(self._h, float(T), *[dptr(o) for o in out])
         return dict(zip(["cp", "ha", "negG", "dBdT", "dcpdT"], out))
 
     def rates(self, p, T, Cc):
         Cc = np.ascontiguousarray(Cc, np.float64)
         om = np.zeros(self.ns)
-        lib().orc_rates(self._h, float(p), float(T), dptr(Cc), dptr(om))
+        self._L.orc_rates(self._h, float(p), float(T), dptr(Cc), dptr(om))
         return om
 
     def calculate(self, rho, T, p, Y):
         """FastChemistryModel::calculate for one cell: RR [ns] (kg/m^3/s)."""
         y = np.ascontiguousarray(Y, np.float64)
         rr = np.zeros(self.ns)
-        lib().orc_calculate(self._h, float(rho), float(T), float(p), dptr(y), dptr(rr))
+        self._L.orc_calculate(self._h, float(rho), float(T), float(p), dptr(y), dptr(rr))
         return rr
 
     def tc(self, rho, T, p, Y):
         """FastChemistryModel::tc for one cell (s)."""
-        return lib().orc_tc(self._h, float(rho), float(T), float(p), dptr(np.ascontiguousarray(Y, np.float64)))
+        return self._L.orc_tc(self._h, float(rho), float(T), float(p), dptr(np.ascontiguousarray(Y, np.float64)))
 
     def hf(self):
-        return np.array([lib().orc_hf(self._h, i) for i in range(self.ns)])
+        return np.array([self._L.orc_hf(self._h, i) for i in range(self.ns)])
 
     def qdot(self, RR):
         """FastChemistryModel::Qdot for one cell (W/m^3)."""
-        return lib().orc_qdot(self._h, dptr(np.ascontiguousarray(RR, np.float64)))
+        return self._L.orc_qdot(self._h, dptr(np.ascontiguousarray(RR, np.float64)))
 
     def derivatives(self, p, YT):
         y = np.array(YT, np.float64)
         d = np.zeros(self.n)
-        lib().orc_derivatives(self._h, float(p), dptr(y), dptr(d))
+        self._L.orc_derivatives(self._h, float(p), dptr(y), dptr(d))
         return d, y
 
     def jacobian(self, p, YT, jac="exact", want_jc=False):
@@ -128,7 +135,7 @@ class Oracle:
         f = np.zeros(self.n)
         J = np.zeros((self.n, self.n))
         Jc = np.zeros((self.n, self.n)) if want_jc else None
-        lib().orc_jacobian(self._h, float(p), dptr(y), dptr(f), dptr(J),
+        self._L.orc_jacobian(self._h, float(p), dptr(y), dptr(f), dptr(J),
                            0 if jac == "exact" else 1, dptr(Jc) if want_jc else None)
         return (f, J, y, Jc) if want_jc else (f, J, y)
 
@@ -138,7 +145,7 @@ class Oracle:
         o = make_opts(**opts)
         y = np.array(YT, np.float64)
         yn = np.zeros(self.n)
-        err = lib().orc_rosenbrock_attempt(self._h, C.byref(o), float(p), float(dx), dptr(y), dptr(yn))
+        err = self._L.orc_rosenbrock_attempt(self._h, C.byref(o), float(p), float(dx), dptr(y), dptr(yn))
         return err, yn, y
 
     def integrate_cell(self, p, YT, deltaT, dxTry, log_cap=0, **opts):
@@ -148,7 +155,7 @@ class Oracle:
         na = np.zeros(1, np.int32)
         ln = np.zeros(1, np.int32)
         recs = (StepRec * max(log_cap, 1))()
-        rc = lib().orc_integrate_cell(self._h, C.byref(o), float(p), dptr(y), float(deltaT),
+        rc = self._L.orc_integrate_cell(self._h, C.byref(o), float(p), dptr(y), float(deltaT),
                                       dptr(dx), iptr(na), recs, int(log_cap), iptr(ln))
         log = [(recs[i].x, recs[i].dx, recs[i].err, recs[i].accepted, recs[i].order)
                for i in range(int(ln[0]))]
@@ -170,13 +177,13 @@ class Oracle:
         Tend = np.zeros(nc)
         na = np.zeros(nc, np.int32)
         if np.ndim(deltaT) == 0:
-            rc = lib().orc_solve(self._h, C.byref(o), nc, float(deltaT), dptr(T0), dptr(p0),
+            rc = self._L.orc_solve(self._h, C.byref(o), nc, float(deltaT), dptr(T0), dptr(p0),
                                  dptr(rho0), dptr(Y0), dptr(dtc), dptr(RR), dptr(dmin),
                                  iptr(st), dptr(Tend), iptr(na), int(threads))
         else:
             dtf = np.ascontiguousarray(deltaT, np.float64)
             assert dtf.shape == (nc,)
-            rc = lib().orc_solve_lts(self._h, C.byref(o), nc, dptr(dtf), dptr(T0), dptr(p0),
+            rc = self._L.orc_solve_lts(self._h, C.byref(o), nc, dptr(dtf), dptr(T0), dptr(p0),
                                      dptr(rho0), dptr(Y0), dptr(dtc), dptr(RR), dptr(dmin),
                                      iptr(st), dptr(Tend), iptr(na), int(threads))
         return dict(rc=rc, RR=RR, deltaTChem=dtc, deltaTMin=float(dmin[0]), status=st,
