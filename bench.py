@@ -10,8 +10,8 @@ Default workload (the metric's): learned-seeded Rodas34 on the 512 x 512
 counterflow field (262,144 cells: 45 % cold cells below Treact = 500 K,
 40 % near-equilibrium products, 15 % flame front), step launches replayed
 from a hipGraph.  The network is trained first (not timed) by the
-library's own BF16-MFMA kernels on samples the plain GPU integrator
-produces from two other seeded counterflow fields.  The same field is also
+library's own BF16-MFMA kernels on 2 M samples the plain GPU integrator
+produces from other seeded 256 x 256 counterflow fields (56 of them).  The same field is also
 run with the plain integrator (no network), and the 65,536-cell
 mixture-fraction field of the plain-integrator configuration is run too;
 their rates are reported beside `value`.  With the CPU baseline, the error
@@ -411,7 +411,7 @@ def large(args):
     Xs, Ts = zip(*(learned.samples(exact, counterflow_field(mech, 512, seed=sd, equilibrate=eq, n_rows=16), DT)
                    for sd in (4001, 4002)))
     X, T = np.concatenate(Xs), np.concatenate(Ts)
-    net, hist = learned.train(X, T, hidden=(256, 256, 256), epochs=args.epochs, batch=512, lr=2e-3, seed=1)
+    net, hist = learned.train(X, T, hidden=(256, 256, 256), epochs=args.large_epochs, batch=512, lr=2e-3, seed=1)
     t_train = time.perf_counter() - t0
     log(f"training: {X.shape[0]} samples, loss {hist[0]:.3g} -> {hist[-1]:.3g}, {t_train:.1f} s")
     runs = {}
@@ -474,7 +474,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the 65,536-cell plain run")
     ap.add_argument("--train-side", type=int, default=256, help="side of each training field")
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--train-samples", type=int, default=2_000_000,
+                    help="training samples of the default workload's network (whole fields, so a few more)")
+    ap.add_argument("--epochs", type=int, default=5)
+    ap.add_argument("--large-epochs", type=int, default=30, help="training epochs of the large workload's network")
     ap.add_argument("--workload", choices=("counterflow", "mixture", "attempt", "large", "large-kernels",
                                            "traffic", "copy", "tutorials", "phases"),
                     default="counterflow",
@@ -529,11 +532,14 @@ def main():
     # ---- learned seed: training data from the plain integrator, training (untimed)
     t0 = time.perf_counter()
     Xs, Ts = [], []
-    for seed in (2001, 2002):
+    seed, n_samples = 2001, 0
+    while n_samples < args.train_samples:   # seeded fields until the sample count is reached
         f = counterflow_field(mech, args.train_side, seed=seed, equilibrate=eq)
         X, T = learned.samples(solver, f, DT)
         Xs.append(X)
         Ts.append(T)
+        n_samples += X.shape[0]
+        seed += 1
     X, T = np.concatenate(Xs), np.concatenate(Ts)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -637,7 +643,7 @@ def main():
                    "cells": field.n_cells, "species": mech.n_species, "reactions": mech.n_reactions,
                    "solver": "OptRodas34", "jacobian": "exact", "absTol": 1e-8, "relTol": 1e-4,
                    "network": "56-128-128-128-55 SiLU, BF16 MFMA / FP32 accumulation",
-                   "training_samples": int(X.shape[0]), "training_epochs": args.epochs,
+                   "training_samples": int(X.shape[0]), "training_fields": len(Xs), "training_epochs": args.epochs,
                    "training_s": t_data + t_train},
         "plain": {"value": plain_value, "ms_per_step": 1e3 * wall_p / args.steps,
                   "attempts_per_step": int(rp["nAttempts"].sum())},

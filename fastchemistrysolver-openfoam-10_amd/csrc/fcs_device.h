@@ -83,7 +83,8 @@ struct DevMech {
     const double* clo;        // [7*SW] coefficient-major: c[k*SW + s]
     const double* chi;        // [7*SW]
     const int* rtype;         // [nr]
-    const int* rrev;          // [nr]  0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius in rA0/rbeta0/rTa0
+    const int* rrev;          // [nr]  0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius in rA0/rbeta0/rTa0,
+                              //       3 kf/Kc over a product side of more than 3 slots (gen row rgen[r])
     const int* rnl;           // [nr]
     const int* rnr;           // [nr]
     const int* rsp;           // [6*nr] slot-major: rsp[q*nr + r], q<3 lhs, q>=3 rhs
@@ -116,6 +117,17 @@ struct DevMech {
     int maxTerms;             // per-species Jacobian term list length, multiple of 16
     const unsigned long long* sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*
     const double* Hf;         // [SW] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
+    // Kc-reversible reactions with more than 3 product slots (rrev 3): their
+    // Jacobian takes one reverse partial kr prod_{b != q} C_b per product slot
+    // q, kept in the reaction arrays of LDS past the nr reactions, so nrq =
+    // nr + those partials sizes the reaction arrays
+    int nrq;
+    // PLOG rows are the reactions plogBase .. plogBase + nPlog - 1 (reactions
+    // are sorted by rate form); plog row = r - plogBase
+    int plogBase, nPlog;
+    const int* rgen;          // [nr] row of gen (rrev 3 only; -1 elsewhere)
+    const int* gen;           // [rows * GEN_STRIDE]: product slots n, LDS index of the first partial
+                              // (>= nr), the n product species
 #ifdef FCS_PHASE_TIMING
     unsigned long long* phase;  // [PH_SLOTS * PH_SPREAD] shader-clock cycles per phase
 #endif
@@ -138,6 +150,9 @@ __device__ __forceinline__ int rn_nu(int w) { return (w >> 16) - 8; }
 __device__ __forceinline__ int pk_species(unsigned long long w, int q) { return (int)((w >> (8 * q)) & 255u); }
 __device__ __forceinline__ int pk_nl(unsigned long long w) { return (int)((w >> 48) & 255u); }
 __device__ __forceinline__ int pk_nr(unsigned long long w) { return (int)(w >> 56); }
+
+// Ints per row of DevMech::gen: slot count, partial base, up to 16 species.
+constexpr int GEN_STRIDE = 18;
 
 // LDS doubles at the end of a cell's LDS that hold a copy of its DevMech
 // (see stage_mech in fcs_kernels.h).
@@ -250,6 +265,8 @@ struct Lds {
     double* J;     // [NP*JSt] Jacobian, then iteration matrix and its LU factors
                    // (LDS; for the workgroup form a per-workgroup slice in HBM
                    // of NP*NP + NP doubles, the last NP holding 1/pivot)
+    double* plog;  // [3*nPlog] per-cell PLOG rows as Arrhenius (ln A, beta, Ta) at the
+                   // cell's pressure (stage_plog), kept for the whole kernel
     int* ip;       // [G] thread holding each logical row after the LU
     double* lu;    // workgroup form: LU / solve scratch over rq..Mtb (dead by then)
     double* red;   // workgroup form: [16] cross-wavefront reduction / broadcast slots
@@ -275,28 +292,30 @@ __host__ __device__ inline int group_of(int NP) { return NP > 64 ? (NP <= 128 ? 
 // each).  Workgroup form: the
 // LU scratch reuses the reaction arrays; the matrix (and its reciprocal
 // pivots) follows them in LDS up to kJLdsMaxNP rows, else it is in HBM.
-__host__ __device__ inline size_t lds_doubles(int NP, int nr, int nTB)
+// nrq: slots of the per-reaction arrays (DevMech::nrq); nPlog: PLOG rows.
+__host__ __device__ inline size_t lds_doubles(int NP, int nrq, int nTB, int nPlog)
 {
-    const size_t nrP = round_up(nr, 2);
+    const size_t nrP = round_up(nrq, 2), plP = round_up(3 * nPlog, 2);
     if (NP <= 64) {
         const size_t kq = 2 * nrP > 128 ? 2 * nrP : 128;   // rk + rdT, later H + Q
-        return 3 * 66 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + (size_t)NP * JS + 32 +
+        return 3 * 66 + nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2) + plP + (size_t)NP * JS + 32 +
                kMechLdsDoubles;
     }
     const size_t G = group_of(NP);
     const size_t kq = 2 * nrP > 2 * G ? 2 * nrP : 2 * G;
     const size_t react = nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2);
     const size_t lu = lu_scratch_doubles(NP);
-    const size_t rest = 256 + 2 * G + (react > lu ? react : lu) + G / 2 + 16;
+    const size_t rest = 256 + 2 * G + (react > lu ? react : lu) + G / 2 + 16 + plP;
     return (jlds_of(NP) ? ((rest + 1) & ~(size_t)1) + (size_t)NP * NP + NP : rest) + kMechLdsDoubles;
 }
 
-__device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jglobal = nullptr)
+__device__ inline Lds make_lds(double* base, int NP, int nrq, int nTB, int nPlog, double* Jglobal = nullptr)
 {
     Lds L;
+    const int plP = round_up(3 * nPlog, 2);
     if (NP > 64) {
         const int G = group_of(NP);
-        const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
+        const int nrP = round_up(nrq, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
         const int kq = 2 * nrP > 2 * G ? 2 * nrP : 2 * G;
         const size_t react = nrP + kq + 2 * (size_t)nTBP;
         const size_t lu = lu_scratch_doubles(NP);
@@ -313,11 +332,12 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jg
         L.lu = L.rq;
         L.ip = (int*)(L.rq + (react > lu ? react : lu));
         L.red = L.rq + (react > lu ? react : lu) + G / 2;
-        const size_t rest = 256 + 2 * (size_t)G + (react > lu ? react : lu) + G / 2 + 16;
+        L.plog = L.red + 16;   // past the LU scratch, which reuses rq .. Mtb
+        const size_t rest = 256 + 2 * (size_t)G + (react > lu ? react : lu) + G / 2 + 16 + plP;
         L.J = jlds_of(NP) ? base + ((rest + 1) & ~(size_t)1) : Jglobal;
         return L;
     }
-    const int nrP = round_up(nr, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
+    const int nrP = round_up(nrq, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
     const int kq = 2 * nrP > 128 ? 2 * nrP : 128;
     L.C = base;              // C[64] = 1.0: the "no other species" slot of the Jacobian terms
     L.expG = base + 66;      // expG[64] = 1.0 and dBdT[64] = 0: the identity slot unused
@@ -329,7 +349,8 @@ __device__ inline Lds make_lds(double* base, int NP, int nr, int nTB, double* Jg
     L.Q = L.rk + 64;
     L.rw = L.rk + kq;
     L.Mtb = L.rw + nTBP;
-    L.J = L.Mtb + nTBP;
+    L.plog = L.Mtb + nTBP;
+    L.J = L.plog + plP;
     L.ip = (int*)(L.J + (size_t)NP * JS);
     L.lu = nullptr;
     L.red = nullptr;
@@ -493,41 +514,59 @@ __device__ __forceinline__ RxnData load_rxn(const DevMech& M, const Lds& L, int 
     return d;
 }
 
-// Rate constant of reaction r (kf includes M for third-body and falloff
-// forms).  JAC selects the Jacobian-path variants, which also return
-// dkf/dT and dkf/dM (src/Reaction/ReactionRate.C:84-672,
-// src/Reaction/Jacobian.C:108-910).
-template <bool JAC>
-__device__ inline RxnK rate_constant(const DevMech& M, const RxnData& D, int r, double T, double lnT, double iT,
-                                     double lnp)
+// PLOG rows at the cell's pressure as Arrhenius triples (ln A, beta, Ta) in
+// L.plog, one per row, for rate_constant.  The reference interpolates ln k
+// linearly in ln p between the tabulated pressures around p, clamped to the
+// end rows (ReactionRate.C:577-650, Jacobian.C:996 for dk/dT):
+//   ln k = lnA_j + beta_j lnT - Ta_j/T + f (dlnA + dbeta lnT - dTa/T)
+//        = (lnA_j + f dlnA) + (beta_j + f dbeta) lnT - (Ta_j + f dTa)/T,
+// with j and f fixed by p, so one triple per row serves every temperature
+// of the cell and the rate rounds read it from LDS instead of walking the
+// tables.  Every thread of the cell's group calls this (it synchronises).
+template <int NP>
+__device__ inline void stage_plog(const DevMech& M, const Lds& L, double lnp)
 {
-    const int t = D.t;
-    RxnK k{0.0, 0.0, 0.0};
-    if (t == RXN_PLOG) {
-        const int row = M.raux[r];
+    __syncthreads();   // earlier readers of L.plog (the previous cell) are done
+    for (int row = tid<NP>(); row < M.nPlog; row += Cell<NP>::G) {
         const int np = M.plog_n[row];
         const double* lp = M.plog_lnp + (size_t)row * M.pmax;
         const double* la = M.plog_lnA + (size_t)row * M.pmax;
         const double* b = M.plog_beta + (size_t)row * M.pmax;
         const double* ta = M.plog_Ta + (size_t)row * M.pmax;
-        double lk, dlk;
+        int j = 0;
+        double f = 0.0;
         if (lnp <= lp[0]) {
-            lk = la[0] + b[0] * lnT - ta[0] * iT;
-            dlk = (b[0] + ta[0] * iT) * iT;
+            j = 0;
         } else if (lnp >= lp[np - 1]) {
-            lk = la[np - 1] + b[np - 1] * lnT - ta[np - 1] * iT;
-            dlk = (b[np - 1] + ta[np - 1] * iT) * iT;
+            j = np - 1;
         } else {
-            int j = 0;
             for (int q = 0; q < np - 1; ++q)
                 if (lp[q] <= lnp && lnp <= lp[q + 1]) { j = q; break; }
-            const double f = (lnp - lp[j]) / (lp[j + 1] - lp[j]);
-            lk = la[j] + b[j] * lnT - ta[j] * iT
-               + (la[j + 1] - la[j] + (b[j + 1] - b[j]) * lnT - (ta[j + 1] - ta[j]) * iT) * f;
-            dlk = iT * (b[j] + ta[j] * iT + (b[j + 1] - b[j] + (ta[j + 1] - ta[j]) * iT) * f);
+            f = (lnp - lp[j]) / (lp[j + 1] - lp[j]);
         }
+        const int j1 = f != 0.0 ? j + 1 : j;
+        L.plog[3 * row + 0] = la[j] + (la[j1] - la[j]) * f;
+        L.plog[3 * row + 1] = b[j] + (b[j1] - b[j]) * f;
+        L.plog[3 * row + 2] = ta[j] + (ta[j1] - ta[j]) * f;
+    }
+    __syncthreads();
+}
+
+// Rate constant of reaction r (kf includes M for third-body and falloff
+// forms).  JAC selects the Jacobian-path variants, which also return
+// dkf/dT and dkf/dM (src/Reaction/ReactionRate.C:84-672,
+// src/Reaction/Jacobian.C:108-910).
+template <bool JAC>
+__device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, const RxnData& D, int r, double T, double lnT,
+                                     double iT)
+{
+    const int t = D.t;
+    RxnK k{0.0, 0.0, 0.0};
+    if (t == RXN_PLOG) {
+        const double* e = L.plog + 3 * (r - M.plogBase);   // stage_plog
+        const double lk = e[0] + e[1] * lnT - e[2] * iT;
         k.kf = exp(lk);
-        if (JAC) k.dkfdT = k.kf * dlk;
+        if (JAC) k.dkfdT = k.kf * ((e[1] + e[2] * iT) * iT);
         return k;
     }
     const double kk = D.A * exp(D.b * lnT - D.Ta * iT);
@@ -687,9 +726,51 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
 {
     const unsigned long long pk = D.pk;
     const int nl = pk_nl(pk), nrh = pk_nr(pk);
-    const RxnK k = rate_constant<JAC>(M, D, r, T, lnT, iT, lnp);
+    const RxnK k = rate_constant<JAC>(M, L, D, r, T, lnT, iT);
     double kr = 0.0, dkrdT = 0.0, dkrdM = 0.0;
-    if (D.rev == 2) {
+    double CRg = 1.0;   // product of every product slot's C (rrev 3)
+    if (D.rev == 3) {
+        // Kc-reversible with more than 3 product slots: the reference's
+        // general path over every slot (RFG, ReactionFunctionI.H:648-720;
+        // JFG, JacobianFunctionI.H:2965-3060).  RFG floors Kc at
+        // 1.49011611938476e-8 (the value of Tc.C), JFG at the usual
+        // 1.4901171103413047e-8.  The Jacobian's reverse partials
+        // kr prod_{b != q} C_b go to the reaction arrays past nr, where the
+        // term walk reads them.
+        const int* gp = M.gen + M.rgen[r] * GEN_STRIDE;
+        const int n = gp[0], base = gp[1];
+        double num = 1.0, den = 1.0, sdb = 0.0;
+        for (int a = 0; a < n; ++a) {
+            const int s = gp[2 + a];
+            num *= L.expG[s];
+            CRg *= L.C[s];
+            if (JAC) sdb += L.dBdT[s];
+        }
+        for (int a = 0; a < nl; ++a) {
+            const int s = pk_species(pk, a);
+            den *= L.expG[s];
+            if (JAC) sdb -= L.dBdT[s];
+        }
+        const int dn = n - nl;   // >= 1: at least 4 product and at most 3 reactant slots
+        double pw = 1.0;
+        for (int q = 0; q < dn; ++q) pw *= PbyRT;
+        const double kcMin = JAC ? KC_FLOOR : TC_KC_FLOOR;
+        double Kc = (num / den) * pw;
+        const bool floored = !(Kc > kcMin);
+        Kc = floored ? kcMin : Kc;
+        const double invKc = 1.0 / Kc;
+        kr = k.kf * invKc;
+        if (JAC) dkrdT = k.dkfdT * invKc - (floored ? 0.0 : kr * (sdb - dn * iT));
+        dkrdM = k.dkfdM * invKc;
+        if (JAC) {
+            for (int q = 0; q < n; ++q) {
+                double P = kr;
+                for (int b = 0; b < n; ++b)
+                    if (b != q) P *= L.C[gp[2 + b]];
+                L.rq[base + q] = P;
+            }
+        }
+    } else if (D.rev == 2) {
         // nonEquilibrium: the reaction's own reverse Arrhenius (in the k_0
         // slot), times M for third-body rows (ReactionFunctionI.H:20-23,
         // ReactionRate.C:362-367; reverse efficiencies = forward ones,
@@ -754,11 +835,14 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
             if (a < nrh) CR *= c[3 + a];
         }
     }
+    if (D.rev == 3) CR = CRg;
     if (!JAC) {
         L.rq[r] = k.kf * CF - kr * CR;
     } else {
         L.rq[r] = k.kf;
-        L.rk[r] = kr;
+        // (rrev 3: kr CR, which the production walk takes with CR = 1 -- the
+        // per-species list entries of these reactions name no product slot)
+        L.rk[r] = D.rev == 3 ? kr * CR : kr;
         L.rdT[r] = k.dkfdT * CF - dkrdT * CR;
         if (D.t == RXN_TB || D.t == RXN_LIND || D.t == RXN_TROE || D.t == RXN_SRI)
             L.rw[D.tb] = k.dkfdM * CF - dkrdM * CR;
@@ -842,7 +926,7 @@ __device__ inline double2 time_scale_sums(const DevMech& M, const Lds& L, double
     double sw = 0.0, sw2 = 0.0;
     for (int r = tid<NP>(); r < M.nr; r += Cell<NP>::G) {
         const RxnData D = load_rxn(M, L, r);
-        const RxnK k = rate_constant<false>(M, D, r, T, lnT, iT, lnp);
+        const RxnK k = rate_constant<false>(M, L, D, r, T, lnT, iT);
         const int nl = pk_nl(D.pk), nrh = pk_nr(D.pk);
         double CF = 1.0, CR = 1.0, num = 1.0, den = 1.0;
 #pragma unroll
@@ -851,7 +935,16 @@ __device__ inline double2 time_scale_sums(const DevMech& M, const Lds& L, double
             if (a < nrh) { CR *= L.C[pk_species(D.pk, 3 + a)]; num *= L.expG[pk_species(D.pk, 3 + a)]; }
         }
         double kr = 0.0;
-        if (D.rev == 1) {
+        if (D.rev == 3) {   // every product slot (Tc.C:578-600 walks the whole side)
+            const int* gp = M.gen + M.rgen[r] * GEN_STRIDE;
+            CR = 1.0;
+            num = 1.0;
+            for (int a = 0; a < gp[0]; ++a) {
+                CR *= L.C[gp[2 + a]];
+                num *= L.expG[gp[2 + a]];
+            }
+        }
+        if (D.rev == 1 || D.rev == 3) {
             const int dn = nrh - nl;
             double pw = 1.0;
             if (dn > 0) for (int q = 0; q < dn; ++q) pw *= PbyRT;

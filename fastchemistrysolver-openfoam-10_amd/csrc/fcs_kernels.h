@@ -54,8 +54,8 @@ struct CellArrays {
 template <int NP>
 __device__ __forceinline__ const DevMech& stage_mech(const DevMech* Mg, double* lds_raw)
 {
-    const int nr = Mg->nr, nTB = Mg->nTB;
-    DevMech* Ms = reinterpret_cast<DevMech*>(lds_raw + lds_doubles(NP, nr, nTB) - kMechLdsDoubles);
+    const int nrq = Mg->nrq, nTB = Mg->nTB, nPlog = Mg->nPlog;
+    DevMech* Ms = reinterpret_cast<DevMech*>(lds_raw + lds_doubles(NP, nrq, nTB, nPlog) - kMechLdsDoubles);
     const unsigned int* src = reinterpret_cast<const unsigned int*>(Mg);
     unsigned int* dst = reinterpret_cast<unsigned int*>(Ms);
     for (int w = tid<NP>(); w < (int)(sizeof(DevMech) / 4); w += Cell<NP>::G) dst[w] = src[w];
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step(const DevMech* Mg, DevOpts
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog, jslice<NP>(a.Jg));
     const int lane = tid<NP>();
     const int ns = M.ns;
     const size_t nc = (size_t)a.nc;
@@ -128,7 +128,9 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step(const DevMech* Mg, DevOpts
         const double xEnd = a.dtCell ? a.dtCell[c] : a.dt;
         const int nStep0 = nStep;
         double dxAcc = 0.0;
-        const int st = rosenbrock_step<NP>(M, L, o, tableau_of<SOLVER>(), a.p0[c], xEnd, y,
+        const double p = a.p0[c];
+        stage_plog<NP>(M, L, log(p));
+        const int st = rosenbrock_step<NP>(M, L, o, tableau_of<SOLVER>(), p, xEnd, y,
                                            x, dxTry, dx, dxTry0, flags, nStep, nAtt, dxAcc);
         if (lane == 0 && nStep0 == 0 && dxAcc > 0.0 && a.dxFirst[c] == 0.0) a.dxFirst[c] = dxAcc;
         if (st == STEP_CONTINUE) {
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(const DevMech* Mg, 
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog, jslice<NP>(a.Jg));
     const int lane = tid<NP>();
     const int ns = M.ns;
     const size_t nc = (size_t)a.nc;
@@ -183,6 +185,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(const DevMech* Mg, 
         sc.nStep = a.nStep[c];
         sc.nAtt = a.nAtt[c];
         double* JyG = a.Jy + (size_t)c * NP * jy_stride<NP>();
+        stage_plog<NP>(M, L, log(a.p0[c]));
         const int st = seulex_pass<NP>(M, L, o, a.p0[c], a.dtCell ? a.dtCell[c] : a.dt, JyG, y, y0s, scale, sc);
         if (st == STEP_CONTINUE) {
             if (row >= 0) {
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_verify(const DevMech* Mg, DevOp
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(a.Jg));
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog, jslice<NP>(a.Jg));
     const int lane = tid<NP>();
     const int ns = M.ns;
     const size_t nc = (size_t)a.nc;
@@ -248,6 +251,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_verify(const DevMech* Mg, DevOp
             const double yP = row >= 0 ? y0 + sinh((double)pr[1 + row]) * (o.absTol + o.relTol * fabs(y0)) : 0.0;
             double yT = 0.0;
             const double p = a.p0[c];
+            stage_plog<NP>(M, L, log(p));
             const double err = rosenbrock_attempt<NP>(M, L, o, kRodas34, p, log(p), dt, y0, yT);
             double r = 0.0;
             if (row >= 0) {
@@ -279,10 +283,11 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_derivatives(const DevMech* Mg, 
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog);
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
+    stage_plog<NP>(M, L, log(p[s]));
     const double d = derivatives<NP>(M, L, p[s], log(p[s]), y);
     if (ext >= 0) {
         YT[(size_t)ext * nS + s] = y;
@@ -296,10 +301,11 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_jacobian(const DevMech* Mg, int
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog, jslice<NP>(Jg));
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
+    stage_plog<NP>(M, L, log(p[s]));
     const double d = jacobian<NP>(M, L, p[s], log(p[s]), jacType, y);
     if (ext >= 0) {
         YT[(size_t)ext * nS + s] = y;
@@ -324,11 +330,12 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_attempt(const DevMech* Mg, DevO
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog, jslice<NP>(Jg));
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     double y = ext >= 0 ? YT[(size_t)ext * nS + s] : 0.0;
     double yT = 0.0;
+    stage_plog<NP>(M, L, log(p[s]));
     const double e = rosenbrock_attempt<NP>(M, L, o, tableau_of<SOLVER>(), p[s], log(p[s]),
                                             dx[s], y, yT);
     if (ext >= 0) {
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_lu_solve(const DevMech* Mg, int
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB, jslice<NP>(Jg));
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog, jslice<NP>(Jg));
     const int s = blockIdx.x, lane = tid<NP>(), ns = M.ns, n = ns + 1;
     const int ext = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
     if (!Cell<NP>::kLarge || lane < NP)
@@ -398,9 +405,10 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_calculate(const DevMech* Mg, in
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog);
     const int c = blockIdx.x, lane = tid<NP>();
     double Tcl, lnT, iT;
+    stage_plog<NP>(M, L, log(p[c]));
     (void)cell_rate_state<NP>(M, L, nc, c, rho, T, Y, Tcl, lnT, iT);
     reaction_round<false, NP>(M, L, Tcl, lnT, iT, log(p[c]));
     wave_sync();
@@ -416,9 +424,10 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_tc(const DevMech* Mg, int nc, c
 {
     extern __shared__ double lds_raw[];
     const DevMech& M = stage_mech<NP>(Mg, lds_raw);
-    const Lds L = make_lds(lds_raw, NP, M.nr, M.nTB);
+    const Lds L = make_lds(lds_raw, NP, M.nrq, M.nTB, M.nPlog);
     const int c = blockIdx.x;
     double Tcl, lnT, iT;
+    stage_plog<NP>(M, L, log(p[c]));
     const double Cl = cell_rate_state<NP>(M, L, nc, c, rho, T, Y, Tcl, lnT, iT);
     const double sumc = gsum<NP>(Cl, L);
     const double2 sw = time_scale_sums<NP>(M, L, Tcl, lnT, iT, log(p[c]));

@@ -128,11 +128,12 @@ const KernelSet* const kKernels[] = {FCS_FOR_EACH_NP(FCS_KSET_PTR) FCS_FOR_EACH_
 // table sizes given (nr >= 0), a workgroup-form size that keeps the matrix in
 // LDS is passed over when its LDS would exceed a CU's; the next size, with
 // the same species-table layout, then takes the mechanism.
-const KernelSet* pick_kernels(int n, int nr = -1, int nTB = 0)
+const KernelSet* pick_kernels(int n, int nr = -1, int nTB = 0, int nPlog = 0)
 {
     for (const KernelSet* k : kKernels) {
         if (k->NP < n) continue;
-        if (nr >= 0 && jlds_of(k->NP) && lds_doubles(k->NP, nr, nTB) * sizeof(double) > kMaxLdsBytes) continue;
+        if (nr >= 0 && jlds_of(k->NP) && lds_doubles(k->NP, nr, nTB, nPlog) * sizeof(double) > kMaxLdsBytes)
+            continue;
         return k;
     }
     return nullptr;
@@ -362,14 +363,12 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         for (int r = 0; r < nr; ++r)
             if (d->rxn_type[r] == t) perm.push_back(r);
     if ((int)perm.size() != nr) return FCS_E_ARG;
-    // slot layout: up to 3 reactant slots; product sides longer than 3 slots
-    // (up to slot_stride) only on reactions without a Kc reverse rate
+    // slot layout: up to 3 reactant slots, up to slot_stride product slots
     const int ss = d->slot_stride > 0 ? d->slot_stride : FCS_MAX_STOICH;
     if (ss < FCS_MAX_STOICH || ss > FCS_MAX_SLOTS) return FCS_E_ARG;
     for (int r = 0; r < nr; ++r) {
         const int nl = d->rxn_n_lhs[r], nrh = d->rxn_n_rhs[r];
         if (nl < 1 || nl > FCS_MAX_STOICH || nrh < 1 || nrh > ss) return FCS_E_ARG;
-        if (nrh > FCS_MAX_STOICH && d->rxn_reversible[r] == 1) return FCS_E_UNSUPPORTED;
         for (int q = 0; q < nl; ++q)
             if (d->rxn_lhs[(size_t)ss * r + q] < 0 || d->rxn_lhs[(size_t)ss * r + q] >= ns) return FCS_E_ARG;
         for (int q = 0; q < nrh; ++q)
@@ -416,6 +415,9 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         // has no reverse rate; the products beyond slot 3 then enter only
         // through the net stoichiometry (nu below) and the slot count
         if (rnr[r] > FCS_MAX_STOICH && rrev[r] == 2) rrev[r] = 0;
+        // a Kc reverse rate over such a side takes every product slot
+        // (the device's general branch, rrev 3)
+        if (rnr[r] > FCS_MAX_STOICH && rrev[r] == 1) rrev[r] = 3;
         for (int q = 0; q < 3; ++q) {
             rsp[q * nr + r] = q < rnl[r] ? d->rxn_lhs[(size_t)ss * f + q] : 0;
             rsp[(3 + q) * nr + r] = q < rnr[r] ? d->rxn_rhs[(size_t)ss * f + q] : 0;
@@ -467,6 +469,25 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             }
         }
     }
+    // PLOG rows: contiguous in device order (sorted by form), row = r - plogBase
+    int plogBase = nr;
+    for (int r = 0; r < nr; ++r)
+        if (rtype[r] == FCS_RXN_PLOG) { plogBase = r; break; }
+    for (int r = 0; r < nr; ++r)
+        if (rtype[r] == FCS_RXN_PLOG && raux[r] != r - plogBase) return FCS_E_ARG;
+    // general-path rows of the rrev 3 reactions: slot count, LDS index of the
+    // first reverse partial (the reaction arrays continue past nr), species
+    std::vector<int> rgen(std::max(nr, 1), -1), gen;
+    int nrq = nr;
+    for (int r = 0; r < nr; ++r) {
+        if (rrev[r] != 3) continue;
+        rgen[r] = (int)gen.size() / GEN_STRIDE;
+        gen.push_back(rnr[r]);
+        gen.push_back(nrq);
+        for (int q = 0; q < GEN_STRIDE - 2; ++q) gen.push_back(q < rnr[r] ? d->rxn_rhs[(size_t)ss * perm[r] + q] : 0);
+        nrq += rnr[r];
+    }
+    if (nrq > 0xffff) return FCS_E_ARG;   // 16-bit reaction field of the Jacobian terms
     // third_body_sums reads rows t0 + [0, 2 SW) per pass unconditionally
     const int nTBpad = std::max(2 * SW, (nTB + 2 * SW - 1) / (2 * SW) * (2 * SW));
     std::vector<double> effT((size_t)SW * nTBpad, 0.0);
@@ -508,9 +529,17 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     for (int s = 0; s < ns; ++s) {
         for (size_t e = 0; e < lst[s].size(); ++e) {
             const int nuI = (int)lst[s][e].second;
-            if (nuI != lst[s][e].second || nuI < -7 || nuI > 7 || lst[s][e].first > 0xffff) return FCS_E_ARG;
-            spRN[e * SW + s] = lst[s][e].first | ((nuI + 8) << 16);
-            spPack[e * SW + s] = rpack[lst[s][e].first];
+            const int r = lst[s][e].first;
+            if (nuI != lst[s][e].second || nuI < -7 || nuI > 7 || r > 0xffff) return FCS_E_ARG;
+            spRN[e * SW + s] = r | ((nuI + 8) << 16);
+            unsigned long long pk = rpack[r];
+            if (rrev[r] == 3) {
+                // the Jacobian round stores kr times the whole product side in
+                // rk: the entry names no product slot (identity slots, count 0)
+                pk &= ~(0xffffffull << 24 | 0xffull << 56);
+                if (kset->NP <= 64) pk |= 64ull << 24 | 64ull << 32 | 64ull << 40;
+            }
+            spPack[e * SW + s] = pk;
         }
         for (size_t e = 0; e < lstTB[s].size(); ++e) {
             tbzRow.push_back(s);
@@ -543,6 +572,17 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             const int r = e.first, nu = (int)e.second;
             for (int side = 0; side < 2; ++side) {
                 if (side == 1 && !rrev[r]) continue;
+                if (side == 1 && rrev[r] == 3) {
+                    // one term per product slot q: -nu times the partial
+                    // kr prod_{b != q} C_b the Jacobian round leaves at LDS
+                    // index gen[1] + q of the reaction arrays
+                    const int* g = gen.data() + (size_t)rgen[r] * GEN_STRIDE;
+                    for (int q = 0; q < g[0]; ++q)
+                        terms[s].push_back((unsigned long long)(g[1] + q) | (unsigned long long)g[2 + q] << 16 |
+                                           (unsigned long long)ONE << 24 | (unsigned long long)ONE << 32 |
+                                           (unsigned long long)(-nu + 8) << 48);
+                    continue;
+                }
                 const int cnt = side == 0 ? rnl[r] : rnr[r];
                 const int sgn = side == 0 ? nu : -nu;
                 for (int q = 0; q < cnt; ++q) {
@@ -580,6 +620,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     const size_t iTbzRow = vi(tbzRow), iTbzTb = vi(tbzTb), iTbzNu = vd(tbzNu);
     auto vq = [&](const std::vector<unsigned long long>& v) { return add(v.data(), v.size() * 8); };
     const size_t iRpack = vq(rpack), iSpPack = vq(spPack);
+    const size_t iRgen = vi(rgen), iGen = vi(gen);
     std::vector<unsigned char> host(total, 0);
     for (const auto& p : pieces)
         if (p.bytes) std::memcpy(host.data() + p.off, p.src, p.bytes);
@@ -606,6 +647,11 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
 #endif
     dm.ns = ns;
     dm.nr = nr;
+    dm.nrq = nrq;
+    dm.plogBase = plogBase;
+    dm.nPlog = (int)plogN.size();
+    dm.rgen = (const int*)P(iRgen);
+    dm.gen = (const int*)P(iGen);
     dm.NPm = SW;
     dm.nTB = nTB;
     dm.pmax = pmax;
@@ -710,14 +756,14 @@ int fcs_solver_create(const fcs_mech* mech, const fcs_solver_opts* opts, fcs_sol
     if (opts->solver < 0 || opts->solver > FCS_SOLVER_ROSENBROCK23 || opts->jacobian < 0 || opts->jacobian > 1 ||
         opts->maxSteps < 1 || opts->maxSteps > 10000000 || !(opts->absTol > 0) || !(opts->relTol > 0))
         return FCS_E_ARG;
-    const KernelSet* ks = pick_kernels(mech->ns + 1, mech->nr, mech->dm.nTB);
+    const KernelSet* ks = pick_kernels(mech->ns + 1, mech->dm.nrq, mech->dm.nTB, mech->dm.nPlog);
     if (!ks || !ks->step[opts->solver]) return FCS_E_UNSUPPORTED;   // no Seulex in the workgroup form
     fcs_solver* s = new (std::nothrow) fcs_solver();
     if (!s) return FCS_E_NOMEM;
     s->mech = mech;
     s->opts = *opts;
     s->ks = ks;
-    s->ldsBytes = lds_doubles(ks->NP, mech->nr, mech->dm.nTB) * sizeof(double);
+    s->ldsBytes = lds_doubles(ks->NP, mech->dm.nrq, mech->dm.nTB, mech->dm.nPlog) * sizeof(double);
     if (s->ldsBytes > kMaxLdsBytes) {   // reaction arrays beyond one CU's LDS
         delete s;
         return FCS_E_UNSUPPORTED;

@@ -241,11 +241,9 @@ def mechanism(reactions: dict, thermo: dict) -> Mechanism:
         t = _FORMS[form]
         lhs_s, rhs_s = r["reaction"].split("=", 1)
         lhs, rhs = _side(lhs_s, index), _side(rhs_s, index)
-        # reactant sides of up to 3 slots; longer product sides only where
-        # the reaction has no Kc reverse rate (irreversible, nonEquilibrium)
-        if len(lhs) > MAX_STOICH or (len(rhs) > MAX_STOICH and rev and not noneq):
-            raise ValueError(f"{key}: {typ} {r['reaction']!r}: {len(lhs)} reactant and {len(rhs)} product "
-                             f"slots; reactant sides and the product side of a Kc-reversible reaction "
+        # reactant sides of up to 3 slots (product sides up to MAX_SLOTS, _side)
+        if len(lhs) > MAX_STOICH:
+            raise ValueError(f"{key}: {typ} {r['reaction']!r}: {len(lhs)} reactant slots; reactant sides "
                              f"take at most {MAX_STOICH}")
         rx = Reaction(t, rev, lhs, rhs, 0.0, 0.0, 0.0)
         if noneq:

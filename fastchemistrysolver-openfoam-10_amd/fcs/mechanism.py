@@ -27,8 +27,8 @@ PSTD = 1.0e5
 RXN_ARRHENIUS, RXN_THIRD_BODY, RXN_LINDEMANN, RXN_TROE, RXN_PLOG, RXN_SRI = 0, 1, 2, 3, 4, 5
 FALLOFF = (RXN_LINDEMANN, RXN_TROE, RXN_SRI)
 USES_M = (RXN_THIRD_BODY,) + FALLOFF
-MAX_STOICH = 3     # reactant slots; product slots of Kc-reversible reactions
-MAX_SLOTS = 16     # product slots of irreversible / nonEquilibrium reactions
+MAX_STOICH = 3     # reactant slots per reaction
+MAX_SLOTS = 16     # product slots per reaction
 ELEM_MASS = np.array([1.00794, 12.0107, 15.9994, 14.0067])  # H C O N [kg/kmol]
 
 # Heats of formation are drawn in +-HF_RANGE J/mol.  At +-3e5 J/mol a random

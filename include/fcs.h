@@ -65,9 +65,8 @@ extern "C" {
 #define FCS_JACOBIAN_EXACT  0   /* jacobian exact; (default)                     */
 #define FCS_JACOBIAN_FAST   1   /* jacobian fast;  diagonal dC/dY only           */
 
-#define FCS_MAX_STOICH      3   /* reactant slots per reaction; product slots of a
-                                   reversible (rxn_reversible 1) reaction           */
-#define FCS_MAX_SLOTS      16   /* product slots of other reactions (slot_stride) */
+#define FCS_MAX_STOICH      3   /* reactant slots per reaction                    */
+#define FCS_MAX_SLOTS      16   /* product slots per reaction (slot_stride)       */
 
 /*
  * Flat mechanism description (species thermo + reactions, file order).
@@ -87,12 +86,12 @@ extern "C" {
  * sri is [n_reactions x 5] = (a, b, c, d, e) read for SRI rows; sri, A_rev,
  * beta_rev and Ta_rev may be NULL when no row needs them.
  * rxn_lhs / rxn_rhs hold slot_stride slots per reaction (slot_stride 0 means
- * FCS_MAX_STOICH; at most FCS_MAX_SLOTS).  Reactant sides have 1..3 slots.
- * Product sides of irreversible and nonEquilibrium reactions may have up to
- * slot_stride slots; such a reaction with more than 3 slots on a side takes
- * the reference's general path (RFG / JFG, ReactionFunctionI.H:648-720,
- * JacobianFunctionI.H:2965-3060), which has no reverse rate for
- * nonEquilibrium rows.  Kc-reversible reactions keep 1..3 slots a side.
+ * FCS_MAX_STOICH; at most FCS_MAX_SLOTS).  Reactant sides have 1..3 slots,
+ * product sides 1..slot_stride.  A reaction with more than 3 product slots
+ * takes the reference's general path (RFG / JFG, ReactionFunctionI.H:648-720,
+ * JacobianFunctionI.H:2965-3060): a Kc reverse rate runs over every product
+ * slot (RFG floors Kc at 1.49011611938476e-8, JFG at 1.4901171103413047e-8),
+ * and a nonEquilibrium row has no reverse rate there.
  */
 typedef struct fcs_mech_desc {
     int32_t n_species;
@@ -106,7 +105,7 @@ typedef struct fcs_mech_desc {
     const int32_t* rxn_type;    /* [nr] FCS_RXN_*                         */
     const int32_t* rxn_reversible; /* [nr] 1: reverse rate from Kc         */
     const int32_t* rxn_n_lhs;   /* [nr] 1..3                              */
-    const int32_t* rxn_n_rhs;   /* [nr] 1..3 (1..slot_stride, see above)  */
+    const int32_t* rxn_n_rhs;   /* [nr] 1..slot_stride                    */
     const int32_t* rxn_lhs;     /* [nr*slot_stride] species index per slot */
     const int32_t* rxn_rhs;     /* [nr*slot_stride]                       */
     const double* A;            /* [nr] forward Arrhenius / k_inf         */

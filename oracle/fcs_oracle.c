@@ -42,6 +42,10 @@
 #define PSTD 100000.0
 /* Kc floor: src/Reaction/ReactionFunctionI.H:17, JacobianFunctionI.H:1153 */
 #define KC_FLOOR 1.4901171103413047e-8
+/* the Kc floor of Tc.C:599 and of the general RHS path RFG
+ * (ReactionFunctionI.H:688), which reactions with more than 3 slots on a side
+ * take (ReactionRate.C:565-569) */
+#define TC_KC_FLOOR 1.49011611938476e-08
 /* Troe clamp of the 4-wide rate/Jacobian paths: src/Reaction/ReactionRate.C:475 */
 #define TROE_SMALL 2.2e-16
 /* zero-pivot replacement: src/LUsolver/LUsolver.C:153-156 */
@@ -365,8 +369,10 @@ static double reverse_k(const orc_mech* m, int r, double kf, double dkfdT, doubl
     if (!m->rev[r]) return 0.0;
     double dlnKc;
     double Kc = kc_of(m, r, expG, dBdT, T, &dlnKc);
-    const int floored = !(Kc > KC_FLOOR);
-    Kc = floored ? KC_FLOOR : Kc;
+    /* RHS (dBdT NULL) of a reaction on the general path: RFG's floor */
+    const double kcMin = (!dBdT && (m->nl[r] > 3 || m->nrh[r] > 3)) ? TC_KC_FLOOR : KC_FLOOR;
+    const int floored = !(Kc > kcMin);
+    Kc = floored ? kcMin : Kc;
     const double invKc = 1.0 / Kc;
     const double kr = kf * invKc;
     if (dBdT) *dkrdT = dkfdT * invKc - (floored ? 0.0 : kr * dlnKc);
@@ -416,7 +422,6 @@ void orc_calculate(const orc_mech* m, double rho, double T, double p, const doub
  * tests isIrreversible == 0, so nonEquilibrium reactions count as
  * irreversible).  tc = sumW / sumWRateByCTot * sum(c), vGreat if the
  * denominator is 0. */
-#define TC_KC_FLOOR 1.49011611938476e-08
 double orc_tc(const orc_mech* m, double rho, double Tin, double p, const double* Y)
 {
     const int ns = m->ns;

@@ -41,7 +41,7 @@ def gri30():
 
 TUTORIAL_CASES = ("NH3_Zhang_counterflow", "NC12H26_Yao_counterflow", "H2_Connaire2004",
                   "Kerosene_Zettervall2020", "NC12H26_Gao2014", "DME_LLNL", "NH3CH4_Konnov",
-                  "NC7H16_Yoo2011", "NC12H26_Narayanaswamy2014", "C2H4_Sandiego")
+                  "NC7H16_Yoo2011", "NC12H26_Narayanaswamy2014", "C2H4_Sandiego", "C3H8_Qin2000")
 
 
 @pytest.fixture(scope="session")

@@ -16,7 +16,8 @@ The cases cover the reaction forms and layouts the tutorials use:
 ArrheniusPLOG (NH3_Zhang), nonEquilibrium reversible and third-body rows
 (H2_Connaire2004, DME_LLNL), irreversible reactions with 4 to 7 product
 slots (NC12H26_Yao, Kerosene_Zettervall2020, NC12H26_Gao2014), SRI falloff
-(NH3CH4_Konnov) and species counts across the padded system sizes of both
+(NH3CH4_Konnov), a Kc-reversible reaction with 4 product slots
+(C3H8_Qin2000) and species counts across the padded system sizes of both
 execution forms (10 to 255).
 
 The fixture is data only: mechanism coefficients and field values.
@@ -42,6 +43,7 @@ CASES = {
     "NC7H16_Yoo2011": "Ignition0D/NC7H16_Yoo2011",
     "NC12H26_Narayanaswamy2014": "Ignition0D/NC12H26_Narayanaswamy2014",
     "C2H4_Sandiego": "Ignition0D/C2H4_Sandiego",
+    "C3H8_Qin2000": "Ignition0D/C3H8_Qin2000",
 }
 N_SNAPSHOT, N_IGNITION = 256, 64
 

@@ -213,20 +213,27 @@ def kc_independent(m, T, lhs, rhs):
     return np.exp(-dG) * (M.PSTD / (RU * T)) ** (len(rhs) - len(lhs))
 
 
-@pytest.mark.parametrize("lhs,rhs", [([0], [1]), ([0], [1, 2]), ([0, 1], [2]), ([0, 1], [2, 3])])
+@pytest.mark.parametrize("lhs,rhs", [([0], [1]), ([0], [1, 2]), ([0, 1], [2]), ([0, 1], [2, 3]),
+                                     # product sides of 4 and 5 slots: the general path (RFG)
+                                     ([0], [1, 2, 3, 1]), ([0, 1], [2, 3, 2, 2, 2])])
 def test_detailed_balance(lhs, rhs):
     """q = 0 at a composition satisfying prod C_rhs / prod C_lhs = Kc."""
     T, p = 1800.0, 1e5
-    for seed in range(11, 200):   # a thermo draw with Kc well above the Kc floor
+    # a thermo draw with Kc well above the Kc floor (the long sides' random
+    # thermo draws give Kc of 1e8 and more: no upper bound there)
+    hi = 1e4 if len(rhs) <= 3 else np.inf
+    for seed in range(11, 200):
         m = tiny_mech([R(M.RXN_ARRHENIUS, lhs, rhs, A=1e8, beta=0.3, Ta=4000.0, rev=True)], seed=seed)
         Kc = kc_independent(m, T, lhs, rhs)
-        if 1e-4 < Kc < 1e4:
+        if 1e-4 < Kc < hi:
             break
+    assert 1e-4 < Kc < hi
     o = Oracle(m)
     C = np.full(4, 0.01)
-    # scale the first product so that the mass-action ratio equals Kc
+    # scale the first product of a single slot (and no reactant slot) so that
+    # the mass-action ratio equals Kc
     ratio = np.prod(C[rhs]) / np.prod(C[lhs])
-    C[rhs[0]] *= Kc / ratio
+    C[next(s for s in rhs if rhs.count(s) == 1 and s not in lhs)] *= Kc / ratio
     om = o.rates(p, T, C)
     kf = 1e8 * T ** 0.3 * np.exp(-4000.0 / T)
     scale = kf * np.prod(C[lhs])
@@ -250,6 +257,31 @@ def test_kc_floor():
     C = np.array([0.0, 0.02, 0.0, 0.0])
     kf = 1e5 * np.exp(-1000.0 / T)
     assert o.rates(p, T, C)[0] == pytest.approx(kf / 1.4901171103413047e-8 * C[1], rel=1e-12)
+
+
+def test_kc_floor_of_the_general_path():
+    """A Kc-reversible reaction with more than 3 product slots takes the
+    general path, whose RHS floors Kc at 1.49011611938476e-8
+    (ReactionFunctionI.H:688) while its Jacobian floors it at
+    1.4901171103413047e-8 (JacobianFunctionI.H:3005); with no reactant
+    present, the product's rate is the reverse rate alone in both, so the
+    RHS and the Jacobian evaluation's f(y) differ by the ratio of the floors."""
+    m = tiny_mech([R(M.RXN_ARRHENIUS, [0], [1, 2, 3, 1], A=1e5, beta=0.0, Ta=1000.0, rev=True)], seed=3)
+    m.coeffs_low[1, 5] += 1e5    # S1 very unstable: Kc << floor
+    m.coeffs_high[1, 5] += 1e5
+    o = Oracle(m)
+    T, p = 1000.0, 1e5
+    assert kc_independent(m, T, [0], [1, 2, 3, 1]) < 1e-20
+    C = np.array([0.0, 0.02, 0.03, 0.01])
+    kf = 1e5 * np.exp(-1000.0 / T)
+    q = -kf / 1.49011611938476e-08 * C[1] * C[2] * C[3] * C[1]
+    np.testing.assert_allclose(o.rates(p, T, C), q * np.array([-1.0, 2.0, 1.0, 1.0]), rtol=1e-12)
+    # f(y) of the RHS and of the Jacobian evaluation on the same state
+    Y = C * m.W / (C * m.W).sum()
+    y = np.r_[Y, T]
+    f_rhs, _ = o.derivatives(p, y)
+    f_jac, _, _ = o.jacobian(p, y)
+    np.testing.assert_allclose(f_jac[1:4] / f_rhs[1:4], 1.49011611938476e-08 / 1.4901171103413047e-8, rtol=1e-12)
 
 
 # ------------------------------------------------------------------ invariants

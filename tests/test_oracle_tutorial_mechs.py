@@ -1,8 +1,8 @@
 """Mechanisms of the reference's tutorials (tests/golden/tutorial_mechs.npz,
 written by tests/golden/make_tutorial_mechs.py): what the loader reads from
 the files, and the oracle on the forms the synthetic mechanisms lack --
-ArrheniusPLOG rows, irreversible reactions with 4 to 7 product slots (the
-reference's general path RFG / JFG, ReactionFunctionI.H:648-720,
+ArrheniusPLOG rows, reactions with 4 to 7 product slots, irreversible ones and
+a Kc-reversible one (the reference's general path RFG / JFG, ReactionFunctionI.H:648-720,
 JacobianFunctionI.H:2965-3060, ReactionRate.C:565-569) and the
 nonEquilibrium rows of real mechanisms.
 
@@ -34,6 +34,7 @@ EXPECTED = {
     "NC7H16_Yoo2011": (188, 939, {0: 905, 1: 19, 2: 5, 3: 10}, 924, 3),
     "NC12H26_Narayanaswamy2014": (255, 1509, {0: 1465, 1: 10, 3: 34}, 3, 4),
     "C2H4_Sandiego": (59, 271, {0: 233, 1: 15, 3: 23}, 0, 3),
+    "C3H8_Qin2000": (70, 463, {0: 420, 1: 6, 2: 1, 3: 36}, 0, 4),
 }
 
 
@@ -155,7 +156,39 @@ def test_long_noneq_reaction_has_no_reverse_rate(tutorial_case):
     np.testing.assert_allclose(d, -nu * kr, rtol=1e-12, atol=1e-12 * kr)
 
 
-def test_loader_rejects_long_kc_reversible_sides():
+def g_over_rt(mech, i, T):
+    """G/RT of species i from its NASA-7 fit, H and S evaluated separately."""
+    low = T < mech.T_common.min() or (T <= mech.T_common.max() and T < mech.T_common[i])
+    a = (mech.coeffs_low if low else mech.coeffs_high)[i]
+    h = a[0] + a[1] * T / 2 + a[2] * T ** 2 / 3 + a[3] * T ** 3 / 4 + a[4] * T ** 4 / 5 + a[5] / T
+    s = a[0] * np.log(T) + a[1] * T + a[2] * T ** 2 / 2 + a[3] * T ** 3 / 3 + a[4] * T ** 4 / 4 + a[6]
+    return h - s
+
+
+def test_long_kc_reversible_closed_form(tutorial_case):
+    """C3H8_Qin2000's Kc-reversible reaction with 4 product slots
+    (C6H5O + O = HCO + 2 C2H2 + CO) alone, on the general path: omega_i =
+    nu_i (kf prod C_reactants - kf/Kc prod C_products) over every slot, Kc =
+    exp(-dG/RT) (Pstd/(Ru T))^dn from the NASA-7 fits, floored at RFG's
+    1.49011611938476e-8 (ReactionFunctionI.H:648-720)."""
+    mech, _ = tutorial_case("C3H8_Qin2000")
+    (rx,) = long_reactions(mech)
+    assert rx.reversible and not rx.rev_arr and len(rx.lhs) == 2 and len(rx.rhs) == 4
+    rng = np.random.default_rng(5)
+    o = Oracle(single(mech, rx))
+    for T, p in ((900.0, 1e5), (1500.0, 2e6), (2400.0, 5e6)):
+        C = rng.uniform(0.1, 1.0, mech.n_species) * 1e-3
+        dG = sum(g_over_rt(mech, i, T) for i in rx.rhs) - sum(g_over_rt(mech, i, T) for i in rx.lhs)
+        Kc = np.exp(-dG) * (M.PSTD / (M.RU * T)) ** (len(rx.rhs) - len(rx.lhs))
+        kf = rx.A * T ** rx.beta * np.exp(-rx.Ta / T)
+        q = kf * np.prod(C[rx.lhs]) - kf / max(Kc, 1.49011611938476e-8) * np.prod(C[rx.rhs])
+        nu = np.zeros(mech.n_species)
+        np.add.at(nu, rx.lhs, -1.0)
+        np.add.at(nu, rx.rhs, 1.0)
+        np.testing.assert_allclose(o.rates(p, T, C), nu * q, rtol=1e-11, atol=1e-11 * kf * np.prod(C[rx.lhs]))
+
+
+def test_loader_reads_long_kc_reversible_sides():
     text = """
 species 5 (A B C D E);
 reactions
@@ -167,13 +200,15 @@ reactions
                   "thermodynamics": {"Tlow": 200, "Thigh": 3500, "Tcommon": 1000,
                                      "highCpCoeffs": [3.0, 0, 0, 0, 0, 0, 0],
                                      "lowCpCoeffs": [3.0, 0, 0, 0, 0, 0, 0]}} for n in "ABCDE"}
-    with pytest.raises(ValueError, match="Kc-reversible"):
-        foam.mechanism(foam.parse(text), thermo)
-    mech = foam.mechanism(foam.parse(text.replace("reversibleArrhenius", "irreversibleArrhenius")), thermo)
-    assert mech.reactions[0].rhs == [1, 2, 3, 4]
-    a = mech.arrays()
-    assert a["slot_stride"] == 4 and list(a["rxn_rhs"]) == [1, 2, 3, 4]
-    assert M.Mechanism.from_arrays(a).reactions[0].rhs == [1, 2, 3, 4]
+    for typ in ("reversibleArrhenius", "irreversibleArrhenius"):
+        mech = foam.mechanism(foam.parse(text.replace("reversibleArrhenius", typ)), thermo)
+        assert mech.reactions[0].rhs == [1, 2, 3, 4]
+        assert mech.reactions[0].reversible == (typ == "reversibleArrhenius")
+        a = mech.arrays()
+        assert a["slot_stride"] == 4 and list(a["rxn_rhs"]) == [1, 2, 3, 4]
+        assert M.Mechanism.from_arrays(a).reactions[0].rhs == [1, 2, 3, 4]
+    with pytest.raises(ValueError, match="reactant slots"):
+        foam.mechanism(foam.parse(text.replace('"A = B + C + D + E"', '"B + C + D + E = A"')), thermo)
 
 
 def fd_jacobian(o, p, y, rel=1e-6):
@@ -188,7 +223,7 @@ def fd_jacobian(o, p, y, rel=1e-6):
     return J
 
 
-@pytest.mark.parametrize("name", ["Kerosene_Zettervall2020", "NC12H26_Gao2014", "H2_Connaire2004"])
+@pytest.mark.parametrize("name", ["Kerosene_Zettervall2020", "NC12H26_Gao2014", "H2_Connaire2004", "C3H8_Qin2000"])
 def test_exact_jacobian_matches_finite_differences(tutorial_case, name):
     """The exact Jacobian on real mechanisms (long product sides,
     nonEquilibrium rows) against central differences of the RHS, at states
