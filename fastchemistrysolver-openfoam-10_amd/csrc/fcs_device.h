@@ -114,8 +114,9 @@ struct DevMech {
     const double* tbz_nu;     // [nTBz] net stoichiometric coefficient (padding: 0)
     const unsigned long long* rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
     const unsigned long long* sp_pack;  // [maxDeg*SW] rpack of each per-species list entry
-    int maxTerms;             // per-species Jacobian term list length, multiple of 16
-    const unsigned long long* sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*
+    int maxTerms;             // Jacobian terms per thread (jacobian's sparse-partial walk), multiple of 16
+    const unsigned long long* sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*: every
+                                        // element's terms in one thread, the threads' loads balanced
     const double* Hf;         // [SW] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
     // Kc-reversible reactions with more than 3 product slots (rrev 3): their
     // Jacobian takes one reverse partial kr prod_{b != q} C_b per product slot
@@ -133,18 +134,21 @@ struct DevMech {
 #endif
 };
 
-// Jacobian term of species row i: nu_ir * d q_r / d C_j for one reactant
-// (forward) or product (reverse) slot of reaction r.  Bits 0-15 r, 16-23
-// column j, 24-31 and 32-39 the other species of the slot's side (slot
-// order), 40-41 their count, 42 reverse, 48-55 the signed coefficient + 8
-// (negated for reverse terms; 0 = padding).
+// Jacobian term: nu_ir * d q_r / d C_j for one reactant (forward) or product
+// (reverse) slot of reaction r, added to matrix element (row i, column j).
+// Bits 0-15 r (index into the reaction arrays), 16-23 column j, 24-31 row i
+// (16-31 together: the element key), 32-39 and 40-47 the other species of the
+// slot's side (slot order; ONE when absent), 48-55 the signed coefficient + 8
+// (negated for reverse terms; 0 = padding), 56 reverse (kr instead of kf),
+// 57 dump (a padding term of a thread with no element of its own: its store
+// goes to a slot no phase reads).
 __device__ __forceinline__ int jterm_r(unsigned long long w) { return (int)(w & 0xffffu); }
-__device__ __forceinline__ int jterm_col(unsigned long long w) { return (int)((w >> 16) & 255u); }
-__device__ __forceinline__ int jterm_o1(unsigned long long w) { return (int)((w >> 24) & 255u); }
-__device__ __forceinline__ int jterm_o2(unsigned long long w) { return (int)((w >> 32) & 255u); }
-__device__ __forceinline__ int jterm_no(unsigned long long w) { return (int)((w >> 40) & 3u); }
-__device__ __forceinline__ bool jterm_rev(unsigned long long w) { return (w >> 42) & 1u; }
+__device__ __forceinline__ int jterm_key(unsigned long long w) { return (int)((w >> 16) & 0xffffu); }
+__device__ __forceinline__ int jterm_o1(unsigned long long w) { return (int)((w >> 32) & 255u); }
+__device__ __forceinline__ int jterm_o2(unsigned long long w) { return (int)((w >> 40) & 255u); }
 __device__ __forceinline__ int jterm_nu(unsigned long long w) { return (int)((w >> 48) & 255u) - 8; }
+__device__ __forceinline__ bool jterm_rev(unsigned long long w) { return (w >> 56) & 1u; }
+__device__ __forceinline__ bool jterm_dump(unsigned long long w) { return (w >> 57) & 1u; }
 __device__ __forceinline__ int rn_r(int w) { return w & 0xffff; }
 __device__ __forceinline__ int rn_nu(int w) { return (w >> 16) - 8; }
 __device__ __forceinline__ int pk_species(unsigned long long w, int q) { return (int)((w >> (8 * q)) & 255u); }
@@ -1055,14 +1059,19 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     reaction_round<true, NP>(M, L, T, lnT, iT, lnp);
     wave_sync();
     clk.mark(M, lane == 0, PH_JAC_RATES);
-    // species rows, sparse partials: row i is a list of terms nu_ir dq_r/dC_j
-    // sorted by column (host order within a column follows the species'
-    // reaction list).  Each lane accumulates one column at a time in a
-    // register and stores it when the column changes (each element is
-    // written once, no read-modify-write).
-    if (sp) {
+    // species rows, sparse partials: the terms nu_ir dq_r/dC_j of every
+    // matrix element, grouped by element (rows, then columns ascending;
+    // within an element in the order of row i's reaction list) and split
+    // over the threads at element boundaries in runs of equal length, so no
+    // thread walks more terms than the longest run (a row-per-thread walk
+    // takes the longest row's list on every lane).  Each thread keeps a
+    // running sum of the current element in a register and stores it after
+    // every term (the element's last store holds its sum): each element is
+    // written by one thread, no read-modify-write, no branch.
+    {
+        const int dumpOff = Cell<NP>::kLarge ? NP * NP + lane % NP : (lane % NP) * JSt + 64;
         double acc = 0.0;
-        int cur = jterm_col(M.sp_term[lane]);
+        int cur = jterm_key(M.sp_term[lane]);
         unsigned long long wn[16];   // next batch, loaded one batch ahead
 #pragma unroll
         for (int u = 0; u < 16; ++u) wn[u] = M.sp_term[u * SW + lane];
@@ -1083,17 +1092,16 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             }
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
-                // running sum of the current column, stored after every term
-                // (the column's last store holds its sum): no branch
                 const int nu = jterm_nu(w[u]);
-                const int col = jterm_col(w[u]);
+                const int key = jterm_key(w[u]);
                 const double t = (double)nu * ((k[u] * c1[u]) * c2[u]);
-                acc = col == cur ? acc + t : 0.0 + t;
-                cur = col;
-                Jrow[col * JSt] = acc;
+                acc = key == cur ? acc + t : 0.0 + t;
+                cur = key;
+                L.J[jterm_dump(w[u]) ? dumpOff : (key & 255) * JSt + (key >> 8)] = acc;
             }
         }
     }
+    wave_sync();   // the walk wrote other threads' rows
     clk.mark(M, lane == 0, PH_JAC_TERMS);
     // production rate and d(omega)/dT of this lane's species (its reaction
     // list, loads and gathers issued per block of 8 entries)

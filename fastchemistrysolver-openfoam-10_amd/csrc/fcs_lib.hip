@@ -579,8 +579,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
                     const int* g = gen.data() + (size_t)rgen[r] * GEN_STRIDE;
                     for (int q = 0; q < g[0]; ++q)
                         terms[s].push_back((unsigned long long)(g[1] + q) | (unsigned long long)g[2 + q] << 16 |
-                                           (unsigned long long)ONE << 24 | (unsigned long long)ONE << 32 |
-                                           (unsigned long long)(-nu + 8) << 48);
+                                           (unsigned long long)s << 24 | (unsigned long long)ONE << 32 |
+                                           (unsigned long long)ONE << 40 | (unsigned long long)(-nu + 8) << 48);
                     continue;
                 }
                 const int cnt = side == 0 ? rnl[r] : rnr[r];
@@ -590,9 +590,9 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
                     for (int b = 0; b < cnt; ++b)
                         if (b != q) oth[no++] = rsp[(3 * side + b) * nr + r];
                     const unsigned long long col = (unsigned long long)rsp[(3 * side + q) * nr + r];
-                    terms[s].push_back((unsigned long long)r | col << 16 | (unsigned long long)oth[0] << 24 |
-                                       (unsigned long long)oth[1] << 32 | (unsigned long long)no << 40 |
-                                       (unsigned long long)side << 42 | (unsigned long long)(sgn + 8) << 48);
+                    terms[s].push_back((unsigned long long)r | col << 16 | (unsigned long long)s << 24 |
+                                       (unsigned long long)oth[0] << 32 | (unsigned long long)oth[1] << 40 |
+                                       (unsigned long long)(sgn + 8) << 48 | (unsigned long long)side << 56);
                 }
             }
         }
@@ -600,14 +600,47 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
             return ((a >> 16) & 255u) < ((b >> 16) & 255u);
         });
     }
+    // the terms of one element stay in one thread; the elements, in row and
+    // then column order, are cut into SW runs of at most maxTerms terms
+    // (the smallest such bound: walked in steps of 16, so the bound is a
+    // multiple of 16)
+    std::vector<std::pair<size_t, size_t>> elems;   // [first, last) in flat
+    std::vector<unsigned long long> flat;
+    for (int s = 0; s < ns; ++s)
+        for (size_t e = 0; e < terms[s].size(); ++e) {
+            if (e == 0 || ((terms[s][e] >> 16) & 255u) != ((terms[s][e - 1] >> 16) & 255u))
+                elems.push_back({flat.size(), flat.size()});
+            flat.push_back(terms[s][e]);
+            elems.back().second = flat.size();
+        }
     int maxTerms = 16;
-    for (int s = 0; s < ns; ++s) maxTerms = std::max(maxTerms, (int)terms[s].size());
-    maxTerms = (maxTerms + 15) / 16 * 16;
-    std::vector<unsigned long long> spTerm((size_t)maxTerms * SW, 8ull << 48);   // padding: column 0, nu 0
-    for (int s = 0; s < ns; ++s) {
-        const unsigned long long lastCol = terms[s].empty() ? 0ull : (terms[s].back() >> 16) & 255u;
-        for (int e = 0; e < maxTerms; ++e)
-            spTerm[(size_t)e * SW + s] = e < (int)terms[s].size() ? terms[s][e] : (lastCol << 16 | 8ull << 48);
+    for (const auto& e : elems) maxTerms = std::max(maxTerms, (int)(e.second - e.first + 15) / 16 * 16);
+    std::vector<size_t> runStart;   // first element of each thread's run
+    for (;; maxTerms += 16) {
+        runStart.assign(1, 0);
+        size_t len = 0;
+        for (size_t e = 0; e < elems.size(); ++e) {
+            const size_t n = elems[e].second - elems[e].first;
+            if (len + n > (size_t)maxTerms) {
+                runStart.push_back(e);
+                len = 0;
+            }
+            len += n;
+        }
+        if ((int)runStart.size() <= SW) break;
+    }
+    runStart.push_back(elems.size());
+    // padding after a thread's last element repeats that element's key with
+    // coefficient 0 (the same sum is stored again); a thread without an
+    // element stores into its dump slot
+    const unsigned long long pad = (unsigned long long)ONE << 32 | (unsigned long long)ONE << 40 | 8ull << 48;
+    std::vector<unsigned long long> spTerm((size_t)maxTerms * SW, pad | 1ull << 57);
+    for (size_t t = 0; t + 1 < runStart.size(); ++t) {
+        const size_t f0 = elems[runStart[t]].first, f1 = runStart[t + 1] > runStart[t]
+                                                         ? elems[runStart[t + 1] - 1].second : f0;
+        for (size_t e = 0; e < (size_t)maxTerms; ++e)
+            spTerm[e * SW + t] = f0 + e < f1 ? flat[f0 + e]
+                                             : (f1 > f0 ? (flat[f1 - 1] & 0xffff0000ull) | pad : spTerm[e * SW + t]);
     }
     // flat third-body entry list padded to a multiple of 16 with zero entries on the last row
     const int nTBz = (int)tbzRow.size();
