@@ -607,6 +607,7 @@ def main():
     # every eager step launch on the solver's stream
     lrn.set_launch(0, 0)
     _, launches, step_ms = timed_runs(lrn, 1, 0, profile=True)
+    list_len, grid = lrn.load_report()   # the same profiled run: one list length per step launch
     att_steps = int(rl["nAttempts"][(rl["status"] & 8) == 0].sum())
     avg_launch_s = 1e-3 * step_ms / max(launches, 1)
     achieved = fpa * att_steps / max(launches, 1) / avg_launch_s / 1e12
@@ -662,6 +663,9 @@ def main():
                      "kernel": "k_step<56,Rodas34> (FP64 VALU; one Rosenbrock attempt per active cell)",
                      "flops_per_attempt": fpa, "flops_source": fsrc, "launches": launches,
                      "avg_launch_ms": 1e3 * avg_launch_s},
+        "load_per_wavefront": dict(lib.load_summary(list_len, grid),
+                                   note="step launches of the roofline run (learned batch, eager launches); "
+                                        "a launch strides its active list over the grid's wavefronts"),
     }
     line["host_buffers"] = host
     line["learned_seed_mode"] = {"value": field.n_cells / (wall_s / args.steps),

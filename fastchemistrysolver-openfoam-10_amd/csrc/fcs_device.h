@@ -1101,7 +1101,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
             }
         }
     }
-    wave_sync();   // the walk wrote other threads' rows
+    // (other threads' rows are read after the wave_sync below)
     clk.mark(M, lane == 0, PH_JAC_TERMS);
     // production rate and d(omega)/dT of this lane's species (its reaction
     // list, loads and gathers issued per block of 8 entries)

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step(const DevMech* Mg, DevOpts
     const unsigned int n = *countIn;
     if (blockIdx.x == 0 && lane == 0) *countZero = 0u;
     for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int c = listIn[i];
+        const int c = __builtin_amdgcn_readfirstlane(listIn[i]);   // wave-uniform cell index
         double y = row >= 0 ? a.Yw[row * nc + c] : 0.0;
         double x = a.x[c], dxTry = a.dxTry[c], dx = a.dx[c], dxTry0 = a.dxTry0[c];
         int flags = a.flags[c], nStep = a.nStep[c], nAtt = a.nAtt[c];
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(const DevMech* Mg, 
     const unsigned int n = *countIn;
     if (blockIdx.x == 0 && lane == 0) *countZero = 0u;
     for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int c = listIn[i];
+        const int c = __builtin_amdgcn_readfirstlane(listIn[i]);   // wave-uniform cell index
         double y = row >= 0 ? a.Yw[row * nc + c] : 0.0;
         double y0s = row >= 0 ? a.Y0s[row * nc + c] : 0.0;
         double scale = row >= 0 ? a.Scl[row * nc + c] : o.absTol;
@@ -211,24 +211,27 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(const DevMech* Mg, 
     }
 }
 
-// Verification of the network's proposals, one wavefront per cell of the
-// list.  Row i of P holds [ln h, z_0 .. z_ns-1, z_T]: h is the proposed
-// first sub-step of an integration that starts by trying the whole flow
-// step, and the increments are tolerance-scaled and compressed,
-// d_k = sinh(z_k) (absTol + relTol |y0_k|).  Only cells whose proposed h is
-// at least half the flow step are verified: the proposed end state y0 + d
-// is accepted when one FP64 Rodas34 step over the whole flow step from y0
-// passes its own error test (norm <= 1) and agrees with the proposal to the
-// same tolerance (max-norm of |yT - yP| / (absTol + relTol max(|yT|, |yP|))
-// <= 1); the cell then finishes with the Rodas34 result and the step
-// proposal of an accepted step.  Every other cell joins the output list
-// unchanged, so the integrator then runs on it exactly as without the
-// network -- or, with seed != 0, starting from min(h, deltaT) instead of
-// deltaTChem.
+// Verification of the network's proposals, one wavefront per gated cell
+// (listG: positions in the reacting-cell list `list`, whose proposals have
+// h >= deltaT/2; k_gate sends every other cell on).  Row i of P holds
+// [ln h, z_0 .. z_ns-1, z_T]: h is the proposed first sub-step of an
+// integration that starts by trying the whole flow step, and the increments
+// are tolerance-scaled and compressed, d_k = sinh(z_k) (absTol + relTol
+// |y0_k|).  The proposed end state y0 + d is accepted when one FP64 Rodas34
+// step over the whole flow step from y0 passes its own error test
+// (norm <= 1) and agrees with the proposal to the same tolerance (max-norm of
+// |yT - yP| / (absTol + relTol max(|yT|, |yP|)) <= 1); the cell then finishes
+// with the Rodas34 result and the step proposal of an accepted step.  A
+// rejected cell joins the output list unchanged, so the integrator then runs
+// on it exactly as without the network -- or, with seed != 0, starting from
+// min(h, deltaT) instead of deltaTChem.  The attempt sits in straight-line
+// code of the cell loop: under a conditional branch the compiled attempt
+// has produced wrong error norms on this device (the same code without the
+// branch, and in every other kernel, is exact).
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_verify(const DevMech* Mg, DevOpts o, CellArrays a, const int* listIn,
-                                                        const unsigned int* countIn, const float* P, int seed,
-                                                        int* listOut, unsigned int* countOut,
+__global__ __launch_bounds__(Cell<NP>::G) void k_verify(const DevMech* Mg, DevOpts o, CellArrays a, const int* list,
+                                                        const int* listG, const unsigned int* countG, const float* P,
+                                                        int seed, int* listOut, unsigned int* countOut,
                                                         unsigned long long* nAccepted)
 {
     extern __shared__ double lds_raw[];
@@ -238,38 +241,32 @@ __global__ __launch_bounds__(Cell<NP>::G) void k_verify(const DevMech* Mg, DevOp
     const int ns = M.ns;
     const size_t nc = (size_t)a.nc;
     const int row = lane < ns ? lane : (lane == NP - 1 ? ns : -1);
-    const unsigned int n = *countIn;
-    for (unsigned int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int c = listIn[i];
+    const unsigned int n = *countG;
+    for (unsigned int g = blockIdx.x; g < n; g += gridDim.x) {
+        const int i = __builtin_amdgcn_readfirstlane(listG[g]);   // wave-uniform
+        const int c = __builtin_amdgcn_readfirstlane(list[i]);
         const float* pr = P + (size_t)i * (ns + 2);
         const double dt = a.dtCell ? a.dtCell[c] : a.dt;
         double y0 = row >= 0 ? a.Yw[row * nc + c] : 0.0;
         const double h = exp((double)pr[0]);
-        bool accept = false;
-        if (h >= 0.5 * dt) {   // uniform: one value per cell
-            if (lane == 0) atomicAdd(nAccepted + 1, 1ull);
-            const double yP = row >= 0 ? y0 + sinh((double)pr[1 + row]) * (o.absTol + o.relTol * fabs(y0)) : 0.0;
-            double yT = 0.0;
-            const double p = a.p0[c];
-            stage_plog<NP>(M, L, log(p));
-            const double err = rosenbrock_attempt<NP>(M, L, o, kRodas34, p, log(p), dt, y0, yT);
-            double r = 0.0;
-            if (row >= 0) {
-                const double m = fabs(yT) > fabs(yP) ? fabs(yT) : fabs(yP);
-                r = fabs(yT - yP) / (o.absTol + o.relTol * m);
-            }
-            const bool bad = gany<NP>(!(r == r), L);
-            const double agree = gmax<NP>(r, L);
-            accept = (err <= 1.0) && !bad && (agree <= 1.0);
-            if (accept) {
-                finish_cell<NP>(M, o, L, a, c, FCS_CELL_LEARNED, yT, accepted_dxTry(err, dt), 1);
-                if (lane == 0) atomicAdd(nAccepted, 1ull);
-            }
+        const double yP = row >= 0 ? y0 + sinh((double)pr[1 + row]) * (o.absTol + o.relTol * fabs(y0)) : 0.0;
+        double yT = 0.0;
+        const double p = a.p0[c];
+        stage_plog<NP>(M, L, log(p));
+        const double err = rosenbrock_attempt<NP>(M, L, o, kRodas34, p, log(p), dt, y0, yT);
+        double r = 0.0;
+        if (row >= 0) {
+            const double m = fabs(yT) > fabs(yP) ? fabs(yT) : fabs(yP);
+            r = fabs(yT - yP) / (o.absTol + o.relTol * m);
         }
-        if (!accept && lane == 0) {
-            // a proposal that is not a positive number (NaN, or exp underflow)
-            // leaves the cell's deltaTChem in place
-            if (seed && h == h && h > 0.0) a.dxTry[c] = h < dt ? h : dt;
+        const bool bad = gany<NP>(!(r == r), L);
+        const double agree = gmax<NP>(r, L);
+        const bool accept = (err <= 1.0) && !bad && (agree <= 1.0);
+        if (accept) {
+            finish_cell<NP>(M, o, L, a, c, FCS_CELL_LEARNED, yT, accepted_dxTry(err, dt), 1);
+            if (lane == 0) atomicAdd(nAccepted, 1ull);
+        } else if (lane == 0) {
+            if (seed) a.dxTry[c] = h < dt ? h : dt;   // h >= deltaT/2 > 0 here
             listOut[atomicAdd(countOut, 1u)] = c;
         }
     }

@@ -23,7 +23,7 @@
 // rather than memset nodes, so the sequence can be captured in a graph).
 __global__ void k_reset(unsigned int* counts, unsigned long long* dtMinBits, unsigned long long* nAccepted)
 {
-    counts[0] = counts[1] = counts[2] = 0u;
+    counts[0] = counts[1] = counts[2] = counts[3] = 0u;   // [3]: gated proposals of the learned seed
     *dtMinBits = 0x7ff0000000000000ull;  // +inf
     nAccepted[0] = nAccepted[1] = 0ull;   // accepted, verified
 }
@@ -99,6 +99,30 @@ __global__ void k_features(int ns, CellArrays a, const int* list, const unsigned
     x[1] = (float)log(a.p0[c]);
     for (int s = 0; s < ns; ++s) x[2 + s] = (float)a.Yw[(size_t)s * nc + c];
     x[ns + 2] = (float)log(a.dtCell ? a.dtCell[c] : a.dt);
+}
+
+// The learned seed's gate, one thread per entry of the reacting-cell list:
+// a proposal whose first sub-step h is at least half the flow step goes to
+// the gated list (its position in list0) for verification; every other cell
+// joins list 1 at once, in seed mode starting from min(h, deltaT) when h is a
+// positive number.  Keeping the gate out of the verification kernel leaves
+// the verification attempt outside any branch.
+__global__ void k_gate(int ns, CellArrays a, const int* list, const unsigned int* count, const float* P, int seed,
+                       int* listG, unsigned int* countG, int* listOut, unsigned int* countOut,
+                       unsigned long long* nAccepted)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const int c = list[i];
+    const double dt = a.dtCell ? a.dtCell[c] : a.dt;
+    const double h = exp((double)P[(size_t)i * (ns + 2)]);
+    if (h >= 0.5 * dt) {
+        listG[atomicAdd(countG, 1u)] = (int)i;
+        atomicAdd(nAccepted + 1, 1ull);
+        return;
+    }
+    if (seed && h == h && h > 0.0) a.dxTry[c] = h < dt ? h : dt;
+    listOut[atomicAdd(countOut, 1u)] = c;
 }
 
 // FastChemistryModel::Qdot (src/ChemistryModel/FastChemistryModel.C:
@@ -186,7 +210,7 @@ struct DevCells {
            *RR = nullptr, *Tend = nullptr, *Yw = nullptr, *x = nullptr, *dxTry = nullptr, *dx = nullptr,
            *dxTry0 = nullptr, *Y0s = nullptr, *Scl = nullptr, *theta = nullptr, *Jy = nullptr;
     int *status = nullptr, *nAtt = nullptr, *flags = nullptr, *nStep = nullptr, *list0 = nullptr, *list1 = nullptr,
-        *kTarg = nullptr;
+        *kTarg = nullptr, *listG = nullptr;   // listG: gated proposals (positions in list0) of the learned seed
     size_t jyPerCell = 0;                    // doubles of Jy per cell (Seulex), 0 otherwise
     double* Jg = nullptr;                    // workgroup form: per-workgroup matrix slices
     size_t jgDoubles = 0;
@@ -208,6 +232,9 @@ struct DevCells {
     std::vector<hipEvent_t> pev;
     long long lastLaunches = 0;
     double lastStepMs = 0.0;
+    // profiling: the active-list length each step launch of the last run
+    // read (eager launches), for the per-wavefront load report
+    std::vector<long long> listLen;
 
     // jyDoubles: per-cell Jacobian storage of the Seulex solver (0 for the
     // Rosenbrock solvers).
@@ -217,7 +244,7 @@ struct DevCells {
         const size_t d = align256(nc * 8), i = align256(nc * 4);
         const size_t nsd = align256(nc * nsp * 8), nwd = align256(nc * (nsp + 1) * 8);
         const size_t jyb = align256(nc * jyDoubles * 8);
-        const size_t total = 13 * d + 2 * nsd + 3 * nwd + 7 * i + jyb + 3 * 256;
+        const size_t total = 13 * d + 2 * nsd + 3 * nwd + 8 * i + jyb + 3 * 256;
         if (hipMalloc(&base, total) != hipSuccess) {
             base = nullptr;
             return FCS_E_NOMEM;
@@ -231,7 +258,7 @@ struct DevCells {
         Y0 = (double*)take(nsd); RR = (double*)take(nsd); Yw = (double*)take(nwd);
         Y0s = (double*)take(nwd); Scl = (double*)take(nwd);
         status = (int*)take(i); nAtt = (int*)take(i); flags = (int*)take(i); nStep = (int*)take(i);
-        list0 = (int*)take(i); list1 = (int*)take(i); kTarg = (int*)take(i);
+        list0 = (int*)take(i); list1 = (int*)take(i); kTarg = (int*)take(i); listG = (int*)take(i);
         Jy = jyDoubles ? (double*)take(jyb) : nullptr;
         jyPerCell = jyDoubles;
         counts = (unsigned int*)take(256);
@@ -902,7 +929,12 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
             int seed = learnedMode == 2 ? 1 : 0;
             unsigned int* c0 = d.counts;
             unsigned int* c1 = d.counts + 1;
-            void* args[] = {&M, &o, &a, &d.list0, &c0, &d.Pf, &seed, &d.list1, &c1, &d.nAccepted};
+            unsigned int* cG = d.counts + 3;
+            {   // proposals with h >= deltaT/2 to the gated list, the others straight to list 1
+                void* args[] = {&ns, &a, &d.list0, &c0, &d.Pf, &seed, &d.listG, &cG, &d.list1, &c1, &d.nAccepted};
+                HIP_OK(hipLaunchKernel((const void*)k_gate, dim3((n0 + 255) / 256), dim3(256), args, 0, st));
+            }
+            void* args[] = {&M, &o, &a, &d.list0, &d.listG, &cG, &d.Pf, &seed, &d.list1, &c1, &d.nAccepted};
             HIP_OK(hipLaunchKernel(s->ks->verify, dim3(std::max(1, std::min(grid, (int)n0))), dim3(blk), args,
                                    s->ldsBytes, st));
             it0 = 1;
@@ -1005,6 +1037,13 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
         HIP_OK(hipStreamSynchronize(st));
         return FCS_OK;
     }
+    d.listLen.clear();
+    if (d.profile) {   // the first launch's list length (later ones come with the polls)
+        unsigned int* c0 = d.counts + it0 % 3;
+        HIP_OK(hipMemcpyAsync(d.hostCount, c0, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        d.listLen.push_back(d.hostCount[0]);
+    }
     for (long long k = 0;; ++k) {
         const long long it = k + it0;
         int *lin, *lout;
@@ -1030,6 +1069,7 @@ static int run_cells(fcs_solver* s, DevCells& d, int nc, double dt, bool lts = f
             const int bs = (int)(back % DevCells::kPoll);
             HIP_OK(hipEventSynchronize(d.ev[bs]));
             if (d.hostCount[bs] == 0u) break;
+            if (d.profile) d.listLen.push_back(d.hostCount[bs]);   // list read by launch back + 1
         }
         if (k > maxLaunches) return abandon(lout, cout);
     }
@@ -1286,6 +1326,17 @@ int fcs_batch_stats(fcs_batch* b, int64_t* stepLaunches, double* stepKernelMs)
     if (!b) return FCS_E_ARG;
     if (stepLaunches) *stepLaunches = b->cells.lastLaunches;
     if (stepKernelMs) *stepKernelMs = b->cells.lastStepMs;
+    return FCS_OK;
+}
+
+int fcs_batch_load_report(fcs_batch* b, int64_t* listLen, int32_t cap, int32_t* nLaunches, int32_t* grid)
+{
+    if (!b || cap < 0 || (cap > 0 && !listLen)) return FCS_E_ARG;
+    const DevCells& d = b->cells;
+    const int n = (int)d.listLen.size();
+    for (int i = 0; i < n && i < cap; ++i) listLen[i] = d.listLen[i];
+    if (nLaunches) *nLaunches = n;
+    if (grid) *grid = std::max(1, std::min(d.gridOverride > 0 ? d.gridOverride : b->s->grid, (int)b->nc));
     return FCS_OK;
 }
 

@@ -21,7 +21,8 @@ _vp = C.c_void_p
 EXPORTS = [
     "fcs_device_count", "fcs_version", "fcs_mech_create", "fcs_mech_destroy", "fcs_solver_create",
     "fcs_solver_destroy", "fcs_solve", "fcs_solve_ex", "fcs_solve_lts", "fcs_batch_create", "fcs_batch_destroy",
-    "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_download",
+    "fcs_batch_upload", "fcs_batch_run", "fcs_batch_profile", "fcs_batch_stats", "fcs_batch_load_report",
+    "fcs_batch_download",
     "fcs_eval_derivatives", "fcs_eval_jacobian", "fcs_eval_attempt", "fcs_lu_solve",
     "fcs_batch_set_learned", "fcs_batch_learned_stats", "fcs_batch_first_steps", "fcs_batch_set_launch",
     "fcs_batch_autotune", "fcs_calculate", "fcs_tc", "fcs_qdot", "fcs_solve_noalloc",
@@ -68,6 +69,8 @@ def load(path: str | None = None):
     L.fcs_batch_download.argtypes = [_vp, _dp, _dp, _dp, _ip, _dp, _ip]
     L.fcs_batch_profile.argtypes = [_vp, C.c_int]
     L.fcs_batch_stats.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.fcs_batch_load_report.argtypes = [_vp, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]
     L.fcs_eval_derivatives.argtypes = [_vp, C.c_int64, _dp, _dp, _dp]
     L.fcs_eval_jacobian.argtypes = [_vp, C.c_int64, _dp, _dp, _dp, _dp]
     L.fcs_lu_solve.argtypes = [_vp, C.c_int64, _dp, _dp]
@@ -271,6 +274,16 @@ class Batch:
         check(load().fcs_batch_stats(self._h, C.byref(n), C.byref(ms)), "fcs_batch_stats")
         return n.value, ms.value
 
+    def load_report(self):
+        """(list length read by each step launch of the last profiled eager
+        run, step-launch grid)."""
+        n, g = C.c_int32(0), C.c_int32(0)
+        check(load().fcs_batch_load_report(self._h, None, 0, C.byref(n), C.byref(g)), "fcs_batch_load_report")
+        out = np.zeros(n.value, np.int64)
+        check(load().fcs_batch_load_report(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), n.value, C.byref(n),
+                                           C.byref(g)), "fcs_batch_load_report")
+        return out, g.value
+
     def set_launch(self, grid=0, graph_launches=0):
         """Step-kernel grid (0: occupancy x CUs) and step launches per
         captured hipGraph (0: eager launches)."""
@@ -382,6 +395,31 @@ class Mlp:
         check(load().fcs_mlp_train(self._h, X.shape[0], _fptr(X), _fptr(T), int(epochs), int(batch), float(lr),
                                    int(seed), _fptr(hist)), "fcs_mlp_train")
         return hist
+
+
+def wavefront_loads(list_len, grid):
+    """Attempts each step-kernel wavefront made over a run, from the list
+    length n of every step launch (Batch.load_report): the launch strides
+    its n cells over the grid, so wavefront w takes cells w, w + grid, ...,
+    ceil((n - w) / grid) of them (0 when n <= w)."""
+    n = np.asarray(list_len, np.int64)[:, None]
+    w = np.arange(int(grid), dtype=np.int64)[None, :]
+    return np.maximum(0, (n - w + grid - 1) // grid).sum(axis=0)
+
+
+def load_summary(list_len, grid):
+    """Per-wavefront load figures of a run: attempts per wavefront (min,
+    mean, max), the launches whose list holds fewer cells than the grid has
+    wavefronts (the tail, where wavefronts idle) and their share of the
+    launches and of the attempts."""
+    n = np.asarray(list_len, np.int64)
+    loads = wavefront_loads(n, grid)
+    tail = n < grid
+    return {"grid": int(grid), "launches": int(n.size), "attempts": int(n.sum()),
+            "attempts_per_wavefront": {"min": int(loads.min()), "mean": float(loads.mean()),
+                                       "max": int(loads.max())},
+            "tail_launches": int(tail.sum()), "tail_attempt_share": float(n[tail].sum() / max(n.sum(), 1)),
+            "mean_busy_wavefronts_per_launch": float(np.minimum(n, grid).mean()) if n.size else 0.0}
 
 
 def equilibrator(solver: "Solver", deltaT=4e-3):

@@ -248,6 +248,15 @@ int fcs_batch_run(fcs_batch* batch, double deltaT, float* elapsed_ms);
  * and the summed device duration of its step kernels (0 when disabled). */
 int fcs_batch_profile(fcs_batch* batch, int enable);
 int fcs_batch_stats(fcs_batch* batch, int64_t* stepLaunches, double* stepKernelMs);
+/* Per-wavefront load of the last profiled run (no counterpart in the
+ * reference; it reports what its MPI load balancer,
+ * FastChemistryModel_transientSolve.H:106-686, evens out): the active-list
+ * length each step launch read, in launch order (at most cap of them; the
+ * count in *nLaunches), and the grid of the step launches.  A launch over n
+ * cells gives wavefront w (0 <= w < grid) ceil((n - w) / grid) attempts, so
+ * the lengths fix every wavefront's load.  Recorded with fcs_batch_profile
+ * on (eager launches); empty otherwise. */
+int fcs_batch_load_report(fcs_batch* batch, int64_t* listLen, int32_t cap, int32_t* nLaunches, int32_t* grid);
 int fcs_batch_download(fcs_batch* batch, double* RR /* [ns*nCells] */,
                        double* deltaTChem, double* deltaTMin,
                        int32_t* cellStatus, double* Tend, int32_t* nAttempts);
