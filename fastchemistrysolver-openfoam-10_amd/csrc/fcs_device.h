@@ -1273,7 +1273,8 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 // reference swaps rows, and ip[r] is the lane holding logical row r when the
 // factorisation ends.  Right-looking elimination; a pivot row's entry is
 // broadcast with readlane.
-struct LuState { double invD; int pos; int ipl; };   // ipl: lane holding logical row `lane`
+// invD: reciprocal pivot of logical row `lane`; ipl: lane that held logical row `lane` during the factorisation
+struct LuState { double invD; int pos; int ipl; };
 
 template <int NP>
 __device__ inline LuState lu_factor(const DevMech& M, const Lds& L)
@@ -1365,40 +1366,47 @@ __device__ inline LuState lu_factor(const DevMech& M, const Lds& L)
         }
         clk.mark(M, lane == 0, PH_LU_TRAIL);
     }
+    // factors stored in logical row order (row pos at slot pos of every
+    // column), the reciprocal pivot in the diagonal slot, so the solves read
+    // logical row r from lane r
+    if (row) {
 #pragma unroll
-    for (int j = 0; j < NP; ++j) Jrow[j * JS] = R[j];
-    if (row) L.ip[pos] = lane;
+        for (int j = 0; j < NP; ++j) L.J[j * JS + pos] = j == pos ? invD : R[j];
+        L.ip[pos] = lane;
+    }
     wave_sync();
     const int ipl = row ? L.ip[lane] : lane;
+    const double invDl = row ? Jrow[lane * JS] : 0.0;   // reciprocal pivot of logical row `lane`
     wave_sync();
-    return LuState{invD, pos, ipl};
+    return LuState{invDl, pos, ipl};
 }
 
 // Solves (L U) x = P b; b is this lane's element in the original row order
 // (lane i = row i); returns x_i for lane i (src/LUsolver/LUsolver.C:
-// 3922-4111: swaps, unit-L forward, U backward scaled by invD).
+// 3922-4111: swaps, unit-L forward, U backward scaled by 1/pivot).  The
+// factors are in logical row order (lu_factor), so after one permuting
+// exchange of b every step reads a fixed lane: y_r = b_r, then every lane
+// below subtracts its L entry times y_r; backward x_r = b_r / u_rr and every
+// lane above subtracts its U entry times x_r.  A lane's entries on the other
+// side of the diagonal enter as zero multipliers.
 template <int NP>
 __device__ inline double lu_solve(const Lds& L, const LuState& st, double b)
 {
     const int lane = lane_id();
     const double* Jrow = L.J + lane;
+    b = __shfl(b, st.ipl, WAVE);                  // lane r: b of logical row r
 #pragma unroll
     for (int r = 0; r < NP - 1; ++r) {
-        const int src = readlane_i(st.ipl, r);
-        const double yr = readlane_d(b, src);
-        const double lr = Jrow[r * JS];              // unconditional: prefetchable
-        const double nb = b - lr * yr;
-        b = (st.pos > r && lane < NP) ? nb : b;
+        const double yr = readlane_d(b, r);
+        const double lr = Jrow[r * JS];
+        b = b - (lane > r ? lr : 0.0) * yr;
     }
     double x = 0.0;
 #pragma unroll
     for (int r = NP - 1; r >= 0; --r) {
-        const int src = readlane_i(st.ipl, r);
-        if (lane == src) b *= st.invD;
-        const double xr = readlane_d(b, src);
+        const double xr = readlane_d(b * st.invD, r);
         const double ur = Jrow[r * JS];
-        const double nb = b - ur * xr;
-        b = st.pos < r ? nb : b;
+        b = b - (lane < r ? ur : 0.0) * xr;
         if (lane == r) x = xr;
     }
     return x;
