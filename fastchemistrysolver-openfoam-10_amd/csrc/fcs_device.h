@@ -59,6 +59,10 @@ struct Cell {
     static constexpr int ONE = kLarge ? 255 : 64;                 // concentration slot holding 1.0
     static constexpr int CS = kLarge ? 256 : 65;                  // concentration slots
     static constexpr int JSt = kLarge ? NP : JS;                  // matrix column stride
+    // wavefronts per SIMD the step kernel is compiled for: two for the
+    // HBM-matrix workgroup form (two cells share a CU, hiding the matrix
+    // round trips of one behind the other; registers capped at 256)
+    static constexpr int kMinWaves = (kLarge && !kJLds) ? 2 : 1;
 };
 
 enum { RXN_ARR = 0, RXN_TB = 1, RXN_LIND = 2, RXN_TROE = 3, RXN_PLOG = 4, RXN_SRI = 5 };

@@ -107,7 +107,7 @@ __device__ void finish_cell(const DevMech& M, const DevOpts& o, const Lds& L, co
 // counter *countZero (the list after next) is cleared for the next launch.
 // Every loop here is bounded by the list length read at entry.
 template <int NP, int SOLVER>
-__global__ __launch_bounds__(Cell<NP>::G) void k_step(const DevMech* Mg, DevOpts o, CellArrays a, const int* listIn,
+__global__ __launch_bounds__(Cell<NP>::G, Cell<NP>::kMinWaves) void k_step(const DevMech* Mg, DevOpts o, CellArrays a, const int* listIn,
                                                       const unsigned int* countIn, int* listOut,
                                                       unsigned int* countOut, unsigned int* countZero)
 {
