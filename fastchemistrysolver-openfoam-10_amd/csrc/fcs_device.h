@@ -116,6 +116,7 @@ struct DevMech {
     const int* tbz_row;       // [nTBz] species, ascending (padding repeats the last)
     const int* tbz_tb;        // [nTBz] third-body row (padding: 0)
     const double* tbz_nu;     // [nTBz] net stoichiometric coefficient (padding: 0)
+    const int* tbz_blk;       // [NPm/16 + 1] first entry of each 16-row block (HBM-matrix form)
     const unsigned long long* rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
     const unsigned long long* sp_pack;  // [maxDeg*SW] rpack of each per-species list entry
     int maxTerms;             // Jacobian terms per thread (jacobian's sparse-partial walk), multiple of 16
@@ -273,6 +274,7 @@ struct Lds {
     double* J;     // [NP*JSt] Jacobian, then iteration matrix and its LU factors
                    // (LDS; for the workgroup form a per-workgroup slice in HBM
                    // of NP*NP + NP doubles, the last NP holding 1/pivot)
+    double* tbt;   // workgroup form, matrix in HBM: [16][NP+1] tile of third-body partials
     double* plog;  // [3*nPlog] per-cell PLOG rows as Arrhenius (ln A, beta, Ta) at the
                    // cell's pressure (stage_plog), kept for the whole kernel
     int* ip;       // [G] thread holding each logical row after the LU
@@ -300,6 +302,11 @@ __host__ __device__ inline int group_of(int NP) { return NP > 64 ? (NP <= 128 ? 
 // each).  Workgroup form: the
 // LU scratch reuses the reaction arrays; the matrix (and its reciprocal
 // pivots) follows them in LDS up to kJLdsMaxNP rows, else it is in HBM.
+// Tile of the third-body partials in the HBM-matrix workgroup form (16 rows
+// by NP columns, row stride NP + 1 so that a column's 16 entries fall in
+// different banks).
+__host__ __device__ inline size_t tbt_doubles(int NP) { return NP > 64 && !jlds_of(NP) ? 16 * ((size_t)NP + 1) : 0; }
+
 // nrq: slots of the per-reaction arrays (DevMech::nrq); nPlog: PLOG rows.
 __host__ __device__ inline size_t lds_doubles(int NP, int nrq, int nTB, int nPlog)
 {
@@ -314,7 +321,8 @@ __host__ __device__ inline size_t lds_doubles(int NP, int nrq, int nTB, int nPlo
     const size_t react = nrP + kq + 2 * (size_t)round_up(nTB > 0 ? nTB : 1, 2);
     const size_t lu = lu_scratch_doubles(NP);
     const size_t rest = 256 + 2 * G + (react > lu ? react : lu) + G / 2 + 16 + plP;
-    return (jlds_of(NP) ? ((rest + 1) & ~(size_t)1) + (size_t)NP * NP + NP : rest) + kMechLdsDoubles;
+    return (jlds_of(NP) ? ((rest + 1) & ~(size_t)1) + (size_t)NP * NP + NP : rest + tbt_doubles(NP)) +
+           kMechLdsDoubles;
 }
 
 __device__ inline Lds make_lds(double* base, int NP, int nrq, int nTB, int nPlog, double* Jglobal = nullptr)
@@ -343,6 +351,7 @@ __device__ inline Lds make_lds(double* base, int NP, int nrq, int nTB, int nPlog
         L.plog = L.red + 16;   // past the LU scratch, which reuses rq .. Mtb
         const size_t rest = 256 + 2 * (size_t)G + (react > lu ? react : lu) + G / 2 + 16 + plP;
         L.J = jlds_of(NP) ? base + ((rest + 1) & ~(size_t)1) : Jglobal;
+        L.tbt = jlds_of(NP) ? nullptr : base + rest;
         return L;
     }
     const int nrP = round_up(nrq, 2), nTBP = round_up(nTB > 0 ? nTB : 1, 2);
@@ -362,6 +371,7 @@ __device__ inline Lds make_lds(double* base, int NP, int nrq, int nTB, int nPlog
     L.ip = (int*)(L.J + (size_t)NP * JS);
     L.lu = nullptr;
     L.red = nullptr;
+    L.tbt = nullptr;
     return L;
 }
 
@@ -1163,7 +1173,53 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     // seen in this batch still holds its sparse-partial value (rows are
     // visited once, in ascending order), and the value read for a row
     // continuing from the last batch is not used.
-    if (M.nTBz > 0) {
+    if constexpr (Cell<NP>::kLarge && !Cell<NP>::kJLds) {
+        // matrix in HBM, where a column-wise read-modify-write per row reads
+        // one cache line per thread: the rows go in blocks of 16 -- each
+        // thread sums its column's partials of the block's rows into an LDS
+        // tile, then the group adds the tile to the slice with 16 threads on
+        // the 16 rows of one column (one 128-byte line), all lines in flight
+        constexpr int TS = NP + 1;
+        for (int b = 0; b < NP / 16; ++b) {
+            const int e0 = __builtin_amdgcn_readfirstlane(M.tbz_blk[b]);
+            const int e1 = __builtin_amdgcn_readfirstlane(M.tbz_blk[b + 1]);
+            if (e0 == e1) continue;   // no entry in this block (uniform)
+            if (lane < NP)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) L.tbt[q * TS + lane] = 0.0;
+            double acc = 0.0;
+            int cur = -1;
+            for (int k0 = e0; k0 < e1; k0 += 16) {
+                const int e = k0 + (lane & 15) < M.nTBz ? k0 + (lane & 15) : M.nTBz - 1;
+                const int vt = M.tbz_tb[e], vrow = M.tbz_row[e];
+                const double vnu = M.tbz_nu[e];
+                int rw_[16];
+                double cf[16], ef[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const bool in = k0 + u < e1;   // uniform
+                    const int t = readlane_i(vt, u);
+                    rw_[u] = in ? readlane_i(vrow, u) : -2;   // -2: past the block, adds nothing
+                    cf[u] = in ? readlane_d(vnu, u) * L.rw[t] : 0.0;
+                    ef[u] = M.eff[t * SW + lane];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    if (rw_[u] != cur && rw_[u] != -2) {
+                        if (cur >= 0 && lane < NP) L.tbt[(cur - 16 * b) * TS + lane] = acc;
+                        acc = 0.0;
+                        cur = rw_[u];
+                    }
+                    acc += cf[u] * ef[u];
+                }
+            }
+            if (cur >= 0 && lane < NP) L.tbt[(cur - 16 * b) * TS + lane] = acc;
+            __syncthreads();
+            const int i = 16 * b + (lane & 15);
+            for (int j = lane >> 4; j < NP; j += G / 16) L.J[(size_t)j * NP + i] += L.tbt[(lane & 15) * TS + j];
+            __syncthreads();   // the tile is cleared by the next block
+        }
+    } else if (M.nTBz > 0) {
         double acc = 0.0, oldv = 0.0;
         int cur = -1;
         int nt = M.tbz_tb[lane & 15], nrow = M.tbz_row[lane & 15];

@@ -677,7 +677,14 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         tbzNu.push_back(0.0);
     }
     const size_t iSpRN = vi(spRN), iSpTerm = add(spTerm.data(), spTerm.size() * 8);
-    const size_t iTbzRow = vi(tbzRow), iTbzTb = vi(tbzTb), iTbzNu = vd(tbzNu);
+    // first entry of each 16-row block (rows ascending; the padding repeats the last row)
+    std::vector<int> tbzBlk(SW / 16 + 1, (int)tbzRow.size());
+    for (int b = SW / 16 - 1; b >= 0; --b) {
+        int e = tbzBlk[b + 1];
+        while (e > 0 && tbzRow[e - 1] >= 16 * b) --e;
+        tbzBlk[b] = nTBz > 0 ? e : 0;
+    }
+    const size_t iTbzRow = vi(tbzRow), iTbzTb = vi(tbzTb), iTbzNu = vd(tbzNu), iTbzBlk = vi(tbzBlk);
     auto vq = [&](const std::vector<unsigned long long>& v) { return add(v.data(), v.size() * 8); };
     const size_t iRpack = vq(rpack), iSpPack = vq(spPack);
     const size_t iRgen = vi(rgen), iGen = vi(gen);
@@ -754,6 +761,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.sp_term = (const unsigned long long*)P(iSpTerm);
     dm.nTBz = nTBz > 0 ? (int)tbzRow.size() : 0;
     dm.tbz_row = (const int*)P(iTbzRow);
+    dm.tbz_blk = (const int*)P(iTbzBlk);
     dm.tbz_tb = (const int*)P(iTbzTb);
     dm.tbz_nu = (const double*)P(iTbzNu);
     dm.rpack = (const unsigned long long*)P(iRpack);
