@@ -296,10 +296,11 @@ def phases(args):
     from fcs.mechanism import config_field, config_mechanism
     L = lib.load(build.LIB_PHASES)
     L.fcs_phase_read.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
-    mech = config_mechanism(2)
+    cfg = args.phase_config   # 2: the default 53-species field; 4: the 160-species one (workgroup form)
+    mech = config_mechanism(cfg)
     gm = lib.Mech(mech)
     solver = lib.Solver(gm, solver="OptRodas34", jacobian="exact", absTol=1e-8, relTol=1e-4, Treact=TREACT)
-    field = config_field(2, mech, n_cells=args.phase_side ** 2, equilibrate=lib.equilibrator(solver))
+    field = config_field(cfg, mech, n_cells=args.phase_side ** 2, equilibrate=lib.equilibrator(solver))
     b = lib.Batch(solver, field)
     b.run(DT)
     buf = (C.c_uint64 * 24)()
@@ -311,8 +312,10 @@ def phases(args):
     att = int(b.download()["nAttempts"].sum())
     cyc = [buf[i] / att for i in range(len(PHASES))]
     tot = sum(cyc)
-    print(json.dumps({"workload": f"phase split of k_step<56,Rodas34>, plain Rodas34 on a {args.phase_side}x"
-                                  f"{args.phase_side} counterflow field (profiling build)",
+    kern = "k_step<56,Rodas34>" if cfg != 4 else "k_step<176,Rodas34> (workgroup form)"
+    print(json.dumps({"workload": f"phase split of {kern}, plain Rodas34 on a {args.phase_side}x"
+                                  f"{args.phase_side} counterflow field of the {mech.n_species}-species mechanism "
+                                  f"(profiling build)",
                       "attempts": att, "wall_s": wall, "cycles_per_attempt": tot,
                       "phases": {n: {"cycles": c, "share": c / tot} for n, c in zip(PHASES, cyc)},
                       "lu_parts": {n: {"cycles": buf[15 + i] / att, "share": buf[15 + i] / att / tot}
@@ -491,6 +494,8 @@ def main():
                          "tutorials: plain Rodas34 on the reference tutorials' mechanisms; "
                          "phases: per-phase cycles of the step kernel (profiling build libfcs_phases.so)")
     ap.add_argument("--phase-side", type=int, default=128, help="side of the phases workload's field")
+    ap.add_argument("--phase-config", type=int, default=2, choices=(2, 4),
+                    help="mechanism of the phases workload (2: 53 species, 4: 160 species)")
     ap.add_argument("--large-cells", type=int, default=131072, help="cells of the large workload (x 512)")
     ap.add_argument("--cpu-cells", type=int, default=4096,
                     help="cells of the CPU baseline's stratified sample (headline field)")
