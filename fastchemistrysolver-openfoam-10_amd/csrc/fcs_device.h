@@ -1430,8 +1430,10 @@ __device__ inline LuState lu_factor(const DevMech& M, const Lds& L)
     // column), the reciprocal pivot in the diagonal slot, so the solves read
     // logical row r from lane r
     if (row) {
+        double* Jlog = L.J + pos;
 #pragma unroll
-        for (int j = 0; j < NP; ++j) L.J[j * JS + pos] = j == pos ? invD : R[j];
+        for (int j = 0; j < NP; ++j) Jlog[j * JS] = R[j];
+        Jlog[pos * JS] = invD;   // the diagonal slot holds 1/pivot
         L.ip[pos] = lane;
     }
     wave_sync();
