@@ -69,6 +69,27 @@ enum { RXN_ARR = 0, RXN_TB = 1, RXN_LIND = 2, RXN_TROE = 3, RXN_PLOG = 4, RXN_SR
 enum { SOLVER_RODAS34 = 0, SOLVER_ROS34 = 1, SOLVER_SEULEX = 2, SOLVER_RODAS23 = 3, SOLVER_ROS23 = 4 };
 enum { CELL_SKIPPED = 1, CELL_MAXSTEPS = 2, CELL_NONFINITE = 4 };
 
+// Pointer to a read-only table in global memory.  Kernels read the tables
+// through a copy of DevMech in LDS, and a plain pointer loaded from there is
+// generic: its reads compile to flat loads, which count against the LDS
+// counter too, so every wait for an LDS read would also wait for them.
+// Reads through address space 1 are global loads, tracked by the vector
+// memory counter alone.
+template <class T>
+struct gptr {
+    T* p;
+    __host__ __device__ gptr& operator=(T* q)
+    {
+        p = q;
+        return *this;
+    }
+    __device__ __forceinline__ T operator[](long long i) const
+    {
+        return ((const __attribute__((address_space(1))) T*)p)[i];
+    }
+    __host__ __device__ gptr operator+(long long i) const { return gptr{p + i}; }
+};
+
 // Device-resident mechanism tables (built on the host by fcs_mech_create).
 // Reactions are sorted by rate form so that lanes of one reaction round
 // mostly take the same branch.
@@ -81,48 +102,48 @@ struct DevMech {
     double TlowMin, ThighMax, TcMin, TcMax;
     // per-species tables are padded to SW = Cell<NP>::SW species (64, or the
     // workgroup size): "[SW]" below
-    const double* W;          // [SW]  (0 beyond ns)
-    const double* invW;       // [SW]  (0 beyond ns)
-    const double* Tc;         // [SW]
-    const double* clo;        // [7*SW] coefficient-major: c[k*SW + s]
-    const double* chi;        // [7*SW]
-    const int* rtype;         // [nr]
-    const int* rrev;          // [nr]  0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius in rA0/rbeta0/rTa0,
+    gptr<const double> W;          // [SW]  (0 beyond ns)
+    gptr<const double> invW;       // [SW]  (0 beyond ns)
+    gptr<const double> Tc;         // [SW]
+    gptr<const double> clo;        // [7*SW] coefficient-major: c[k*SW + s]
+    gptr<const double> chi;        // [7*SW]
+    gptr<const int> rtype;         // [nr]
+    gptr<const int> rrev;          // [nr]  0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius in rA0/rbeta0/rTa0,
                               //       3 kf/Kc over a product side of more than 3 slots (gen row rgen[r])
-    const int* rnl;           // [nr]
-    const int* rnr;           // [nr]
-    const int* rsp;           // [6*nr] slot-major: rsp[q*nr + r], q<3 lhs, q>=3 rhs
-    const int* rtb;           // [nr]  eff row or -1
-    const int* raux;          // [nr]  troe / plog row
-    const double* rA;         // [nr]  forward (k_inf)
-    const double* rbeta;
-    const double* rTa;
-    const double* rA0;        // [nr]  k_0 (falloff) or the reverse Arrhenius (nonEquilibrium)
-    const double* rbeta0;
-    const double* rTa0;
-    const double* troe;       // [4*nTroe] (alpha, Tsss, Ts, Tss) per row
-    const double* rtroe;      // [5*nr] coefficient-major rtroe[q*nr + r]: Troe (alpha, Tsss, Ts, Tss, 0)
+    gptr<const int> rnl;           // [nr]
+    gptr<const int> rnr;           // [nr]
+    gptr<const int> rsp;           // [6*nr] slot-major: rsp[q*nr + r], q<3 lhs, q>=3 rhs
+    gptr<const int> rtb;           // [nr]  eff row or -1
+    gptr<const int> raux;          // [nr]  troe / plog row
+    gptr<const double> rA;         // [nr]  forward (k_inf)
+    gptr<const double> rbeta;
+    gptr<const double> rTa;
+    gptr<const double> rA0;        // [nr]  k_0 (falloff) or the reverse Arrhenius (nonEquilibrium)
+    gptr<const double> rbeta0;
+    gptr<const double> rTa0;
+    gptr<const double> troe;       // [4*nTroe] (alpha, Tsss, Ts, Tss) per row
+    gptr<const double> rtroe;      // [5*nr] coefficient-major rtroe[q*nr + r]: Troe (alpha, Tsss, Ts, Tss, 0)
                               // or SRI (a, b, c, d, e); zero for other rows
-    const double* eff;        // [nTB*SW] row-major, zero beyond ns
-    const double* effT;       // [SW*nTBpad] species-major for M sums (zero padding)
-    const int* plog_n;        // [nPlog]
-    const double* plog_lnp;   // [nPlog*pmax]
-    const double* plog_lnA;
-    const double* plog_beta;
-    const double* plog_Ta;
-    const int* sp_rn;         // [maxDeg*SW] thread-interleaved: reaction | (nu + 8) << 16, nu = net
+    gptr<const double> eff;        // [nTB*SW] row-major, zero beyond ns
+    gptr<const double> effT;       // [SW*nTBpad] species-major for M sums (zero padding)
+    gptr<const int> plog_n;        // [nPlog]
+    gptr<const double> plog_lnp;   // [nPlog*pmax]
+    gptr<const double> plog_lnA;
+    gptr<const double> plog_beta;
+    gptr<const double> plog_Ta;
+    gptr<const int> sp_rn;         // [maxDeg*SW] thread-interleaved: reaction | (nu + 8) << 16, nu = net
                               // stoichiometric coefficient (padding: reaction 0, nu 0)
     int nTBz;                 // third-body/falloff (species, row) entries, padded to a multiple of 16
-    const int* tbz_row;       // [nTBz] species, ascending (padding repeats the last)
-    const int* tbz_tb;        // [nTBz] third-body row (padding: 0)
-    const double* tbz_nu;     // [nTBz] net stoichiometric coefficient (padding: 0)
-    const int* tbz_blk;       // [NPm/16 + 1] first entry of each 16-row block (HBM-matrix form)
-    const unsigned long long* rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
-    const unsigned long long* sp_pack;  // [maxDeg*SW] rpack of each per-species list entry
+    gptr<const int> tbz_row;       // [nTBz] species, ascending (padding repeats the last)
+    gptr<const int> tbz_tb;        // [nTBz] third-body row (padding: 0)
+    gptr<const double> tbz_nu;     // [nTBz] net stoichiometric coefficient (padding: 0)
+    gptr<const int> tbz_blk;       // [NPm/16 + 1] first entry of each 16-row block (HBM-matrix form)
+    gptr<const unsigned long long> rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
+    gptr<const unsigned long long> sp_pack;  // [maxDeg*SW] rpack of each per-species list entry
     int maxTerms;             // Jacobian terms per thread (jacobian's sparse-partial walk), multiple of 16
-    const unsigned long long* sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*: every
+    gptr<const unsigned long long> sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*: every
                                         // element's terms in one thread, the threads' loads balanced
-    const double* Hf;         // [SW] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
+    gptr<const double> Hf;         // [SW] mass-specific enthalpy of formation at 298.15 K (0 beyond ns)
     // Kc-reversible reactions with more than 3 product slots (rrev 3): their
     // Jacobian takes one reverse partial kr prod_{b != q} C_b per product slot
     // q, kept in the reaction arrays of LDS past the nr reactions, so nrq =
@@ -131,8 +152,8 @@ struct DevMech {
     // PLOG rows are the reactions plogBase .. plogBase + nPlog - 1 (reactions
     // are sorted by rate form); plog row = r - plogBase
     int plogBase, nPlog;
-    const int* rgen;          // [nr] row of gen (rrev 3 only; -1 elsewhere)
-    const int* gen;           // [rows * GEN_STRIDE]: product slots n, LDS index of the first partial
+    gptr<const int> rgen;          // [nr] row of gen (rrev 3 only; -1 elsewhere)
+    gptr<const int> gen;           // [rows * GEN_STRIDE]: product slots n, LDS index of the first partial
                               // (>= nr), the n product species
 #ifdef FCS_PHASE_TIMING
     unsigned long long* phase;  // [PH_SLOTS * PH_SPREAD] shader-clock cycles per phase
@@ -547,10 +568,10 @@ __device__ inline void stage_plog(const DevMech& M, const Lds& L, double lnp)
     __syncthreads();   // earlier readers of L.plog (the previous cell) are done
     for (int row = tid<NP>(); row < M.nPlog; row += Cell<NP>::G) {
         const int np = M.plog_n[row];
-        const double* lp = M.plog_lnp + (size_t)row * M.pmax;
-        const double* la = M.plog_lnA + (size_t)row * M.pmax;
-        const double* b = M.plog_beta + (size_t)row * M.pmax;
-        const double* ta = M.plog_Ta + (size_t)row * M.pmax;
+        const auto lp = M.plog_lnp + (size_t)row * M.pmax;
+        const auto la = M.plog_lnA + (size_t)row * M.pmax;
+        const auto b = M.plog_beta + (size_t)row * M.pmax;
+        const auto ta = M.plog_Ta + (size_t)row * M.pmax;
         int j = 0;
         double f = 0.0;
         if (lnp <= lp[0]) {
@@ -755,7 +776,7 @@ __device__ __forceinline__ void reaction_eval(const DevMech& M, const Lds& L, co
         // 1.4901171103413047e-8.  The Jacobian's reverse partials
         // kr prod_{b != q} C_b go to the reaction arrays past nr, where the
         // term walk reads them.
-        const int* gp = M.gen + M.rgen[r] * GEN_STRIDE;
+        const auto gp = M.gen + M.rgen[r] * GEN_STRIDE;
         const int n = gp[0], base = gp[1];
         double num = 1.0, den = 1.0, sdb = 0.0;
         for (int a = 0; a < n; ++a) {
@@ -954,7 +975,7 @@ __device__ inline double2 time_scale_sums(const DevMech& M, const Lds& L, double
         }
         double kr = 0.0;
         if (D.rev == 3) {   // every product slot (Tc.C:578-600 walks the whole side)
-            const int* gp = M.gen + M.rgen[r] * GEN_STRIDE;
+            const auto gp = M.gen + M.rgen[r] * GEN_STRIDE;
             CR = 1.0;
             num = 1.0;
             for (int a = 0; a < gp[0]; ++a) {

@@ -1675,7 +1675,7 @@ int fcs_qdot(fcs_solver* s, int64_t nCells, const double* const* RR, double* Qdo
         hipStream_t st = s->stream;
         for (int i = 0; i < ns; ++i)
             HIP_OK(hipMemcpyAsync(dRR + (size_t)i * nc, RR[i], nc * 8, hipMemcpyHostToDevice, st));
-        const double* hf = s->mech->dm.Hf;
+        const double* hf = s->mech->dm.Hf.p;
         int nc32 = (int)nc, ns32 = ns;
         void* args[] = {&nc32, &ns32, &hf, &dRR, &dQ};
         HIP_OK(hipLaunchKernel((const void*)k_qdot, dim3((unsigned)((nc + 255) / 256)), dim3(256), args, 0, st));
