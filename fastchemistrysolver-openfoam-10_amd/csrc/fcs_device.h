@@ -140,6 +140,8 @@ struct DevMech {
     gptr<const int> tbz_blk;       // [NPm/16 + 1] first entry of each 16-row block (HBM-matrix form)
     gptr<const unsigned long long> rpack;    // [nr] species of slots 0-5 in bytes 0-5, nl, nr in bytes 6, 7
     gptr<const unsigned long long> sp_pack;  // [maxDeg*SW] rpack of each per-species list entry
+    gptr<const int> sp_deg;   // [SW] length of each species' list (0 beyond ns): the Jacobian's
+                              // production walk skips the padding blocks of shorter lists
     int maxTerms;             // Jacobian terms per thread (jacobian's sparse-partial walk), multiple of 16
     gptr<const unsigned long long> sp_term;  // [maxTerms*SW] thread-interleaved terms, see jterm_*: every
                                         // element's terms in one thread, the threads' loads balanced
@@ -152,6 +154,10 @@ struct DevMech {
     // PLOG rows are the reactions plogBase .. plogBase + nPlog - 1 (reactions
     // are sorted by rate form); plog row = r - plogBase
     int plogBase, nPlog;
+    // first reaction that reads the k_0 / reverse Arrhenius triple (falloff
+    // forms, nonEquilibrium reverse rates), and the first that reads the Troe
+    // / SRI coefficients (nr when none does)
+    int extLo, troeLo;
     gptr<const int> rgen;          // [nr] row of gen (rrev 3 only; -1 elsewhere)
     gptr<const int> gen;           // [rows * GEN_STRIDE]: product slots n, LDS index of the first partial
                               // (>= nr), the n product species
@@ -532,23 +538,36 @@ struct RxnData {
     double A, b, Ta, A0, b0, Ta0, alpha, Tsss, Ts, Tss, Mc;
 };
 
-__device__ __forceinline__ RxnData load_rxn(const DevMech& M, const Lds& L, int r)
+// Table entries of reaction r (on: the lane has a reaction; an idle lane
+// fetches nothing and gets a reaction that is never evaluated).  The k_0 /
+// reverse-Arrhenius triple is fetched only from M.extLo on and the Troe /
+// SRI coefficients only from M.troeLo on (every reaction that reads them
+// lies there): a round whose lanes all lie below skips those loads, which
+// otherwise occupy the CU's vector-memory pipeline that the four cells of
+// a CU share.
+__device__ __forceinline__ RxnData load_rxn(const DevMech& M, const Lds& L, int r, bool on = true)
 {
-    RxnData d;
-    d.t = M.rtype[r];
-    d.rev = M.rrev[r];
-    d.tb = M.rtb[r];
-    d.pk = M.rpack[r];
-    d.A = M.rA[r];
-    d.b = M.rbeta[r];
-    d.Ta = M.rTa[r];
-    d.A0 = M.rA0[r];
-    d.b0 = M.rbeta0[r];
-    d.Ta0 = M.rTa0[r];
-    d.alpha = M.rtroe[r];
-    d.Tsss = M.rtroe[M.nr + r];
-    d.Ts = M.rtroe[2 * M.nr + r];
-    d.Tss = M.rtroe[3 * M.nr + r];
+    RxnData d{RXN_ARR, 0, -1, 0ull, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (on) {
+        d.t = M.rtype[r];
+        d.rev = M.rrev[r];
+        d.tb = M.rtb[r];
+        d.pk = M.rpack[r];
+        d.A = M.rA[r];
+        d.b = M.rbeta[r];
+        d.Ta = M.rTa[r];
+        if (r >= M.extLo) {
+            d.A0 = M.rA0[r];
+            d.b0 = M.rbeta0[r];
+            d.Ta0 = M.rTa0[r];
+        }
+        if (r >= M.troeLo) {
+            d.alpha = M.rtroe[r];
+            d.Tsss = M.rtroe[M.nr + r];
+            d.Ts = M.rtroe[2 * M.nr + r];
+            d.Tss = M.rtroe[3 * M.nr + r];
+        }
+    }
     d.Mc = L.Mtb[d.tb >= 0 ? d.tb : 0];
     return d;
 }
@@ -908,7 +927,7 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
         const int r1 = r0 + G;
         const bool has1 = r1 < M.nr;
         const RxnData D0 = load_rxn(M, L, r0);
-        const RxnData D1 = load_rxn(M, L, has1 ? r1 : r0);
+        const RxnData D1 = load_rxn(M, L, has1 ? r1 : r0, has1);
         double c0[6], g0[6], db0[6], c1[6], g1[6], db1[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -1142,14 +1161,22 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     // list, loads and gathers issued per block of 8 entries)
     double om = 0.0, jT = 0.0;
     if (sp) {
+        const int deg = M.sp_deg[lane];
         for (int e0 = 0; e0 < M.maxDeg; e0 += 8) {
             double kf[8], kr[8], rd[8], c[8][6];
             unsigned long long pk[8];
             int wv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                wv[u] = M.sp_rn[(e0 + u) * SW + lane];
-                pk[u] = M.sp_pack[(e0 + u) * SW + lane];
+            for (int u = 0; u < 8; ++u) {   // padding: reaction 0, nu 0, species slots 0
+                wv[u] = 8 << 16;
+                pk[u] = 0ull;
+            }
+            if (e0 < deg) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    wv[u] = M.sp_rn[(e0 + u) * SW + lane];
+                    pk[u] = M.sp_pack[(e0 + u) * SW + lane];
+                }
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {

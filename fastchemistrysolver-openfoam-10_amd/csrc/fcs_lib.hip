@@ -502,6 +502,14 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         if (rtype[r] == FCS_RXN_PLOG) { plogBase = r; break; }
     for (int r = 0; r < nr; ++r)
         if (rtype[r] == FCS_RXN_PLOG && raux[r] != r - plogBase) return FCS_E_ARG;
+    // the device fetches the k_0 / reverse Arrhenius triples from extLo on and
+    // the Troe / SRI coefficients from troeLo on
+    int extLo = nr, troeLo = nr;
+    for (int r = nr - 1; r >= 0; --r) {
+        const int t = rtype[r];
+        if (t == FCS_RXN_LINDEMANN || t == FCS_RXN_TROE || t == FCS_RXN_SRI || rrev[r] == 2) extLo = r;
+        if (t == FCS_RXN_TROE || t == FCS_RXN_SRI) troeLo = r;
+    }
     // general-path rows of the rrev 3 reactions: slot count, LDS index of the
     // first reverse partial (the reaction arrays continue past nr), species
     std::vector<int> rgen(std::max(nr, 1), -1), gen;
@@ -551,6 +559,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     }
     std::vector<unsigned long long> spPack((size_t)maxDeg * SW, 0ull);
     std::vector<int> spRN((size_t)maxDeg * SW, 8 << 16);   // padding: reaction 0, nu 0
+    std::vector<int> spDeg(SW, 0);
+    for (int s = 0; s < ns; ++s) spDeg[s] = (int)lst[s].size();
     std::vector<int> tbzRow, tbzTb;
     std::vector<double> tbzNu;
     for (int s = 0; s < ns; ++s) {
@@ -676,6 +686,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
         tbzTb.push_back(0);
         tbzNu.push_back(0.0);
     }
+    const size_t iSpDeg = vi(spDeg);
     const size_t iSpRN = vi(spRN), iSpTerm = add(spTerm.data(), spTerm.size() * 8);
     // first entry of each 16-row block (rows ascending; the padding repeats the last row)
     std::vector<int> tbzBlk(SW / 16 + 1, (int)tbzRow.size());
@@ -715,6 +726,8 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.ns = ns;
     dm.nr = nr;
     dm.nrq = nrq;
+    dm.extLo = extLo;
+    dm.troeLo = troeLo;
     dm.plogBase = plogBase;
     dm.nPlog = (int)plogN.size();
     dm.rgen = (const int*)P(iRgen);
@@ -757,6 +770,7 @@ int fcs_mech_create(const fcs_mech_desc* d, fcs_mech** out)
     dm.plog_beta = (const double*)P(iPb);
     dm.plog_Ta = (const double*)P(iPT);
     dm.sp_rn = (const int*)P(iSpRN);
+    dm.sp_deg = (const int*)P(iSpDeg);
     dm.maxTerms = maxTerms;
     dm.sp_term = (const unsigned long long*)P(iSpTerm);
     dm.nTBz = nTBz > 0 ? (int)tbzRow.size() : 0;
