@@ -1311,28 +1311,31 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     const double Yd = sp ? om * WiByRho : 0.0;
     // (every column up to NP without a species test: the padding columns
     // and C are zero there, and the T column is rewritten after the loop)
+    // (batches of CB elements, all loads of a batch in flight together: in
+    // the HBM-matrix form every batch costs a round trip to the slice)
+    constexpr int CB = Cell<NP>::kLarge ? 16 : (NP % 8 == 0 ? 8 : 4);
     if (sp) {
         double sC = 0.0;
-        for (int j0 = 0; j0 < NP; j0 += 4) {
-            double v[4], c[4];
+        for (int j0 = 0; j0 < NP; j0 += CB) {
+            double v[CB], c[CB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CB; ++u) {
                 v[u] = Jrow[(j0 + u) * JSt];
                 c[u] = L.C[j0 + u];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) sC += v[u] * c[u];
+            for (int u = 0; u < CB; ++u) sC += v[u] * c[u];
         }
         const double sCx = jacType == 0 ? sC : 0.0;   // fast: diagonal dC/dY only
-        for (int j0 = 0; j0 < NP; j0 += 4) {
-            double v[4], iw[4];
+        for (int j0 = 0; j0 < NP; j0 += CB) {
+            double v[CB], iw[CB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CB; ++u) {
                 v[u] = Jrow[(j0 + u) * JSt];
                 iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : readlane_d(S.invW, j0 + u);   // lane j holds 1/W_j
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CB; ++u) {
                 const double iWj = iw[u];
                 Jrow[(j0 + u) * JSt] = jacType == 0
                     ? WiByRho * (v[u] * (rho * iWj) - (MW * iWj) * sCx) + (MW * iWj) * Yd
@@ -1351,17 +1354,17 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     if (lane < NP) {
         const bool tcol = lane == NP - 1;
         double colsum = 0.0;
-        for (int q0 = 0; q0 < NP; q0 += 4) {   // every row: H and Q are zero beyond ns
-            double v[4], h[4], qq[4];
+        for (int q0 = 0; q0 < NP; q0 += CB) {   // every row: H and Q are zero beyond ns
+            double v[CB], h[CB], qq[CB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < CB; ++u) {
                 const int i = q0 + u;
                 v[u] = L.J[lane * JSt + i];
                 h[u] = L.H[i];
                 qq[u] = tcol ? L.Q[i] : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) colsum += v[u] * h[u] + qq[u];
+            for (int u = 0; u < CB; ++u) colsum += v[u] * h[u] + qq[u];
         }
         const double rowT = !tcol ? (sp ? (-colsum - th.cp * dTdt) / cpM : 0.0)
                                   : (-colsum - dTdt * dcpMdT) / cpM;

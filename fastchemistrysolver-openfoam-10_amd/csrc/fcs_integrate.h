@@ -77,15 +77,25 @@ __device__ inline double error_norm(const DevOpts& o, const Lds& L, double y0, d
     return bad ? __builtin_nan("") : mx;
 }
 
-// Iteration matrix diag*I - J in place.
+// Iteration matrix diag*I - J in place, in batches of FB elements whose
+// loads are all in flight together (one round trip to LDS or to the HBM
+// slice per batch rather than per element).
 template <int NP>
 __device__ inline void form_matrix(const Lds& L, double diag)
 {
     constexpr int JSt = Cell<NP>::JSt;
+    constexpr int FB = NP % 16 == 0 ? 16 : (NP % 8 == 0 ? 8 : 4);
     const int lane = tid<NP>();
     double* Jrow = L.J + lane;
-    if (!Cell<NP>::kLarge || lane < NP)
-        for (int j = 0; j < NP; ++j) Jrow[j * JSt] = (lane == j ? diag : 0.0) - Jrow[j * JSt];
+    if (!Cell<NP>::kLarge || lane < NP) {
+        for (int j0 = 0; j0 < NP; j0 += FB) {
+            double v[FB];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) v[u] = Jrow[(j0 + u) * JSt];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) Jrow[(j0 + u) * JSt] = (lane == j0 + u ? diag : 0.0) - v[u];
+        }
+    }
     wave_sync();
 }
 
@@ -375,24 +385,32 @@ __device__ __forceinline__ constexpr int jy_stride() { return Cell<NP>::kLarge ?
 template <int NP>
 __device__ __forceinline__ void jy_store(const Lds& L, double* JyG)
 {
+    constexpr int FB = NP % 16 == 0 ? 16 : (NP % 8 == 0 ? 8 : 4);   // loads in flight together
     const int lane = tid<NP>();
     if (!Cell<NP>::kLarge || lane < NP)
-        for (int j = 0; j < NP; ++j) JyG[j * jy_stride<NP>() + lane] = L.J[j * Cell<NP>::JSt + lane];
+        for (int j0 = 0; j0 < NP; j0 += FB) {
+            double v[FB];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) v[u] = L.J[(j0 + u) * Cell<NP>::JSt + lane];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) JyG[(j0 + u) * jy_stride<NP>() + lane] = v[u];
+        }
 }
 
 // Iteration matrix diag*I - Jy in the LDS matrix, Jy read from HBM.
 template <int NP>
 __device__ __forceinline__ void form_matrix_from(const Lds& L, const double* JyG, double diag)
 {
+    constexpr int FB = NP % 16 == 0 ? 16 : (NP % 8 == 0 ? 8 : 4);   // loads in flight together
     const int lane = tid<NP>();
     double* Jrow = L.J + lane;
     if (!Cell<NP>::kLarge || lane < NP)
-        for (int j0 = 0; j0 < NP; j0 += 4) {
-            double v[4];
+        for (int j0 = 0; j0 < NP; j0 += FB) {
+            double v[FB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = JyG[(j0 + u) * jy_stride<NP>() + lane];
+            for (int u = 0; u < FB; ++u) v[u] = JyG[(j0 + u) * jy_stride<NP>() + lane];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) Jrow[(j0 + u) * Cell<NP>::JSt] = (lane == j0 + u ? diag : 0.0) - v[u];
+            for (int u = 0; u < FB; ++u) Jrow[(j0 + u) * Cell<NP>::JSt] = (lane == j0 + u ? diag : 0.0) - v[u];
         }
     wave_sync();
 }
