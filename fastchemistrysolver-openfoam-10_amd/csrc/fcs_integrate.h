@@ -103,7 +103,7 @@ __device__ inline void form_matrix(const Lds& L, double diag)
 template <int NP>
 __device__ inline auto lu_factor_any(const DevMech& M, const Lds& L)
 {
-    if constexpr (Cell<NP>::kLarge) return lu_factor_large<NP>(L);
+    if constexpr (Cell<NP>::kLarge) return lu_factor_large<NP>(M, L);
     else return lu_factor<NP>(M, L);
 }
 template <int NP, class St>

@@ -66,7 +66,15 @@ __device__ __forceinline__ LuScratch lu_scratch(const Lds& L)
 }
 
 // One 4-block (columns k0 + 4B .. k0 + 4B + 3) of the panel factorisation;
-// B is a template argument so that every register index is static.
+// B is a template argument so that every register index is static.  The
+// 4 x LU_PW strip of the block's pivot rows is factored by wavefront 0, lane
+// (q, e) = (lane / 16, lane % 16) holding element e of the block's physical
+// row kb + q: pivot choice, swaps, 1/pivot, multipliers and the pivot rows'
+// updated values, with the reference's rules.  It leaves the factored strip,
+// the reciprocal pivots, the picks and the pivot rows' places in LDS; every
+// thread then applies the picks to its position and, when its row lies
+// below the block, forms its multipliers and updates its panel columns, in
+// the elimination's order.
 template <int NP, int B>
 __device__ __forceinline__ void lu_panel_block(const LuScratch& S, double* invDg, int t, bool act, int k0, int& pos,
                                                int& buf, double (&pa)[LU_PW])
@@ -76,124 +84,130 @@ __device__ __forceinline__ void lu_panel_block(const LuScratch& S, double* invDg
     const int kb = k0 + 4 * b;
     double* blk = S.blk + buf * 4 * PW;
     int* bpos = S.blkPos + buf * 4;
+    double* invS = S.v0;                 // [4] reciprocal pivots (the solves' exchange vectors, free here)
+    int* pickS = (int*)S.v1;             // [4] physical row picked for logical kb + d
+    int* plS = pickS + 4;                // [4] place (0..3) of the pivot row of logical kb + d
     if (t >= kb && t < kb + 4) {
 #pragma unroll
-        for (int e = 0; e < PW; ++e) blk[(t - kb) * PW + e] = pa[e];
+        for (int e = 4 * b; e < PW; ++e) blk[(t - kb) * PW + e] = pa[e];
         bpos[t - kb] = pos;
     }
     __syncthreads();
-    // the 4 pivot rows (physical kb..kb+3), redundantly in every thread
-    double v[4][PW];
-    int vp[4];
+    if (t < 64) {
+        const int q = t >> 4, e = t & 15;
+        double v = e >= 4 * b ? blk[q * PW + e] : 0.0;
+        int vp[4];   // logical position of the block's physical row kb + z (uniform)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        vp[q] = bpos[q];
+        for (int z = 0; z < 4; ++z) vp[z] = __builtin_amdgcn_readfirstlane(bpos[z]);
 #pragma unroll
-        for (int e = 4 * b; e < PW; ++e) v[q][e] = blk[q * PW + e];
-    }
-    double inv[4];
-    int pl[4];   // physical (0..3 within the block) holding logical kb+d
+        for (int d = 0; d < 4; ++d) {
+            const int k = kb + d;
+            const int c = 4 * b + d;   // panel column
+            double colv[4];            // column c of the block's rows (uniform)
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        const int k = kb + d;
-        const int c = 4 * b + d;   // panel column
-        // value in column k of the row holding logical kb+q
-        auto V = [&](int q) -> double {
-            double r = 0.0;
+            for (int z = 0; z < 4; ++z) colv[z] = readlane_d(v, z * 16 + c);
+            // value in column k of the row holding logical kb+qq
+            auto V = [&](int qq) -> double {
+                double r = 0.0;
+#pragma unroll
+                for (int z = 0; z < 4; ++z)
+                    if (vp[z] == kb + qq) r = colv[z];
+                return r;
+            };
+            int pick = k;
+            if (d == 0) {
+                double temp = 0.0;
+                int im = 0;
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const double a = fabs(V(qq));
+                    if (temp < a) { im = qq; temp = a; }
+                }
+                pick = kb + im;
+            } else if (d == 1) {
+                double tt = fabs(V(1));
+                int im = 1;
+                if (tt < fabs(V(2))) {
+                    im = 2;
+                    tt = V(2);   // signed, LUsolver.C:177
+                }
+                if (tt < fabs(V(3))) im = 3;
+                pick = kb + im;
+            } else if (d == 2) {
+                if (fabs(V(2)) < fabs(V(3))) pick = kb + 3;
+            }
+            if (pick != k) {
+#pragma unroll
+                for (int z = 0; z < 4; ++z) {
+                    if (vp[z] == k) vp[z] = pick;
+                    else if (vp[z] == pick) vp[z] = k;
+                }
+            }
+            int p = 0;
 #pragma unroll
             for (int z = 0; z < 4; ++z)
-                if (vp[z] == kb + q) r = v[z][c];
-            return r;
-        };
-        int pick = k;
-        if (d == 0) {
-            double temp = 0.0;
-            int im = 0;
+                if (vp[z] == k) p = z;
+            double piv = 0.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double a = fabs(V(q));
-                if (temp < a) { im = q; temp = a; }
+            for (int z = 0; z < 4; ++z)
+                if (z == p) piv = colv[z];
+            if (piv == 0.0) {                        // LUsolver.C:155
+                piv = PIVOT_EPS;
+                if (q == p && e == c) v = PIVOT_EPS;
             }
-            pick = kb + im;
-        } else if (d == 1) {
-            double tt = fabs(V(1));
-            int im = 1;
-            if (tt < fabs(V(2))) {
-                im = 2;
-                tt = V(2);   // signed, LUsolver.C:177
+            const double inv = 1.0 / piv;
+            if (t == 0) {
+                invS[d] = inv;
+                pickS[d] = pick;
+                plS[d] = p;
+                invDg[kb + p] = inv;
             }
-            if (tt < fabs(V(3))) im = 3;
-            pick = kb + im;
-        } else if (d == 2) {
-            if (fabs(V(2)) < fabs(V(3))) pick = kb + 3;
+            // block rows below the pivot in logical order: multiplier in
+            // column c, the later columns less multiplier x pivot row
+            const double pe = __shfl(v, p * 16 + e, 64);
+            const double l = __shfl(v, q * 16 + c, 64) * inv;
+            int vq = vp[0];
+#pragma unroll
+            for (int z = 1; z < 4; ++z)
+                if (q == z) vq = vp[z];
+            if (vq > k) {
+                if (e == c) v = l;
+                else if (e > c) v = v - l * pe;
+            }
         }
-        if (pick != k) {
+        if (e >= 4 * b) blk[q * PW + e] = v;
+    }
+    __syncthreads();
+    // positions after the block's swaps, then this thread's row
 #pragma unroll
-            for (int z = 0; z < 4; ++z) {
-                if (vp[z] == k) vp[z] = pick;
-                else if (vp[z] == pick) vp[z] = k;
-            }
+    for (int d = 0; d < 4; ++d) {
+        const int k = kb + d, pick = pickS[d];
+        if (pick != k) {
             if (pos == k) pos = pick;
             else if (pos == pick) pos = k;
         }
-        int p = 0;
+    }
+    if (act && t >= kb + 4) {
 #pragma unroll
-        for (int z = 0; z < 4; ++z)
-            if (vp[z] == k) p = z;
-        pl[d] = p;
-        double piv = 0.0;
-#pragma unroll
-        for (int z = 0; z < 4; ++z)
-            if (z == p) piv = v[z][c];
-        if (piv == 0.0) {                        // LUsolver.C:155
-            piv = PIVOT_EPS;
-#pragma unroll
-            for (int z = 0; z < 4; ++z)
-                if (z == p) v[z][c] = PIVOT_EPS;
-        }
-        inv[d] = 1.0 / piv;
-        if (t == kb + p) invDg[t] = inv[d];
-        // pivot row values of the later panel columns
-        double pe[PW];
-#pragma unroll
-        for (int e = c + 1; e < PW; ++e) {
-            pe[e] = 0.0;
-#pragma unroll
-            for (int z = 0; z < 4; ++z)
-                if (z == p) pe[e] = v[z][e];
-        }
-        // block rows below the pivot in logical order
-#pragma unroll
-        for (int z = 0; z < 4; ++z) {
-            if (vp[z] > k) {
-                const double l = v[z][c] * inv[d];
-                v[z][c] = l;
-#pragma unroll
-                for (int e = c + 1; e < PW; ++e) v[z][e] = v[z][e] - l * pe[e];
-            }
-        }
-        // this thread's row, when it lies below the block
-        if (act && t >= kb + 4) {
-            const double l = pa[c] * inv[d];
+        for (int d = 0; d < 4; ++d) {
+            const int c = 4 * b + d;
+            const double* pr = blk + plS[d] * PW;   // the pivot row: final beyond column c
+            const double l = pa[c] * invS[d];
             pa[c] = l;
 #pragma unroll
-            for (int e = c + 1; e < PW; ++e) pa[e] = pa[e] - l * pe[e];
+            for (int e = c + 1; e < PW; ++e) pa[e] = pa[e] - l * pr[e];
         }
     }
     // pivot-row threads take their final values
     if (t >= kb && t < kb + 4) {
 #pragma unroll
-        for (int z = 0; z < 4; ++z)
-            if (z == t - kb) {
-#pragma unroll
-                for (int e = 4 * b; e < PW; ++e) pa[e] = v[z][e];
-            }
+        for (int e = 4 * b; e < PW; ++e) pa[e] = blk[(t - kb) * PW + e];
     }
     buf ^= 1;
 }
 
 template <int NP>
-__device__ inline LuLarge lu_factor_large(const Lds& L)
+__device__ inline LuLarge lu_factor_large(const DevMech& M, const Lds& L)
 {
     static_assert(NP % LU_PW == 0 && NP % 16 == 0 && LU_PW == 16, "workgroup-form LU: NP a multiple of 16");
     constexpr int PW = LU_PW;
@@ -204,6 +218,7 @@ __device__ inline LuLarge lu_factor_large(const Lds& L)
     const LuScratch S = lu_scratch<NP>(L);
     int pos = t;
     int buf = 0;
+    PhaseClock clk;
     for (int k0 = 0; k0 < NP; k0 += PW) {
         // ---- 1. panel factorisation (rows >= k0)
         double pa[PW];
@@ -230,6 +245,7 @@ __device__ inline LuLarge lu_factor_large(const Lds& L)
             for (int d = 0; d < PW; ++d) S.Lp[q * PW + d] = d < q ? pa[d] : 0.0;
         }
         __syncthreads();
+        clk.mark(M, t == 0, PH_LU_PANEL);
         if (k0 + PW >= NP) break;
         // ---- 2. U12: thread j forms the pivot rows' entries of column j
         const int j = t;
@@ -251,6 +267,7 @@ __device__ inline LuLarge lu_factor_large(const Lds& L)
             }
         }
         __syncthreads();
+        clk.mark(M, t == 0, PH_LU_UROWS);
         // ---- 3. trailing update on FP64 MFMA: tiles of rows r0.., columns c0..
         const int m0 = k0 + PW, nT = (NP - m0) / 16;
         const int lane = lane_id(), w = t >> 6;
@@ -292,6 +309,7 @@ __device__ inline LuLarge lu_factor_large(const Lds& L)
             }
         }
         __syncthreads();
+        clk.mark(M, t == 0, PH_LU_TRAIL);
     }
     if (row) L.ip[pos] = t;
     __syncthreads();
