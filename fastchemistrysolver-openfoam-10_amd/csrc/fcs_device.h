@@ -59,9 +59,10 @@ struct Cell {
     static constexpr int ONE = kLarge ? 255 : 64;                 // concentration slot holding 1.0
     static constexpr int CS = kLarge ? 256 : 65;                  // concentration slots
     static constexpr int JSt = kLarge ? NP : JS;                  // matrix column stride
-    // wavefronts per SIMD the step kernel is compiled for: two for the
-    // HBM-matrix workgroup form (two cells share a CU, hiding the matrix
-    // round trips of one behind the other; registers capped at 256)
+    // wavefronts per SIMD the step kernels (Rosenbrock and Seulex) are
+    // compiled for: two for the HBM-matrix workgroup form (two cells share a
+    // CU, hiding the matrix round trips of one behind the other; registers
+    // capped at 256)
     static constexpr int kMinWaves = (kLarge && !kJLds) ? 2 : 1;
 };
 

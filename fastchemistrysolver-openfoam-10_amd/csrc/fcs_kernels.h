@@ -155,7 +155,7 @@ __global__ __launch_bounds__(Cell<NP>::G, Cell<NP>::kMinWaves) void k_step(const
 // list (same list protocol as k_step).  The per-cell Jacobian lives in
 // a.Jy between launches.
 template <int NP>
-__global__ __launch_bounds__(Cell<NP>::G) void k_step_seulex(const DevMech* Mg, DevOpts o, CellArrays a,
+__global__ __launch_bounds__(Cell<NP>::G, Cell<NP>::kMinWaves) void k_step_seulex(const DevMech* Mg, DevOpts o, CellArrays a,
                                                              const int* listIn, const unsigned int* countIn,
                                                              int* listOut, unsigned int* countOut,
                                                              unsigned int* countZero)
