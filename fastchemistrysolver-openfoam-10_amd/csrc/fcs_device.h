@@ -1315,6 +1315,14 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     // (batches of CB elements, all loads of a batch in flight together: in
     // the HBM-matrix form every batch costs a round trip to the slice)
     constexpr int CB = Cell<NP>::kLarge ? 16 : (NP % 8 == 0 ? 8 : 4);
+    // 1/W_j of every column: from the table in the workgroup form, from LDS
+    // in the wavefront form (lane j holds it; d(-G/RT)/dT is not read again
+    // in this evaluation, so its slots take the copy)
+    double* iWl = L.dBdT;
+    if constexpr (!Cell<NP>::kLarge) {
+        iWl[lane] = S.invW;
+        wave_sync();
+    }
     if (sp) {
         double sC = 0.0;
         for (int j0 = 0; j0 < NP; j0 += CB) {
@@ -1333,7 +1341,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < CB; ++u) {
                 v[u] = Jrow[(j0 + u) * JSt];
-                iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : readlane_d(S.invW, j0 + u);   // lane j holds 1/W_j
+                iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : iWl[j0 + u];
             }
 #pragma unroll
             for (int u = 0; u < CB; ++u) {
