@@ -37,11 +37,12 @@ def build():
     return OUT
 
 
-def solve(mech, field, deltaT, absTol=1e-8, relTol=1e-4, maxSteps=10000, Treact=0.0, dtChemMax=1.0e15,
-          timeout=60.0):
-    """OptRodas34 solve(deltaT) of every cell of the field through the host
-    build of k_step; returns the results and the rendezvous counts."""
-    from fcs.abi import make_desc
+def solve(mech, field, deltaT, solver="OptRodas34", absTol=1e-8, relTol=1e-4, maxSteps=10000, Treact=0.0,
+          dtChemMax=1.0e15, timeout=60.0):
+    """solve(deltaT) of every cell of the field through the host build of
+    k_step (k_step_seulex for OptSeulex); returns the results and the
+    rendezvous counts."""
+    from fcs.abi import SOLVER_NAMES, make_desc
     L = C.CDLL(build())
     L.emu_solve.restype = C.c_int
     desc, keep = make_desc(mech.arrays())
@@ -60,7 +61,7 @@ def solve(mech, field, deltaT, absTol=1e-8, relTol=1e-4, maxSteps=10000, Treact=
     msg = C.create_string_buffer(512)
     d = lambda a: a.ctypes.data_as(_dp)
     i = lambda a: a.ctypes.data_as(_ip)
-    rc = L.emu_solve(C.byref(desc), C.c_double(absTol), C.c_double(relTol), C.c_int(maxSteps), C.c_double(Treact),
+    rc = L.emu_solve(C.byref(desc), C.c_int(SOLVER_NAMES[solver]), C.c_double(absTol), C.c_double(relTol), C.c_int(maxSteps), C.c_double(Treact),
                      C.c_double(dtChemMax), C.c_int(nc), C.c_double(deltaT), d(T0), d(p0), d(rho0), d(Y0), d(dtc),
                      d(RR), d(Tend), d(dto), i(st), i(na), C.byref(checks), C.byref(viol), msg, C.c_int(512),
                      C.c_double(timeout))

@@ -190,10 +190,18 @@ static bool run_wave(emu::Wave& w, unsigned b, unsigned g, F&& body)
 }
 
 template <int NP>
-static bool run_launch(const DevMech* M, DevOpts o, CellArrays a, int* lin, unsigned* cin, int* lout, unsigned* cout,
-                       unsigned* czero, emu::Wave& w)
+static bool run_launch(int solver, const DevMech* M, DevOpts o, CellArrays a, int* lin, unsigned* cin, int* lout,
+                       unsigned* cout, unsigned* czero, emu::Wave& w)
 {
-    return run_wave(w, 0, 1, [&] { k_step<NP, SOLVER_RODAS34>(M, o, a, lin, cin, lout, cout, czero); });
+    return run_wave(w, 0, 1, [&] {
+        switch (solver) {
+        case SOLVER_RODAS34: k_step<NP, SOLVER_RODAS34>(M, o, a, lin, cin, lout, cout, czero); break;
+        case SOLVER_ROS34: k_step<NP, SOLVER_ROS34>(M, o, a, lin, cin, lout, cout, czero); break;
+        case SOLVER_RODAS23: k_step<NP, SOLVER_RODAS23>(M, o, a, lin, cin, lout, cout, czero); break;
+        case SOLVER_ROS23: k_step<NP, SOLVER_ROS23>(M, o, a, lin, cin, lout, cout, czero); break;
+        default: k_step_seulex<NP>(M, o, a, lin, cin, lout, cout, czero); break;
+        }
+    });
 }
 
 static int pick_np(int ns)
@@ -335,7 +343,7 @@ extern "C" int emu_selftest(int kind, double timeoutSeconds, long long* checks, 
     return ok ? FCS_OK : FCS_E_CELLS;
 }
 
-extern "C" int emu_solve(const fcs_mech_desc* d, double absTol, double relTol, int maxSteps, double Treact,
+extern "C" int emu_solve(const fcs_mech_desc* d, int solver, double absTol, double relTol, int maxSteps, double Treact,
                          double dtChemMax, int nc,
                          double dt, const double* T0, const double* p0, const double* rho0, const double* Y0,
                          const double* dtcIn, double* RR, double* Tend, double* dtcOut, int* status, int* nAtt,
@@ -353,10 +361,15 @@ extern "C" int emu_solve(const fcs_mech_desc* d, double absTol, double relTol, i
     };
     const int rc = build_mech_tables(d, NP, 64, place, dm, perm);
     if (rc != FCS_OK) return rc;
-    DevOpts o{SOLVER_RODAS34, 0, maxSteps, absTol, relTol, Treact, dtChemMax};
+    if (solver < SOLVER_RODAS34 || solver > SOLVER_ROS23) return FCS_E_ARG;
+    DevOpts o{solver, 0, maxSteps, absTol, relTol, Treact, dtChemMax};
     const size_t n = (size_t)nc;
     std::vector<double> Yw((ns + 1) * n), x(n), dxTry(n), dx(n), dxTry0(n), dxFirst(n);
-    std::vector<int> flags(n), nStep(n), list0(n), list1(n);
+    std::vector<int> flags(n), nStep(n), list0(n), list1(n), kTarg(n);
+    // Seulex: step-start state and scale vector [ns+1][nc], theta, kTarg, the
+    // per-cell Jacobian (NP columns of 64)
+    std::vector<double> Y0s(solver == SOLVER_SEULEX ? (ns + 1) * n : 1), Scl(solver == SOLVER_SEULEX ? (ns + 1) * n : 1),
+        theta(n), Jy(solver == SOLVER_SEULEX ? n * (size_t)NP * 64 : 1);
     unsigned long long dtMin = 0x7ff0000000000000ull;
     CellArrays a{};
     a.nc = nc;
@@ -380,6 +393,11 @@ extern "C" int emu_solve(const fcs_mech_desc* d, double absTol, double relTol, i
     a.flags = flags.data();
     a.nStep = nStep.data();
     a.dxFirst = dxFirst.data();
+    a.Y0s = Y0s.data();
+    a.Scl = Scl.data();
+    a.theta = theta.data();
+    a.Jy = Jy.data();
+    a.kTarg = kTarg.data();
     // initial state as the library's set-up kernel makes it
     unsigned counts[3] = {0, 0, 0};
     for (int c = 0; c < nc; ++c) {
@@ -398,8 +416,10 @@ extern "C" int emu_solve(const fcs_mech_desc* d, double absTol, double relTol, i
         dxTry[c] = dtcIn[c];
         dx[c] = 0.0;
         dxTry0[c] = 0.0;
-        flags[c] = 0;
+        flags[c] = solver == SOLVER_SEULEX ? SX_FIRST : 0;
         dxFirst[c] = 0.0;
+        theta[c] = 0.0;
+        kTarg[c] = 0;
         nStep[c] = 0;
         nAtt[c] = 0;
         list0[counts[0]++] = c;
@@ -418,11 +438,11 @@ extern "C" int emu_solve(const fcs_mech_desc* d, double absTol, double relTol, i
         unsigned* czero = counts + (it + 2) % 3;
         if (*cin == 0) break;
         switch (NP) {
-        case 12: ok = run_launch<12>(&dm, o, a, lin, cin, lout, cout, czero, w); break;
-        case 24: ok = run_launch<24>(&dm, o, a, lin, cin, lout, cout, czero, w); break;
-        case 40: ok = run_launch<40>(&dm, o, a, lin, cin, lout, cout, czero, w); break;
-        case 56: ok = run_launch<56>(&dm, o, a, lin, cin, lout, cout, czero, w); break;
-        default: ok = run_launch<64>(&dm, o, a, lin, cin, lout, cout, czero, w); break;
+        case 12: ok = run_launch<12>(solver, &dm, o, a, lin, cin, lout, cout, czero, w); break;
+        case 24: ok = run_launch<24>(solver, &dm, o, a, lin, cin, lout, cout, czero, w); break;
+        case 40: ok = run_launch<40>(solver, &dm, o, a, lin, cin, lout, cout, czero, w); break;
+        case 56: ok = run_launch<56>(solver, &dm, o, a, lin, cin, lout, cout, czero, w); break;
+        default: ok = run_launch<64>(solver, &dm, o, a, lin, cin, lout, cout, czero, w); break;
         }
     }
     *checks = w.checks;

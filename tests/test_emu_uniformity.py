@@ -32,11 +32,11 @@ def test_checker_reports_divergent_lanes():
         assert what in r["message"], (kind, r["message"])
 
 
-def assert_emulated_solve(mech, field, dt):
-    r = emu.solve(mech, field, dt)
+def assert_emulated_solve(mech, field, dt, solver="OptRodas34"):
+    r = emu.solve(mech, field, dt, solver=solver)
     assert r["violations"] == 0, r["message"]
     assert r["rc"] == 0 and r["checks"] > 10000
-    o = Oracle(mech).solve(field, dt)
+    o = Oracle(mech).solve(field, dt, solver=solver)
     assert (r["status"] == o["status"]).all()
     assert (r["nAttempts"] == o["nAttempts"]).all(), (r["nAttempts"], o["nAttempts"])
     assert np.abs(r["RR"] - o["RR"]).max() <= 1e-10 * np.abs(o["RR"]).max()
@@ -62,3 +62,12 @@ def test_emulated_step_kernel_gri30(gri30):
     sub = field.subset(idx)
     r = assert_emulated_solve(mech, sub, 1e-6)
     assert r["nAttempts"].max() >= 1
+
+
+@pytest.mark.parametrize("solver", ["OptRosenbrock34", "OptRodas23", "OptRosenbrock23", "OptSeulex"])
+def test_emulated_step_kernels_other_solvers(mech0, solver):
+    """The other four solvers' step kernels (k_step with their tableaus,
+    k_step_seulex's extrapolation passes) on the 10-species mechanism
+    through 1e-7 s: two cells, one for Seulex."""
+    field = M.config_field(0, mech0, n_cells=1 if solver == "OptSeulex" else 2)
+    assert_emulated_solve(mech0, field, 1e-7, solver)
