@@ -318,8 +318,10 @@ __device__ inline LuLarge lu_factor_large(const DevMech& M, const Lds& L)
 
 // Solves (L U) x = P b (src/LUsolver/LUsolver.C:3922-4111: unit-L forward,
 // U backward scaled by 1/pivot); b is this thread's element in the original
-// row order; returns x_t for thread t.  The factors of the next panel are
-// loaded while the current one is solved.
+// row order; returns x_t for thread t.  Each panel's factors are loaded at
+// the start of its step: a prefetch of the next panel into a second
+// register set cost more than it hid, through the spills it added under the
+// 256-register cap of the two-workgroups-per-CU build.
 template <int NP>
 __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double b)
 {
@@ -333,7 +335,7 @@ __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double 
     const bool diag = t < PW;   // wavefront 0, lanes 0..15: the diagonal block
     // row updates (lr: this thread's factors in the panel's columns) and the
     // diagonal block's factors (dq: lane q = logical row k0 + q)
-    double lr[PW], dq[PW], lrN[PW], dqN[PW];
+    double lr[PW], dq[PW];
     auto load_fwd = [&](int k0, double (&l)[PW], double (&d)[PW]) {
 #pragma unroll
         for (int r = 0; r < PW; ++r) l[r] = (row && t >= k0 + PW) ? J[(size_t)(k0 + r) * NP + t] : 0.0;
@@ -349,9 +351,8 @@ __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double 
         for (int r = 0; r < PW; ++r) d[r] = (diag && r > lane) ? J[(size_t)(k0 + r) * NP + ph] : 0.0;
     };
     // forward: y = L^-1 P b, panel by panel in logical order
-    load_fwd(0, lr, dq);
     for (int k0 = 0; k0 < NP; k0 += PW) {
-        if (k0 + PW < NP) load_fwd(k0 + PW, lrN, dqN);
+        load_fwd(k0, lr, dq);
         if (t >= k0 && t < k0 + PW) S.v0[st.pos - k0] = b;
         __syncthreads();
         if (diag) {
@@ -372,17 +373,11 @@ __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double 
         } else if (t >= k0 && t < k0 + PW) {
             b = S.v1[st.pos - k0];
         }
-#pragma unroll
-        for (int r = 0; r < PW; ++r) {
-            lr[r] = lrN[r];
-            dq[r] = dqN[r];
-        }
     }
     // backward: x = U^-1 y, panels in descending logical order
     double x = 0.0;
-    load_bwd(NP - PW, lr, dq);
     for (int k0 = NP - PW; k0 >= 0; k0 -= PW) {
-        if (k0 >= PW) load_bwd(k0 - PW, lrN, dqN);
+        load_bwd(k0, lr, dq);
         const double id = diag ? invDg[L.ip[k0 + lane]] : 0.0;
         __syncthreads();
         if (t >= k0 && t < k0 + PW) S.v0[st.pos - k0] = b;
@@ -405,11 +400,6 @@ __device__ inline double lu_solve_large(const Lds& L, const LuLarge& st, double 
             b = acc;
         }
         if (t >= k0 && t < k0 + PW) x = S.v1[t - k0];
-#pragma unroll
-        for (int r = 0; r < PW; ++r) {
-            lr[r] = lrN[r];
-            dq[r] = dqN[r];
-        }
     }
     __syncthreads();
     return x;
