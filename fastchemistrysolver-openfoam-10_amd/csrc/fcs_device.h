@@ -742,22 +742,25 @@ __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 {
     constexpr int G = Cell<NP>::G;
     const int lane = tid<NP>();
+    // batches of TS species: 16 in the wavefront form, 8 in the workgroup
+    // form, whose two-workgroups-per-CU build has half the registers
+    constexpr int TS = Cell<NP>::kLarge ? 8 : 16;
     for (int t0 = 0; t0 < M.nTB; t0 += 2 * G) {
         const int ta = t0 + lane, tb = t0 + G + lane;
         const bool ina = ta < M.nTB, inb = tb < M.nTB;
         double ma = 0.0, mb = 0.0;
-        for (int j0 = 0; j0 < M.ns; j0 += 16) {   // batches of 16 species
+        for (int j0 = 0; j0 < M.ns; j0 += TS) {
             // unconditional loads (the table is zero-padded to SW species and
-            // nTBpad rows, C is zero beyond ns), so all 48 are in flight at once
-            double ea[16], eb[16], c[16];
+            // nTBpad rows, C is zero beyond ns), so all 3 TS are in flight at once
+            double ea[TS], eb[TS], c[TS];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < TS; ++u) {
                 c[u] = L.C[j0 + u];
                 ea[u] = M.effT[(size_t)(j0 + u) * M.nTBpad + ta];
                 eb[u] = M.effT[(size_t)(j0 + u) * M.nTBpad + tb];
             }
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {   // no species test: padding adds 0 * 0
+            for (int u = 0; u < TS; ++u) {   // no species test: padding adds 0 * 0
                 ma += ea[u] * c[u];
                 mb += eb[u] * c[u];
             }
