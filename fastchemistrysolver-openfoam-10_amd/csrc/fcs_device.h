@@ -1318,15 +1318,14 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
     // (batches of CB elements, all loads of a batch in flight together: in
     // the HBM-matrix form every batch costs a round trip to the slice)
     constexpr int CB = Cell<NP>::kLarge ? 16 : (NP % 8 == 0 ? 8 : 4);
-    // 1/W_j of every column: from the table in the workgroup form, from LDS
-    // in the wavefront form (lane j holds it; d(-G/RT)/dT is not read again
-    // in this evaluation, so its slots take the copy)
-    double* iWl = L.dBdT;
-    if constexpr (!Cell<NP>::kLarge) {
-        iWl[lane] = S.invW;
-        wave_sync();
-    }
-    if (sp) {
+    // In the wavefront form every lane runs the conversion, so that the
+    // readlanes of 1/W_j (lane j holds it) sit in wave-uniform control flow:
+    // WiByRho and Yd are zero outside the species lanes, so the padding rows
+    // and the temperature row get zeros (the latter is rewritten below), and
+    // lanes past NP write only column slots no phase reads.  The workgroup
+    // form reads 1/W_j from the table in the species threads.
+    const bool conv = Cell<NP>::kLarge ? sp : true;
+    if (conv) {
         double sC = 0.0;
         for (int j0 = 0; j0 < NP; j0 += CB) {
             double v[CB], c[CB];
@@ -1344,7 +1343,7 @@ __device__ inline double jacobian(const DevMech& M, const Lds& L, double p, doub
 #pragma unroll
             for (int u = 0; u < CB; ++u) {
                 v[u] = Jrow[(j0 + u) * JSt];
-                iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : iWl[j0 + u];
+                iw[u] = Cell<NP>::kLarge ? M.invW[j0 + u] : readlane_d(S.invW, j0 + u);
             }
 #pragma unroll
             for (int u = 0; u < CB; ++u) {
