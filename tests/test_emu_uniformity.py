@@ -45,10 +45,10 @@ def assert_emulated_solve(mech, field, dt, solver="OptRodas34"):
 
 
 def test_emulated_step_kernel_ten_species(mech0):
-    """Four cells of the 10-species mechanism through 1e-6 s (about 150 to
+    """Two cells of the 10-species mechanism through 1e-6 s (about 150 to
     200 attempts each: rejected and accepted steps, the last-step cut, cell
     completion), NP = 12."""
-    field = M.config_field(0, mech0, n_cells=4)
+    field = M.config_field(0, mech0, n_cells=2)
     r = assert_emulated_solve(mech0, field, 1e-6)
     assert r["nAttempts"].min() > 100
 
