@@ -38,10 +38,14 @@ def build():
 
 
 def solve(mech, field, deltaT, solver="OptRodas34", absTol=1e-8, relTol=1e-4, maxSteps=10000, Treact=0.0,
-          dtChemMax=1.0e15, timeout=60.0):
+          dtChemMax=1.0e15, timeout=60.0, P=None, seed=0):
     """solve(deltaT) of every cell of the field through the host build of
     k_step (k_step_seulex for OptSeulex); returns the results and the
-    rendezvous counts."""
+    rendezvous counts.  P (float32 [reacting cells, ns+2], rows in cell
+    order: [ln h, z_0 .. z_ns-1, z_T]) runs the learned seed first: the
+    gate, then k_verify's Rodas34 attempt per gated proposal (seed != 0:
+    rejected and ungated cells start from min(h, deltaT)); nAccepted is
+    [accepted, gated]."""
     from fcs.abi import SOLVER_NAMES, make_desc
     L = C.CDLL(build())
     L.emu_solve.restype = C.c_int
@@ -61,12 +65,17 @@ def solve(mech, field, deltaT, solver="OptRodas34", absTol=1e-8, relTol=1e-4, ma
     msg = C.create_string_buffer(512)
     d = lambda a: a.ctypes.data_as(_dp)
     i = lambda a: a.ctypes.data_as(_ip)
+    if P is not None:
+        P = np.ascontiguousarray(P, np.float32)
+        assert P.ndim == 2 and P.shape[1] == ns + 2 and P.shape[0] == int((field.T > Treact).sum())
+    nacc = (C.c_ulonglong * 2)()
     rc = L.emu_solve(C.byref(desc), C.c_int(SOLVER_NAMES[solver]), C.c_double(absTol), C.c_double(relTol), C.c_int(maxSteps), C.c_double(Treact),
                      C.c_double(dtChemMax), C.c_int(nc), C.c_double(deltaT), d(T0), d(p0), d(rho0), d(Y0), d(dtc),
                      d(RR), d(Tend), d(dto), i(st), i(na), C.byref(checks), C.byref(viol), msg, C.c_int(512),
-                     C.c_double(timeout))
+                     C.c_double(timeout), None if P is None else P.ctypes.data_as(C.POINTER(C.c_float)),
+                     C.c_int(seed), nacc)
     return dict(rc=rc, RR=RR, Tend=Tend, deltaTChem=dto, status=st, nAttempts=na, checks=checks.value,
-                violations=viol.value, message=msg.value.decode())
+                violations=viol.value, message=msg.value.decode(), nAccepted=(nacc[0], nacc[1]))
 
 
 def derivatives(mech, p, YT):

@@ -204,6 +204,19 @@ static bool run_launch(int solver, const DevMech* M, DevOpts o, CellArrays a, in
     });
 }
 
+static bool run_verify(int NP, const DevMech* M, DevOpts o, CellArrays a, const int* list, const int* listG,
+                       const unsigned* countG, const float* P, int seed, int* listOut, unsigned* countOut,
+                       unsigned long long* nAccepted, emu::Wave& w)
+{
+    switch (NP) {
+    case 12: return run_wave(w, 0, 1, [&] { k_verify<12>(M, o, a, list, listG, countG, P, seed, listOut, countOut, nAccepted); });
+    case 24: return run_wave(w, 0, 1, [&] { k_verify<24>(M, o, a, list, listG, countG, P, seed, listOut, countOut, nAccepted); });
+    case 40: return run_wave(w, 0, 1, [&] { k_verify<40>(M, o, a, list, listG, countG, P, seed, listOut, countOut, nAccepted); });
+    case 56: return run_wave(w, 0, 1, [&] { k_verify<56>(M, o, a, list, listG, countG, P, seed, listOut, countOut, nAccepted); });
+    default: return run_wave(w, 0, 1, [&] { k_verify<64>(M, o, a, list, listG, countG, P, seed, listOut, countOut, nAccepted); });
+    }
+}
+
 static int pick_np(int ns)
 {
     for (int n : {12, 24, 40, 56, 64})
@@ -347,7 +360,8 @@ extern "C" int emu_solve(const fcs_mech_desc* d, int solver, double absTol, doub
                          double dtChemMax, int nc,
                          double dt, const double* T0, const double* p0, const double* rho0, const double* Y0,
                          const double* dtcIn, double* RR, double* Tend, double* dtcOut, int* status, int* nAtt,
-                         long long* checks, long long* violations, char* msg, int msgLen, double timeoutSeconds)
+                         long long* checks, long long* violations, char* msg, int msgLen, double timeoutSeconds,
+                         const float* P, int seed, unsigned long long* nAccepted)
 {
     const int ns = d->n_species;
     const int NP = pick_np(ns);
@@ -430,7 +444,30 @@ extern "C" int emu_solve(const fcs_mech_desc* d, int solver, double absTol, doub
     emu::g_grid = 1;
     const long long maxLaunches = 101LL * maxSteps + 2;
     bool ok = true;
-    for (long long it = 0; it < maxLaunches && ok; ++it) {
+    long long it0 = 0;
+    unsigned long long acc[2] = {0, 0};
+    if (P && counts[0] > 0) {
+        // learned seed as the library runs it: the gate (proposals with h >=
+        // deltaT/2 to the gated list, the others, seeded when asked, straight
+        // to list 1), then k_verify's one Rodas34 attempt per gated cell
+        std::vector<int> listG(n);
+        unsigned countG = 0;
+        const unsigned n0 = counts[0];
+        for (unsigned i = 0; i < n0; ++i) {
+            const int c = list0[i];
+            const double h = std::exp((double)P[(size_t)i * (ns + 2)]);
+            if (h >= 0.5 * dt) {
+                listG[countG++] = (int)i;
+                ++acc[1];
+                continue;
+            }
+            if (seed && h == h && h > 0.0) dxTry[c] = h < dt ? h : dt;
+            list1[counts[1]++] = c;
+        }
+        ok = run_verify(NP, &dm, o, a, list0.data(), listG.data(), &countG, P, seed, list1.data(), counts + 1, acc, w);
+        it0 = 1;
+    }
+    for (long long it = it0; it < maxLaunches && ok; ++it) {
         int* lin = (it & 1) ? list1.data() : list0.data();
         int* lout = (it & 1) ? list0.data() : list1.data();
         unsigned* cin = counts + it % 3;
@@ -447,6 +484,10 @@ extern "C" int emu_solve(const fcs_mech_desc* d, int solver, double absTol, doub
     }
     *checks = w.checks;
     *violations = w.violations;
+    if (nAccepted) {
+        nAccepted[0] = acc[0];
+        nAccepted[1] = acc[1];
+    }
     if (msg && msgLen > 0) std::snprintf(msg, (size_t)msgLen, "%s", w.first.c_str());
     return ok ? FCS_OK : FCS_E_CELLS;
 }
