@@ -742,9 +742,51 @@ __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 {
     constexpr int G = Cell<NP>::G;
     const int lane = tid<NP>();
-    // batches of TS species: 16 in the wavefront form, 8 in the workgroup
-    // form, whose two-workgroups-per-CU build has half the registers
-    constexpr int TS = Cell<NP>::kLarge ? 8 : 16;
+    if constexpr (!Cell<NP>::kLarge) {
+        // wavefront form: passes of 64 rows, one row per lane; a pass of at
+        // most 32 (16) rows splits the species over 2 (4) lane groups, each
+        // summing H of them, and adds the groups' sums by xor shuffles, so
+        // it issues 1/2 (1/4) of the efficiency loads.  H is a multiple of
+        // 8 and S H <= 64 (ns <= 63): every index stays inside the
+        // zero-padded table and C[0..63], zero beyond ns.
+        for (int t0 = 0; t0 < M.nTB; t0 += 64) {
+            const int R = M.nTB - t0;   // uniform
+            const int S = R > 32 ? 1 : (R > 16 ? 2 : 4);
+            const int rows = 64 / S;
+            const int t = t0 + lane % rows;
+            const int H = round_up((M.ns + S - 1) / S, 8);
+            const int jb = (lane / rows) * H;
+            double m = 0.0;
+            int k = 0;
+            for (; k + 16 <= H; k += 16) {
+                double e[16], c[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    c[u] = L.C[jb + k + u];
+                    e[u] = M.effT[(size_t)(jb + k + u) * M.nTBpad + t];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) m += e[u] * c[u];
+            }
+            if (k < H) {   // H - k == 8
+                double e[8], c[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    c[u] = L.C[jb + k + u];
+                    e[u] = M.effT[(size_t)(jb + k + u) * M.nTBpad + t];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) m += e[u] * c[u];
+            }
+            if (S >= 4) m += __shfl(m, lane ^ 16, 64);
+            if (S >= 2) m += __shfl(m, lane ^ 32, 64);
+            if (lane < rows && t < M.nTB) L.Mtb[t] = m;
+        }
+        return;
+    }
+    // workgroup form: batches of 8 species (the two-workgroups-per-CU build
+    // has half the registers of the wavefront form)
+    constexpr int TS = 8;
     for (int t0 = 0; t0 < M.nTB; t0 += 2 * G) {
         const int ta = t0 + lane, tb = t0 + G + lane;
         const bool ina = ta < M.nTB, inb = tb < M.nTB;
@@ -927,9 +969,14 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
 {
     constexpr int G = Cell<NP>::G;
     const double PbyRT = PSTD / (RU * T);
-    for (int r0 = tid<NP>(); r0 < M.nr; r0 += 2 * G) {
-        const int r1 = r0 + G;
-        const bool has1 = r1 < M.nr;
+    // reactions from the last one down: the tables order them by rate form,
+    // falloff forms last, so the costliest forms share the first slot and
+    // the partly filled slot of the last round holds the cheapest ones (a
+    // form's code runs once per slot in which any lane holds it)
+    for (int k = tid<NP>(); k < M.nr; k += 2 * G) {
+        const int r0 = M.nr - 1 - k;
+        const int r1 = r0 - G;
+        const bool has1 = r1 >= 0;
         const RxnData D0 = load_rxn(M, L, r0);
         const RxnData D1 = load_rxn(M, L, has1 ? r1 : r0, has1);
         double c0[6], g0[6], db0[6], c1[6], g1[6], db1[6];
