@@ -179,17 +179,23 @@ def error_vs_oracle(mech, field, idx, runs):
     return out
 
 
-def timed_runs(batch, steps, warmup, profile):
-    for _ in range(warmup):
+def timed_runs(batch, steps, warmup, profile, label=None):
+    """label: log a line after every step (runs of a minute per step)."""
+    for k in range(warmup):
+        t0 = time.perf_counter()
         batch.run(DT, timed=True)
+        if label:
+            log(f"{label}: warm-up step {k + 1} of {warmup}, {time.perf_counter() - t0:.1f} s")
     batch.profile(profile)
     launches, step_ms = 0, 0.0
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         batch.run(DT, timed=True)
         n, ms = batch.stats()
         launches += n
         step_ms += ms
+        if label:
+            log(f"{label}: step {k + 1} of {steps} done")
     wall = time.perf_counter() - t0
     batch.profile(False)
     return wall, launches, step_ms
@@ -425,7 +431,7 @@ def large(args):
         b = lib.Batch(s, field)
         if mlp is not None:
             b.set_learned(mlp, "verify")
-        wall, launches, step_ms = timed_runs(b, args.steps, args.warmup, profile=(name == "plain"))
+        wall, launches, step_ms = timed_runs(b, args.steps, args.warmup, profile=(name == "plain"), label=name)
         r = b.download()
         runs[name] = dict(wall=wall, launches=launches, step_ms=step_ms, att=int(r["nAttempts"].sum()),
                           failed=int(((r["status"] & 6) != 0).sum()),

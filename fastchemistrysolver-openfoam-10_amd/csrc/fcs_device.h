@@ -736,38 +736,42 @@ __device__ inline RxnK rate_constant(const DevMech& M, const Lds& L, const RxnDa
 
 // Third-body concentrations M_t = sum_j eff_tj C_j of every third-body and
 // falloff row (src/Reaction/ReactionRate.C:178-399 forms the same sums per
-// row), one row per lane, two rows per lane pass; L.C must be complete.
+// row); L.C must be complete.  Passes of G rows, one row per thread; a last
+// pass of at most G/2 (G/4) rows splits the species over 2 (4) lane groups
+// of each wavefront, each summing H of them, and adds the groups' sums by
+// xor shuffles, so it issues 1/2 (1/4) of the efficiency loads.  H is a
+// multiple of 8 and S H <= SW, so every index stays inside the zero-padded
+// table (nTBpad rows, SW species) and C[0..SW-1] (zero beyond ns, 1.0 in
+// the workgroup form's ONE slot, which meets a zero efficiency).
 template <int NP>
 __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 {
     constexpr int G = Cell<NP>::G;
-    const int lane = tid<NP>();
-    if constexpr (!Cell<NP>::kLarge) {
-        // wavefront form: passes of 64 rows, one row per lane; a pass of at
-        // most 32 (16) rows splits the species over 2 (4) lane groups, each
-        // summing H of them, and adds the groups' sums by xor shuffles, so
-        // it issues 1/2 (1/4) of the efficiency loads.  H is a multiple of
-        // 8 and S H <= 64 (ns <= 63): every index stays inside the
-        // zero-padded table and C[0..63], zero beyond ns.
-        for (int t0 = 0; t0 < M.nTB; t0 += 64) {
-            const int R = M.nTB - t0;   // uniform
-            const int S = R > 32 ? 1 : (R > 16 ? 2 : 4);
-            const int rows = 64 / S;
-            const int t = t0 + lane % rows;
-            const int H = round_up((M.ns + S - 1) / S, 8);
-            const int jb = (lane / rows) * H;
-            double m = 0.0;
-            int k = 0;
-            for (; k + 16 <= H; k += 16) {
-                double e[16], c[16];
+    // batches of TS species: 16 in the wavefront form, 8 in the workgroup
+    // form, whose two-workgroups-per-CU build has half the registers
+    constexpr int TS = Cell<NP>::kLarge ? 8 : 16;
+    const int lane = lane_id(), w = tid<NP>() >> 6;
+    for (int t0 = 0; t0 < M.nTB; t0 += G) {
+        const int R = M.nTB - t0;   // uniform
+        const int S = R > G / 2 ? 1 : (R > G / 4 ? 2 : 4);
+        const int rows = 64 / S;    // rows per wavefront
+        const int t = t0 + w * rows + lane % rows;
+        const int H = round_up((M.ns + S - 1) / S, 8);
+        const int jb = (lane / rows) * H;
+        double m = 0.0;
+        int k = 0;
+        for (; k + TS <= H; k += TS) {
+            // unconditional loads, all 2 TS of a batch in flight at once
+            double e[TS], c[TS];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    c[u] = L.C[jb + k + u];
-                    e[u] = M.effT[(size_t)(jb + k + u) * M.nTBpad + t];
-                }
-#pragma unroll
-                for (int u = 0; u < 16; ++u) m += e[u] * c[u];
+            for (int u = 0; u < TS; ++u) {
+                c[u] = L.C[jb + k + u];
+                e[u] = M.effT[(size_t)(jb + k + u) * M.nTBpad + t];
             }
+#pragma unroll
+            for (int u = 0; u < TS; ++u) m += e[u] * c[u];   // no species test: padding adds 0 * c
+        }
+        if constexpr (TS > 8) {
             if (k < H) {   // H - k == 8
                 double e[8], c[8];
 #pragma unroll
@@ -778,37 +782,10 @@ __device__ inline void third_body_sums(const DevMech& M, const Lds& L)
 #pragma unroll
                 for (int u = 0; u < 8; ++u) m += e[u] * c[u];
             }
-            if (S >= 4) m += __shfl(m, lane ^ 16, 64);
-            if (S >= 2) m += __shfl(m, lane ^ 32, 64);
-            if (lane < rows && t < M.nTB) L.Mtb[t] = m;
         }
-        return;
-    }
-    // workgroup form: batches of 8 species (the two-workgroups-per-CU build
-    // has half the registers of the wavefront form)
-    constexpr int TS = 8;
-    for (int t0 = 0; t0 < M.nTB; t0 += 2 * G) {
-        const int ta = t0 + lane, tb = t0 + G + lane;
-        const bool ina = ta < M.nTB, inb = tb < M.nTB;
-        double ma = 0.0, mb = 0.0;
-        for (int j0 = 0; j0 < M.ns; j0 += TS) {
-            // unconditional loads (the table is zero-padded to SW species and
-            // nTBpad rows, C is zero beyond ns), so all 3 TS are in flight at once
-            double ea[TS], eb[TS], c[TS];
-#pragma unroll
-            for (int u = 0; u < TS; ++u) {
-                c[u] = L.C[j0 + u];
-                ea[u] = M.effT[(size_t)(j0 + u) * M.nTBpad + ta];
-                eb[u] = M.effT[(size_t)(j0 + u) * M.nTBpad + tb];
-            }
-#pragma unroll
-            for (int u = 0; u < TS; ++u) {   // no species test: padding adds 0 * 0
-                ma += ea[u] * c[u];
-                mb += eb[u] * c[u];
-            }
-        }
-        if (ina) L.Mtb[ta] = ma;
-        if (inb) L.Mtb[tb] = mb;
+        if (S >= 4) m += __shfl(m, lane ^ 16, 64);
+        if (S >= 2) m += __shfl(m, lane ^ 32, 64);
+        if (lane < rows && t < M.nTB) L.Mtb[t] = m;
     }
 }
 

@@ -147,9 +147,15 @@ def test_solve_config1_sample(fcslib, mech1, solver):
     field = M.config_field(1, mech1).subset(np.random.default_rng(1).choice(65536, 192, replace=False))
     dt = M.CONFIG_DT[1]
     g = fcslib.Solver(fcslib.Mech(mech1), solver=solver).solve(field, dt)
-    o = Oracle(mech1).solve(field, dt, threads=1, solver=solver)
+    orc = Oracle(mech1)
+    o = orc.solve(field, dt, threads=1, solver=solver)
     assert g["rc"] == 0 and o["rc"] == 0
-    assert_solve_parity(g, o, field, dt)
+    # deltaTMin follows the smallest suggested step, which can jump between
+    # discrete values with round-off (OptSeulex on this sample: cell 22's
+    # 1.108e-8 s or cell 2's 1.347e-8 s, the oracle itself giving either
+    # under 1e-14 relative perturbations of Y): the rerun accepts a value
+    # the oracle reaches so
+    assert_solve_parity(g, o, field, dt, rerun=lambda f2: orc.solve(f2, dt, threads=8, solver=solver))
 
 
 def test_full_size_invariants_and_determinism(fcslib, mech1):
