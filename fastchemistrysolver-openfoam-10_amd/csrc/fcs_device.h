@@ -108,13 +108,15 @@ struct DevMech {
     gptr<const double> Tc;         // [SW]
     gptr<const double> clo;        // [7*SW] coefficient-major: c[k*SW + s]
     gptr<const double> chi;        // [7*SW]
-    gptr<const int> rtype;         // [nr]
-    gptr<const int> rrev;          // [nr]  0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius in rA0/rbeta0/rTa0,
-                              //       3 kf/Kc over a product side of more than 3 slots (gen row rgen[r])
+    // [nr] one word per reaction, one load in a rate round: third-body row
+    // + 1 (0: none) in bits 0-23, rate form (RXN_*) in bits 24-27, reverse
+    // kind in bits 28-31: 0 irreversible, 1 kf/Kc, 2 own reverse Arrhenius
+    // in rA0/rbeta0/rTa0, 3 kf/Kc over a product side of more than 3 slots
+    // (gen row rgen[r])
+    gptr<const int> rmeta;
     gptr<const int> rnl;           // [nr]
     gptr<const int> rnr;           // [nr]
     gptr<const int> rsp;           // [6*nr] slot-major: rsp[q*nr + r], q<3 lhs, q>=3 rhs
-    gptr<const int> rtb;           // [nr]  eff row or -1
     gptr<const int> raux;          // [nr]  troe / plog row
     gptr<const double> rA;         // [nr]  forward (k_inf)
     gptr<const double> rbeta;
@@ -550,9 +552,10 @@ __device__ __forceinline__ RxnData load_rxn(const DevMech& M, const Lds& L, int 
 {
     RxnData d{RXN_ARR, 0, -1, 0ull, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (on) {
-        d.t = M.rtype[r];
-        d.rev = M.rrev[r];
-        d.tb = M.rtb[r];
+        const unsigned m = (unsigned)M.rmeta[r];
+        d.t = (int)((m >> 24) & 15u);
+        d.rev = (int)(m >> 28);
+        d.tb = (int)(m & 0xffffffu) - 1;
         d.pk = M.rpack[r];
         d.A = M.rA[r];
         d.b = M.rbeta[r];

@@ -242,8 +242,14 @@ int build_mech_tables(const fcs_mech_desc* d, int NP, int SW, Place&& place, Dev
     auto vd = [&](const std::vector<double>& v) { return add(v.data(), v.size() * sizeof(double)); };
     auto vi = [&](const std::vector<int>& v) { return add(v.data(), v.size() * sizeof(int)); };
     const size_t iW = vd(W), iInvW = vd(invW), iTc = vd(Tc), iClo = vd(clo), iChi = vd(chi), iHf = vd(Hf);
-    const size_t iType = vi(rtype), iRev = vi(rrev), iNl = vi(rnl), iNr = vi(rnr), iSp = vi(rsp), iTb = vi(rtb),
-                 iAux = vi(raux);
+    // rate form, reverse kind and third-body row of a reaction in one word
+    // (DevMech::rmeta)
+    std::vector<int> rmeta(std::max(nr, 1), 0);
+    for (int r = 0; r < nr; ++r) {
+        if (rtb[r] + 1 >= (1 << 24)) return FCS_E_ARG;
+        rmeta[r] = (int)((unsigned)(rtb[r] + 1) | ((unsigned)rtype[r] << 24) | ((unsigned)rrev[r] << 28));
+    }
+    const size_t iMeta = vi(rmeta), iNl = vi(rnl), iNr = vi(rnr), iSp = vi(rsp), iAux = vi(raux);
     const size_t iA = vd(rA), iB = vd(rbeta), iTa = vd(rTa), iA0 = vd(rA0), iB0 = vd(rbeta0), iTa0 = vd(rTa0);
     const size_t iTroe = vd(troe), iEff = vd(eff), iEffT = vd(effT), iRtroe = vd(rtroe);
     const size_t iPn = vi(plogN), iPp = vd(plogLnp), iPA = vd(plogLnA), iPb = vd(plogBeta), iPT = vd(plogTa);
@@ -375,12 +381,10 @@ int build_mech_tables(const fcs_mech_desc* d, int NP, int SW, Place&& place, Dev
     dm.clo = (const double*)P(iClo);
     dm.chi = (const double*)P(iChi);
     dm.Hf = (const double*)P(iHf);
-    dm.rtype = (const int*)P(iType);
-    dm.rrev = (const int*)P(iRev);
     dm.rnl = (const int*)P(iNl);
     dm.rnr = (const int*)P(iNr);
     dm.rsp = (const int*)P(iSp);
-    dm.rtb = (const int*)P(iTb);
+    dm.rmeta = (const int*)P(iMeta);
     dm.raux = (const int*)P(iAux);
     dm.rA = (const double*)P(iA);
     dm.rbeta = (const double*)P(iB);
