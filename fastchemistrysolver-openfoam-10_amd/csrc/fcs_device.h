@@ -950,7 +950,7 @@ __device__ inline void reaction_round(const DevMech& M, const Lds& L, double T, 
     constexpr int G = Cell<NP>::G;
     const double PbyRT = PSTD / (RU * T);
     // reactions from the last one down: the tables order them by rate form,
-    // falloff forms last, so the costliest forms share the first slot and
+    // falloff, PLOG and SRI forms last, so the costliest forms share the first slot and
     // the partly filled slot of the last round holds the cheapest ones (a
     // form's code runs once per slot in which any lane holds it)
     for (int k = tid<NP>(); k < M.nr; k += 2 * G) {

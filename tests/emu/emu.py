@@ -141,3 +141,12 @@ def selftest(kind, timeout=5.0):
     msg = C.create_string_buffer(512)
     rc = L.emu_selftest(C.c_int(kind), C.c_double(timeout), C.byref(checks), C.byref(viol), msg, C.c_int(512))
     return dict(rc=rc, checks=checks.value, violations=viol.value, message=msg.value.decode())
+
+
+def tables(mech, NP):
+    """Return code of the library's table build (fcs_mech_tables.h) for the
+    padded size NP."""
+    from fcs.abi import make_desc
+    L = C.CDLL(build())
+    desc, keep = make_desc(mech.arrays())
+    return L.emu_tables(C.byref(desc), C.c_int(NP))

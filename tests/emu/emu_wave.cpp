@@ -224,6 +224,20 @@ static int pick_np(int ns)
     return 0;
 }
 
+// The library's table build alone, for a padded size NP of either form
+// (host images only): its return code.
+extern "C" int emu_tables(const fcs_mech_desc* d, int NP)
+{
+    std::vector<unsigned char> image;
+    DevMech dm{};
+    std::vector<int> perm;
+    const int SW = NP > 64 ? (NP <= 128 ? 128 : 256) : 64;
+    return build_mech_tables(d, NP, SW, [&](const std::vector<unsigned char>& host) -> void* {
+        image = host;
+        return image.data();
+    }, dm, perm);
+}
+
 // RHS of nS states (YT [ns+1][nS], clipped in place) through k_derivatives.
 extern "C" int emu_derivatives(const fcs_mech_desc* d, int nS, const double* p, double* YT, double* out,
                                long long* checks, long long* violations, char* msg, int msgLen)
